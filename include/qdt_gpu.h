@@ -1,0 +1,63 @@
+/*
+ * qdt_gpu.h -- GPU-specific entry points beside the drop-in qdt.h.
+ *
+ * None of these exist in the reference; they are for callers that keep their
+ * points in GPU memory, that want the results in bulk, or that need the
+ * pipeline's counters.  Plain C types only: device pointers are passed as
+ * ordinary pointers to memory of the current HIP device.
+ */
+#ifndef QDT_MI355X_QDT_GPU_H
+#define QDT_MI355X_QDT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "qdt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Counters of the last batch call on a context. */
+typedef struct qdt_gpu_stats
+{
+    uint64_t orient_calls;        /* orient2d evaluations in leaves and merges */
+    uint64_t orient_exact;        /* ... that the FP64 filter could not decide */
+    uint64_t incircle_calls;
+    uint64_t incircle_exact;
+    uint64_t extract_exact;       /* face tests queued for the exact kernel */
+    uint64_t unique_points;
+    uint64_t triangles;
+    uint64_t kd_levels;           /* levels of the median-cut tree */
+    double device_ms;             /* time of the device pipeline (events) */
+} qdt_gpu_stats;
+
+/* Like qdt_execute(), but d_pts points to n points in device memory.  The
+ * triangles stay on the device (see qdt_gpu_device_triangles); the host-side
+ * copies that the query functions read are made on the first query.  The
+ * same checks and status codes as qdt_execute() apply.  The points are
+ * copied to the host only when qdt_get_point() is first called. */
+QDT_API qdt_status qdt_gpu_execute_device(qdt_t* ptr, const qdt_vec2d* d_pts, size_t n);
+
+/* Device pointer to the 3 * T uint32 vertex indices of the last batch call,
+ * valid until the next call on the context; returns T. */
+QDT_API size_t qdt_gpu_device_triangles(const qdt_t* ptr, const uint32_t** d_tris);
+
+/* Copies all triangles (3 * T uint32, counter-clockwise) to host memory out,
+ * which must hold 3 * qdt_triangles_size() entries; returns T. */
+QDT_API size_t qdt_gpu_copy_triangles(const qdt_t* ptr, uint32_t* out);
+
+/* Counters of the last batch call; returns 1, or 0 for a NULL argument. */
+QDT_API int qdt_gpu_get_stats(const qdt_t* ptr, qdt_gpu_stats* out);
+
+/* Number of HIP devices visible to the library (0 without a GPU). */
+QDT_API int qdt_gpu_device_count(void);
+
+/* Frees the device workspace a context keeps between calls. */
+QDT_API void qdt_gpu_release_workspace(qdt_t* ptr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,235 @@
+"""libqdt_amd -- MI355X (gfx950) batch Delaunay triangulation behind libqdt's C API.
+
+The product is the shared library libqdt_amd/_lib/libqdt_amd.so, which
+exports the fourteen functions of include/qdt.h (libqdt's public interface)
+plus the GPU extensions of include/qdt_gpu.h.  This module loads it with
+ctypes and mirrors the reference's context interface (libqdt/qdt.h) for
+Python callers, tests and the bench.  There is no CPU fallback: without the
+built library, or without a GPU, triangulation calls fail loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libqdt_amd.so")
+
+# qdt_status values (include/qdt.h, same as libqdt/qdt.h:79-93)
+QDT_OK = 0
+QDT_ERR_NULL_CONTEXT = -1
+QDT_ERR_NULL_ARGUMENT = -2
+QDT_ERR_TOO_MANY_POINTS = -3
+QDT_ERR_NOT_FINITE = -4
+QDT_ERR_INVALID_INDEX = -5
+QDT_ERR_SAME_POINT = -6
+QDT_ERR_NO_TRIANGLES = -7
+QDT_ERR_CROSSES_CONSTRAINT = -8
+QDT_ERR_THROUGH_POINT = -9
+QDT_ERR_OUT_OF_MEMORY = -10
+QDT_ERR_INTERNAL = -11
+
+EXPORTS = [
+    "qdt_status_string", "qdt_create", "qdt_execute", "qdt_execute_parallel", "qdt_insert_point",
+    "qdt_insert_point_parallel", "qdt_insert_constraint", "qdt_insert_constraints_batch",
+    "qdt_triangles_size", "qdt_get_triangle", "qdt_points_size", "qdt_get_point", "qdt_point_is_used",
+    "qdt_destroy",
+]
+GPU_EXPORTS = [
+    "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
+    "qdt_gpu_device_count", "qdt_gpu_release_workspace",
+]
+
+
+class Vec2(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double)]
+
+
+class Edge(ctypes.Structure):
+    _fields_ = [("v0", ctypes.c_size_t), ("v1", ctypes.c_size_t)]
+
+
+class GpuStats(ctypes.Structure):
+    _fields_ = [
+        ("orient_calls", ctypes.c_uint64), ("orient_exact", ctypes.c_uint64),
+        ("incircle_calls", ctypes.c_uint64), ("incircle_exact", ctypes.c_uint64),
+        ("extract_exact", ctypes.c_uint64), ("unique_points", ctypes.c_uint64),
+        ("triangles", ctypes.c_uint64), ("kd_levels", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libqdt_amd.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, st = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+        L.qdt_status_string.restype = ctypes.c_char_p
+        L.qdt_status_string.argtypes = [st]
+        L.qdt_create.restype = vp
+        L.qdt_create.argtypes = []
+        L.qdt_destroy.restype = None
+        L.qdt_destroy.argtypes = [vp]
+        for f in ("qdt_execute", "qdt_execute_parallel"):
+            getattr(L, f).restype = st
+            getattr(L, f).argtypes = [vp, vp, sz]
+        for f in ("qdt_insert_point", "qdt_insert_point_parallel"):
+            getattr(L, f).restype = st
+            getattr(L, f).argtypes = [vp, Vec2, ctypes.POINTER(sz)]
+        L.qdt_insert_constraint.restype = st
+        L.qdt_insert_constraint.argtypes = [vp, sz, sz]
+        L.qdt_insert_constraints_batch.restype = st
+        L.qdt_insert_constraints_batch.argtypes = [vp, vp, sz, vp]
+        L.qdt_triangles_size.restype = sz
+        L.qdt_triangles_size.argtypes = [vp]
+        L.qdt_get_triangle.restype = sz
+        L.qdt_get_triangle.argtypes = [vp, ctypes.POINTER(sz), sz]
+        L.qdt_points_size.restype = sz
+        L.qdt_points_size.argtypes = [vp]
+        L.qdt_get_point.restype = sz
+        L.qdt_get_point.argtypes = [vp, ctypes.POINTER(Vec2), sz]
+        L.qdt_point_is_used.restype = sz
+        L.qdt_point_is_used.argtypes = [vp, sz]
+        L.qdt_gpu_execute_device.restype = st
+        L.qdt_gpu_execute_device.argtypes = [vp, vp, sz]
+        L.qdt_gpu_device_triangles.restype = sz
+        L.qdt_gpu_device_triangles.argtypes = [vp, ctypes.POINTER(vp)]
+        L.qdt_gpu_copy_triangles.restype = sz
+        L.qdt_gpu_copy_triangles.argtypes = [vp, vp]
+        L.qdt_gpu_get_stats.restype = ctypes.c_int
+        L.qdt_gpu_get_stats.argtypes = [vp, ctypes.POINTER(GpuStats)]
+        L.qdt_gpu_device_count.restype = ctypes.c_int
+        L.qdt_gpu_device_count.argtypes = []
+        L.qdt_gpu_release_workspace.restype = None
+        L.qdt_gpu_release_workspace.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def status_string(status):
+    return lib().qdt_status_string(int(status)).decode()
+
+
+def device_count():
+    return int(lib().qdt_gpu_device_count())
+
+
+def _as_points(pts):
+    a = np.ascontiguousarray(pts, dtype=np.float64)
+    if a.size == 0:
+        return a.reshape(0, 2)
+    return a.reshape(-1, 2)
+
+
+class Triangulation:
+    """One qdt_t context (libqdt/qdt.h:125): batch execute, then queries."""
+
+    def __init__(self):
+        self._h = lib().qdt_create()
+        if not self._h:
+            raise MemoryError("qdt_create returned NULL")
+
+    def close(self):
+        if self._h:
+            lib().qdt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # --- batch (libqdt/qdt.h:160,181) -------------------------------------
+    def execute(self, pts, parallel=False):
+        a = _as_points(pts)
+        f = lib().qdt_execute_parallel if parallel else lib().qdt_execute
+        return int(f(self._h, a.ctypes.data if len(a) else None, len(a)))
+
+    def execute_parallel(self, pts):
+        return self.execute(pts, parallel=True)
+
+    def execute_device(self, dev_ptr, n):
+        """Points already in GPU memory (e.g. a torch tensor's data_ptr())."""
+        return int(lib().qdt_gpu_execute_device(self._h, ctypes.c_void_p(int(dev_ptr)), int(n)))
+
+    # --- incremental (libqdt/qdt.h:219-314) ----------------------------------
+    def insert_point(self, x, y, parallel=False):
+        idx = ctypes.c_size_t(0)
+        f = lib().qdt_insert_point_parallel if parallel else lib().qdt_insert_point
+        st = int(f(self._h, Vec2(x, y), ctypes.byref(idx)))
+        return st, idx.value
+
+    def insert_constraint(self, v0, v1):
+        return int(lib().qdt_insert_constraint(self._h, v0, v1))
+
+    # --- queries (libqdt/qdt.h:326-394) --------------------------------------
+    def triangles_size(self):
+        return int(lib().qdt_triangles_size(self._h))
+
+    def get_triangle(self, i):
+        T = (ctypes.c_size_t * 3)()
+        ok = lib().qdt_get_triangle(self._h, T, i)
+        return (T[0], T[1], T[2]) if ok else None
+
+    def points_size(self):
+        return int(lib().qdt_points_size(self._h))
+
+    def get_point(self, i):
+        p = Vec2()
+        ok = lib().qdt_get_point(self._h, ctypes.byref(p), i)
+        return (p.x, p.y) if ok else None
+
+    def point_is_used(self, i):
+        return int(lib().qdt_point_is_used(self._h, i))
+
+    # --- bulk / GPU extras (include/qdt_gpu.h) -------------------------------
+    def triangles(self):
+        """All triangles as a (T, 3) uint32 array (counter-clockwise)."""
+        t = self.triangles_size()
+        out = np.zeros((t, 3), dtype=np.uint32)
+        if t:
+            got = lib().qdt_gpu_copy_triangles(self._h, out.ctypes.data)
+            assert got == t
+        return out
+
+    def device_triangles(self):
+        """(device pointer, T) of the last batch's triangles."""
+        p = ctypes.c_void_p()
+        t = lib().qdt_gpu_device_triangles(self._h, ctypes.byref(p))
+        return (p.value or 0), int(t)
+
+    def stats(self):
+        s = GpuStats()
+        lib().qdt_gpu_get_stats(self._h, ctypes.byref(s))
+        return s.as_dict()
+
+    def release_workspace(self):
+        lib().qdt_gpu_release_workspace(self._h)
+
+
+def triangulate(pts):
+    """Convenience: (T, 3) uint32 triangles of pts on the GPU."""
+    with Triangulation() as t:
+        st = t.execute(pts)
+        if st != QDT_OK:
+            raise RuntimeError("qdt_execute failed: %s" % status_string(st))
+        return t.triangles()

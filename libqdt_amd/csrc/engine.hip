@@ -1,0 +1,728 @@
+// engine.hip -- the GPU batch Delaunay pipeline for gfx950.
+//
+// Stages (one HIP stream, no host work between them except reading m):
+//   1. keys:      order-preserving 64-bit keys of x and y, finiteness check
+//   2. sort:      stable LSD radix sorts by y then x -> (x, y, index) order
+//   3. dedupe:    runs of equal coordinates -> distinct points P0 (frame-0
+//                 order), lowest index ID0, rep[] for every input index
+//                 (libqdt/builder.hxx:356-390)
+//   4. frame 1:   stable sort of the reversed frame-0 order by y -> the
+//                 (y asc, x desc) order of builder.hxx:75
+//   5. kd levels: Dwyer's alternating median cuts (builder.hxx:303-334),
+//                 level by level: a positional split of the array sorted in
+//                 the cut's frame and a stable two-way partition of the
+//                 array sorted in the other frame, so both stay sorted
+//   6. leaves:    2- and 3-point leaves (builder.hxx:177-201)
+//   7. merges:    Guibas-Stolfi merges bottom-up, one thread per kd node,
+//                 one launch per level (builder.hxx:205-301)
+//   8. extract:   faces of the quad-edge store with a filtered orient2d;
+//                 undecided tests go through a queue to an exact kernel;
+//                 compaction in record order (builder.hxx:98-116)
+// Topology equals the reference's for every input: the same cuts, leaves,
+// merges and hull walks run with exact predicates.
+#include "engine.hpp"
+#include "primitives.hpp"
+#include "quadedge.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace qdt
+{
+
+namespace
+{
+
+constexpr int kMergeBlock = 64;
+
+#define QDT_HIP(x)                                                                                       \
+    do                                                                                                   \
+    {                                                                                                    \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) return e_ == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal; \
+    } while (0)
+
+inline uint32_t cdiv(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
+
+__device__ __forceinline__ uint64_t order_key(double v)
+{
+    // -0.0 compares equal to +0.0, so both map to the key of +0.0
+    const uint64_t b = v == 0.0 ? 0ull : (uint64_t)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// counters[]: 0 orient, 1 orient exact, 2 incircle, 3 incircle exact, 4 extract queue
+__device__ __forceinline__ void flush_counts(unsigned long long* counters, const PredCount& pc)
+{
+    unsigned long long v[4] = { pc.orient, pc.orient_exact, pc.incircle, pc.incircle_exact };
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        unsigned long long x = v[i];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+        if (__lane_id() == 0 && x) atomicAdd(&counters[i], x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1-4: keys, sorts, dedupe, frame-1 order
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_keys(const double2* pts, uint32_t n, uint64_t* kx, uint64_t* ky,
+                                                 uint32_t* idx, int* err)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double2 p = pts[i];
+    if (!isfinite(p.x) || !isfinite(p.y)) atomicOr(err, 1);
+    kx[i] = order_key(p.x);
+    ky[i] = order_key(p.y);
+    idx[i] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_u64(const uint32_t* idx, const uint64_t* src, uint32_t n,
+                                                       uint64_t* dst)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+struct DedupeFlag
+{
+    const uint64_t* kx_sorted;
+    const uint32_t* idx;
+    const uint64_t* ky;
+    __device__ uint32_t operator()(uint32_t p) const
+    {
+        if (p == 0) return 1;
+        return (kx_sorted[p] != kx_sorted[p - 1] || ky[idx[p]] != ky[idx[p - 1]]) ? 1 : 0;
+    }
+};
+
+struct LoadU32
+{
+    const uint32_t* in;
+    __device__ uint32_t operator()(uint32_t i) const { return in[i]; }
+};
+
+struct StoreU32
+{
+    uint32_t* out;
+    __device__ void operator()(uint32_t i, uint32_t v) const { out[i] = v; }
+};
+
+__global__ __launch_bounds__(kBlock) void k_dedupe_write(const double2* pts, const uint32_t* idx, DedupeFlag f,
+                                                         const uint32_t* dist, uint32_t n, double2* P0,
+                                                         uint32_t* ID0)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    if (f(p))
+    {
+        const uint32_t d = dist[p];
+        const uint32_t i = idx[p];
+        P0[d] = pts[i];
+        ID0[d] = i;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rep(const uint32_t* idx, DedupeFlag f, const uint32_t* dist,
+                                                uint32_t n, const uint32_t* ID0, uint32_t* rep)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t d = f(p) ? dist[p] : dist[p] - 1;
+    rep[idx[p]] = ID0[d];
+}
+
+__global__ __launch_bounds__(kBlock) void k_frame1_keys(const double2* P0, uint32_t m, uint64_t* keys,
+                                                        uint32_t* vals)
+{
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= m) return;
+    const uint32_t d = m - 1 - q;
+    keys[q] = order_key(P0[d].y);
+    vals[q] = d;
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(uint32_t* a, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* a, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 5: kd levels
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ size_t off_base(int d) { return ((size_t)1 << d) - 1 + (size_t)d; }
+__device__ __forceinline__ size_t node_index(int d, uint32_t j) { return ((size_t)1 << d) - 1 + j; }
+__device__ __forceinline__ int left_size(int s) { return s >= 4 ? s / 2 : s; }
+
+__global__ void k_root(int* off, uint32_t m)
+{
+    off[0] = 0;
+    off[1] = (int)m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_children(int* off, int d)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t cnt = 1u << d;
+    if (j >= cnt) return;
+    const int* od = off + off_base(d);
+    int* oc = off + off_base(d + 1);
+    const int o = od[j];
+    const int s = od[j + 1] - o;
+    oc[2 * j] = o;
+    oc[2 * j + 1] = o + left_size(s);
+    if (j == cnt - 1) oc[2 * cnt] = od[cnt];
+}
+
+// positional split of the array sorted in the cut's frame
+__global__ __launch_bounds__(kBlock) void k_split(const uint32_t* Bf, const uint32_t* node, const int* off, int d,
+                                                  uint32_t m, uint32_t* node_next, uint8_t* bits)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const int* od = off + off_base(d);
+    const uint32_t j = node[p];
+    const int o = od[j];
+    const int s = od[j + 1] - o;
+    const uint32_t c = ((int)p - o) >= left_size(s) ? 1u : 0u;
+    node_next[p] = 2 * j + c;
+    bits[Bf[p]] = (uint8_t)c;
+}
+
+struct ZeroFlag
+{
+    const uint32_t* B;
+    const uint8_t* bits;
+    __device__ uint32_t operator()(uint32_t p) const { return bits[B[p]] ? 0u : 1u; }
+};
+
+// stable two-way partition, within each node, of the array sorted in the other frame
+__global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const uint32_t* node, const int* off,
+                                                      int d, const uint8_t* bits, const uint32_t* Z, uint32_t m,
+                                                      uint32_t* Bout)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const int* od = off + off_base(d);
+    const uint32_t j = node[p];
+    const int o = od[j];
+    const int s = od[j + 1] - o;
+    const uint32_t v = Bo[p];
+    const int z = (int)(Z[p] - Z[o]);
+    const int pos = bits[v] ? o + left_size(s) + ((int)p - o - z) : o + z;
+    Bout[pos] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 6-7: leaves and merges
+// ---------------------------------------------------------------------------
+
+using GQE = QuadEdges<int*, const double2*>;
+
+__global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const double2* PT, const uint32_t* B0,
+                                                        const uint32_t* B1, const int* off, int D, uint32_t m,
+                                                        int4* res, int* err, unsigned long long* counters)
+{
+    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    GQE q{ NX, OR, PT, (int)(12 * m), (int)m, 0, {} };
+    if (j < (1u << D))
+    {
+        const int* oD = off + off_base(D);
+        const int o = oD[j];
+        const int s = oD[j + 1] - o;
+        if (s >= 2)
+        {
+            int db = D;
+            uint32_t jb = j;
+            while (db > 0)
+            {
+                const uint32_t pj = jb >> 1;
+                const int* op = off + off_base(db - 1);
+                if (op[pj + 1] - op[pj] >= 4) break;
+                --db;
+                jb = pj;
+            }
+            const int out_axis = db > 0 ? ((db - 1) & 1) : 0;
+            const uint32_t* B = out_axis ? B1 : B0;
+            int v[3];
+            for (int i = 0; i < s; ++i) v[i] = (int)B[o + i];
+            int first, second;
+            Pool pool;
+            q.leaf(v, s, 3 * o, first, second, pool);
+            res[node_index(db, jb)] = make_int4(first, second, pool.head, pool.tail);
+        }
+    }
+    flush_counts(counters, q.pc);
+    if (q.err) atomicOr(err, 2 << q.err);
+}
+
+__global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const double2* PT, const int* off, int d,
+                                                       uint32_t m, int4* res, int* err, unsigned long long* counters)
+{
+    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    GQE q{ NX, OR, PT, (int)(12 * m), (int)m, 0, {} };
+    if (j < (1u << d))
+    {
+        const int* od = off + off_base(d);
+        const int s = od[j + 1] - od[j];
+        if (s >= 4)
+        {
+            const int4 L = res[node_index(d + 1, 2 * j)];
+            const int4 R = res[node_index(d + 1, 2 * j + 1)];
+            Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
+            const int axis = d & 1;
+            const int out_axis = d > 0 ? ((d - 1) & 1) : 0;
+            int first, second;
+            q.merge_node(L.x, L.y, R.x, R.y, axis, out_axis, s, first, second, pool);
+            res[node_index(d, j)] = make_int4(first, second, pool.head, pool.tail);
+        }
+    }
+    flush_counts(counters, q.pc);
+    if (q.err) atomicOr(err, 2 << q.err);
+}
+
+// ---------------------------------------------------------------------------
+// 8: extraction
+// ---------------------------------------------------------------------------
+
+// face[k]: bit 0/1 = the face left of edge 4k / 4k+2 is an output triangle,
+// bit 2/3 = undecided by the filter (queued)
+// Face of directed edge e: its lnext 3-cycle (f, g), or false.  Indices are
+// checked so that a damaged store cannot send a load out of bounds.
+__device__ __forceinline__ bool face_of(const int* NX, const int* OR, int e, int lim_e, int lim_v, int& f, int& g,
+                                        int& a, int& b, int& c)
+{
+    f = e_rot(NX[e_invrot(e)]);
+    if ((unsigned)f >= (unsigned)lim_e) return false;
+    g = e_rot(NX[e_invrot(f)]);
+    if ((unsigned)g >= (unsigned)lim_e) return false;
+    if (e_rot(NX[e_invrot(g)]) != e) return false;
+    a = OR[e >> 1];
+    b = OR[f >> 1];
+    c = OR[g >> 1];
+    return (unsigned)a < (unsigned)lim_v && (unsigned)b < (unsigned)lim_v && (unsigned)c < (unsigned)lim_v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_extract_test(const int* NX, const int* OR, const double2* PT,
+                                                         uint32_t nrec, uint32_t m, uint32_t* face, uint32_t* queue,
+                                                         uint32_t qcap, unsigned long long* counters, int* err)
+{
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nrec) return;
+    uint32_t fl = 0;
+    if (OR[2 * k] >= 0)
+    {
+#pragma unroll
+        for (int r = 0; r < 4; r += 2)
+        {
+            const int e = 4 * (int)k + r;
+            int f, g, a, b, c;
+            if (!face_of(NX, OR, e, (int)(12 * m), (int)m, f, g, a, b, c)) continue;
+            if (e > f || e > g) continue;
+            const double2 A = PT[a], B = PT[b], C = PT[c];
+            const double detleft = (A.x - C.x) * (B.y - C.y);
+            const double detright = (A.y - C.y) * (B.x - C.x);
+            const double det = detleft - detright;
+            const double mag = fabs(detleft) + fabs(detright);
+            const double bound = kErrOrient * mag;
+            if ((det > bound || -det > bound) && mag > kFilterFloor)
+            {
+                if (det > 0) fl |= 1u << (r >> 1);
+            }
+            else
+            {
+                fl |= 4u << (r >> 1);
+                const unsigned long long slot = atomicAdd(&counters[4], 1ull);
+                if (slot < qcap)
+                    queue[slot] = (uint32_t)e;
+                else
+                    atomicOr(err, 4);
+            }
+        }
+    }
+    face[k] = fl;
+}
+
+__global__ __launch_bounds__(kBlock) void k_extract_exact(const int* NX, const int* OR, const double2* PT,
+                                                          uint32_t m, const uint32_t* queue,
+                                                          const unsigned long long* counters, uint32_t qcap,
+                                                          uint32_t* face)
+{
+    const unsigned long long cnt = counters[4] < qcap ? counters[4] : qcap;
+    for (unsigned long long t = blockIdx.x * kBlock + threadIdx.x; t < cnt; t += (unsigned long long)gridDim.x * kBlock)
+    {
+        const int e = (int)queue[t];
+        int f, g, a, b, c;
+        if (!face_of(NX, OR, e, (int)(12 * m), (int)m, f, g, a, b, c)) continue;
+        const double2 A = PT[a], B = PT[b], C = PT[c];
+        if (orient2d_exact(A.x, A.y, B.x, B.y, C.x, C.y) > 0) atomicOr(&face[e >> 2], 1u << ((e & 2) >> 1));
+    }
+}
+
+struct FaceCount
+{
+    const uint32_t* face;
+    __device__ uint32_t operator()(uint32_t k) const { return __popc(face[k] & 3u); }
+};
+
+struct FaceWrite
+{
+    const uint32_t* face;
+    const int* NX;
+    const int* OR;
+    const uint32_t* ID0;
+    uint32_t* tris;
+    uint32_t m;
+    __device__ void operator()(uint32_t k, uint32_t pre) const
+    {
+        const uint32_t fl = face[k] & 3u;
+        if (!fl) return;
+        for (int r = 0; r < 4; r += 2)
+        {
+            if (!(fl & (1u << (r >> 1)))) continue;
+            const int e = 4 * (int)k + r;
+            int f, g, a, b, c;
+            if (!face_of(NX, OR, e, (int)(12 * m), (int)m, f, g, a, b, c)) a = b = c = 0;
+            tris[3 * pre] = ID0[a];
+            tris[3 * pre + 1] = ID0[b];
+            tris[3 * pre + 2] = ID0[c];
+            ++pre;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+template <class In, class Out>
+void scan_u32(In in, uint32_t n, Out out, uint32_t* partials, hipStream_t s)
+{
+    const uint32_t nb = cdiv(n, kScanTile);
+    k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, partials);
+    k_scan_partials<<<1, 1024, 0, s>>>(partials, nb);
+    k_scan_down<<<nb, kBlock, 0, s>>>(in, n, partials, out);
+}
+
+// Kd depth at which every node has at most three points.
+int kd_depth(uint32_t m)
+{
+    std::vector<uint32_t> sizes{ m }, next;
+    int d = 0;
+    for (;;)
+    {
+        bool any = false;
+        next.clear();
+        for (uint32_t s : sizes)
+        {
+            if (s >= 4)
+            {
+                any = true;
+                next.push_back(s / 2);
+                next.push_back(s - s / 2);
+            }
+            else
+                next.push_back(s);
+        }
+        if (!any) return d;
+        std::sort(next.begin(), next.end());
+        next.erase(std::unique(next.begin(), next.end()), next.end());
+        sizes.swap(next);
+        ++d;
+    }
+}
+
+} // namespace
+
+struct Engine::Bufs
+{
+    uint64_t *kA, *kB, *kx, *ky;
+    uint32_t *vA, *vB, *dist, *ID0, *rep, *B0, *B1, *Bt, *nodeA, *nodeB, *Z, *tris, *face, *queue;
+    uint32_t *partials, *hist, *ghist;
+    double2* P0;
+    uint8_t* bits;
+    int *NX, *OR, *off;
+    int4* res;
+    unsigned long long* counters;
+    int* err;
+    uint32_t qcap;
+    int dcap;
+    uint32_t* host_scalars; // pinned
+};
+
+Engine::Engine(hipStream_t stream) : stream_(stream), own_stream_(stream == nullptr)
+{
+    if (own_stream_)
+    {
+        if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) stream_ = nullptr;
+    }
+    b_ = new Bufs();
+    std::memset(b_, 0, sizeof(Bufs));
+}
+
+Engine::~Engine()
+{
+    release();
+    if (b_ && b_->host_scalars) (void)hipHostFree(b_->host_scalars);
+    delete b_;
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::release()
+{
+    if (ws_) (void)hipFree(ws_);
+    ws_ = nullptr;
+    ws_bytes_ = 0;
+    cap_n_ = 0;
+    res_ = DeviceResult();
+}
+
+Status Engine::ensure(size_t n)
+{
+    if (!stream_) return Status::internal;
+    if (!b_->host_scalars)
+    {
+        void* h = nullptr;
+        QDT_HIP(hipHostMalloc(&h, 64, hipHostMallocDefault));
+        b_->host_scalars = (uint32_t*)h;
+    }
+    if (!ev0_)
+    {
+        QDT_HIP(hipEventCreate(&ev0_));
+        QDT_HIP(hipEventCreate(&ev1_));
+    }
+    if (n <= cap_n_ && ws_) return Status::ok;
+    release();
+    const size_t N = std::max<size_t>(n, 64);
+    const int D = kd_depth((uint32_t)N);
+    const size_t nodes = ((size_t)2 << D) + 64;
+    const size_t offs = ((size_t)2 << D) + (size_t)D + 64;
+    const uint32_t nb_sort = cdiv(N, kSortTile);
+    const uint32_t nb_scan = cdiv(3 * N, kScanTile) + 1;
+    const size_t qcap = std::max<size_t>(4096, N / 4);
+    struct Item
+    {
+        void** p;
+        size_t bytes;
+    };
+    Bufs& b = *b_;
+    const Item items[] = {
+        { (void**)&b.kA, 8 * N },       { (void**)&b.kB, 8 * N },       { (void**)&b.kx, 8 * N },
+        { (void**)&b.ky, 8 * N },       { (void**)&b.vA, 4 * N },       { (void**)&b.vB, 4 * N },
+        { (void**)&b.dist, 4 * N },     { (void**)&b.ID0, 4 * N },      { (void**)&b.rep, 4 * N },
+        { (void**)&b.B0, 4 * N },       { (void**)&b.B1, 4 * N },       { (void**)&b.Bt, 4 * N },
+        { (void**)&b.nodeA, 4 * N },    { (void**)&b.nodeB, 4 * N },    { (void**)&b.Z, 4 * N },
+        { (void**)&b.tris, 4 * 6 * N }, { (void**)&b.face, 4 * 3 * N }, { (void**)&b.queue, 4 * qcap },
+        { (void**)&b.partials, 4 * (size_t)(nb_scan + 1 + 256 * nb_sort) },
+        { (void**)&b.hist, 4 * 256 * (size_t)nb_sort },
+        { (void**)&b.ghist, 4 * 8 * 256 },
+        { (void**)&b.P0, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.NX, 4 * 12 * N },
+        { (void**)&b.OR, 4 * 6 * N },   { (void**)&b.off, 4 * offs },   { (void**)&b.res, 16 * nodes },
+        { (void**)&b.counters, 8 * 8 }, { (void**)&b.err, 64 },
+    };
+    size_t total = 0;
+    for (const Item& it : items) total += (it.bytes + 255) & ~(size_t)255;
+    void* ws = nullptr;
+    const hipError_t e = hipMalloc(&ws, total);
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal;
+    ws_ = (char*)ws;
+    ws_bytes_ = total;
+    size_t at = 0;
+    for (const Item& it : items)
+    {
+        *it.p = ws_ + at;
+        at += (it.bytes + 255) & ~(size_t)255;
+    }
+    b.qcap = (uint32_t)qcap;
+    b.dcap = D;
+    cap_n_ = N;
+    return Status::ok;
+}
+
+// Sorts (k, v) pairs stably by key; the result ends in kA/vA.
+static Status sort_pairs(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t* hist,
+                         uint32_t* partials, uint32_t* ghist, uint32_t* host_ghist, hipStream_t s)
+{
+    if (n <= 1) return Status::ok;
+    QDT_HIP(hipMemsetAsync(ghist, 0, 4 * 8 * 256, s));
+    k_radix_global_hist<<<std::min<uint32_t>(cdiv(n, kBlock), 1024), kBlock, 0, s>>>(kA, n, ghist);
+    QDT_HIP(hipMemcpyAsync(host_ghist, ghist, 4 * 8 * 256, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    const uint32_t nb = cdiv(n, kSortTile);
+    for (int pass = 0; pass < 8; ++pass)
+    {
+        bool trivial = false;
+        for (int dgt = 0; dgt < 256; ++dgt)
+            if (host_ghist[pass * 256 + dgt] == n) trivial = true;
+        if (trivial) continue;
+        const int shift = 8 * pass;
+        k_radix_hist<<<nb, kBlock, 0, s>>>(kA, n, shift, hist);
+        scan_u32(LoadU32{ hist }, 256 * nb, StoreU32{ hist }, partials, s);
+        k_radix_scatter<<<nb, kBlock, 0, s>>>(kA, vA, kB, vB, n, shift, hist);
+        std::swap(kA, kB);
+        std::swap(vA, vB);
+    }
+    return Status::ok;
+}
+
+Status Engine::run(const double* d_pts, size_t n)
+{
+    res_ = DeviceResult();
+    st_ = RunStats();
+    if (n == 0) return Status::ok;
+    Status s = ensure(n);
+    if (s != Status::ok) return s;
+    s = run_impl(d_pts, n);
+    if (s != Status::ok) res_ = DeviceResult();
+    return s;
+}
+
+Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
+{
+    Bufs& b = *b_;
+    hipStream_t s = stream_;
+    const uint32_t n = (uint32_t)n_sz;
+    const double2* pts = (const double2*)d_pts_raw;
+    uint32_t* hs = b.host_scalars;
+    uint32_t* hghist = nullptr;
+    std::vector<uint32_t> ghist_host(8 * 256);
+    hghist = ghist_host.data();
+
+    QDT_HIP(hipEventRecord(ev0_, s));
+    QDT_HIP(hipMemsetAsync(b.counters, 0, 8 * 8, s));
+    QDT_HIP(hipMemsetAsync(b.err, 0, 64, s));
+
+    // 1. keys; the first sort orders by y and carries the input index
+    k_keys<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, n, b.kx, b.kA, b.vA, b.err);
+    uint64_t *kA = b.kA, *kB = b.kB;
+    uint32_t *vA = b.vA, *vB = b.vB;
+    // y keys by input index are needed again for the equality test
+    QDT_HIP(hipMemcpyAsync(b.ky, b.kA, 8 * (size_t)n, hipMemcpyDeviceToDevice, s));
+    Status st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
+    if (st != Status::ok) return st;
+    // non-finite input is reported before any further work
+    QDT_HIP(hipMemcpyAsync(hs, b.err, 4, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    if (hs[0] & 1) return Status::not_finite;
+    // 2. stable sort by x -> (x, y, index)
+    k_gather_u64<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.kx, n, kA);
+    st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
+    if (st != Status::ok) return st;
+
+    // 3. dedupe
+    DedupeFlag df{ kA, vA, b.ky };
+    scan_u32(df, n, StoreU32{ b.dist }, b.partials, s);
+    const uint32_t nbs = cdiv(n, kScanTile);
+    QDT_HIP(hipMemcpyAsync(hs, b.partials + nbs, 4, hipMemcpyDeviceToHost, s));
+    k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, vA, df, b.dist, n, b.P0, b.ID0);
+    k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, df, b.dist, n, b.ID0, b.rep);
+    QDT_HIP(hipStreamSynchronize(s));
+    const uint32_t m = hs[0];
+    res_.n = n;
+    res_.rep = b.rep;
+    res_.unique = m;
+    st_.unique = m;
+    if (m < 3)
+    {
+        QDT_HIP(hipEventRecord(ev1_, s));
+        QDT_HIP(hipEventSynchronize(ev1_));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+        st_.device_ms = ms;
+        return Status::ok;
+    }
+
+    // 4. frame-1 order: (y asc, x desc) = stable y-sort of the reversed frame-0 order
+    k_frame1_keys<<<cdiv(m, kBlock), kBlock, 0, s>>>(b.P0, m, b.kA, b.vA);
+    kA = b.kA;
+    kB = b.kB;
+    vA = b.vA;
+    vB = b.vB;
+    st = sort_pairs(kA, vA, kB, vB, m, b.hist, b.partials, b.ghist, hghist, s);
+    if (st != Status::ok) return st;
+    uint32_t* B1 = vA;
+    uint32_t* B1spare = vB;
+    uint32_t* B0 = b.B0;
+    k_iota<<<cdiv(m, kBlock), kBlock, 0, s>>>(B0, m);
+
+    // 5. kd levels
+    const int D = kd_depth(m);
+    if (D > b.dcap) return Status::internal;
+    st_.levels = (uint32_t)D;
+    k_root<<<1, 1, 0, s>>>(b.off, m);
+    QDT_HIP(hipMemsetAsync(b.res, 0xff, 16 * (((size_t)2 << D) - 1), s));
+    QDT_HIP(hipMemsetAsync(b.nodeA, 0, 4 * (size_t)m, s));
+    uint32_t *nodeA = b.nodeA, *nodeB = b.nodeB;
+    uint32_t* Bt = b.Bt;
+    for (int d = 0; d < D; ++d)
+    {
+        const uint32_t cnt = 1u << d;
+        k_children<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(b.off, d);
+        uint32_t*& Bf = (d & 1) ? B1 : B0;
+        uint32_t*& Bo = (d & 1) ? B0 : B1;
+        k_split<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bf, nodeA, b.off, d, m, nodeB, b.bits);
+        scan_u32(ZeroFlag{ Bo, b.bits }, m, StoreU32{ b.Z }, b.partials, s);
+        k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, nodeA, b.off, d, b.bits, b.Z, m, Bt);
+        std::swap(Bo, Bt);
+        std::swap(nodeA, nodeB);
+    }
+    (void)B1spare;
+
+    // 6. leaves, 7. merges
+    {
+        const uint32_t cnt = 1u << D;
+        k_leaves<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, B0, B1, b.off, D, m, b.res,
+                                                                 b.err, b.counters);
+    }
+    for (int d = D - 1; d >= 0; --d)
+    {
+        const uint32_t cnt = 1u << d;
+        k_merge<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.off, d, m, b.res, b.err,
+                                                                b.counters);
+    }
+
+    // 8. extraction
+    const uint32_t nrec = 3 * m;
+    k_extract_test<<<cdiv(nrec, kBlock), kBlock, 0, s>>>(b.NX, b.OR, b.P0, nrec, m, b.face, b.queue, b.qcap,
+                                                          b.counters, b.err);
+    k_extract_exact<<<256, kBlock, 0, s>>>(b.NX, b.OR, b.P0, m, b.queue, b.counters, b.qcap, b.face);
+    scan_u32(FaceCount{ b.face }, nrec, FaceWrite{ b.face, b.NX, b.OR, b.ID0, b.tris, m }, b.partials, s);
+    QDT_HIP(hipEventRecord(ev1_, s));
+    const uint32_t nbx = cdiv(nrec, kScanTile);
+    QDT_HIP(hipMemcpyAsync(hs, b.partials + nbx, 4, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipMemcpyAsync(hs + 1, b.err, 4, hipMemcpyDeviceToHost, s));
+    unsigned long long cnts[8];
+    QDT_HIP(hipMemcpyAsync(cnts, b.counters, 8 * 8, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    if (hs[1] & ~1) return Status::internal;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+    st_.device_ms = ms;
+    st_.orient = cnts[0];
+    st_.orient_exact = cnts[1];
+    st_.incircle = cnts[2];
+    st_.incircle_exact = cnts[3];
+    st_.extract_queue = cnts[4];
+    st_.triangles = hs[0];
+    res_.tris = b.tris;
+    res_.triangles = hs[0];
+    return Status::ok;
+}
+
+} // namespace qdt

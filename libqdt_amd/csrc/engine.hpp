@@ -1,0 +1,86 @@
+// engine.hpp -- host-side interface of the GPU batch Delaunay engine.
+//
+// One Engine owns a HIP stream and a device workspace; it triangulates one
+// point set at a time (qdt_execute semantics) from device-resident input and
+// leaves the results on the device.  The C ABI (qdt_capi.cpp) drives it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace qdt
+{
+
+// Counters of one run, for reports and tests.
+struct RunStats
+{
+    uint64_t orient, orient_exact, incircle, incircle_exact; // predicate calls / filter failures
+    uint64_t extract_queue;                                   // extraction tests sent to the exact kernel
+    uint32_t unique;                                          // distinct points
+    uint32_t triangles;
+    uint32_t levels;      // kd levels with merges
+    uint32_t leaf_levels; // levels handled inside LDS leaf blocks
+    uint32_t block_level; // first level of the LDS blocks (0 = whole set)
+    double device_ms;     // first kernel start .. last kernel end
+};
+
+enum class Status : int
+{
+    ok = 0,
+    not_finite = -4,
+    out_of_memory = -10,
+    internal = -11,
+};
+
+struct DeviceResult
+{
+    const uint32_t* tris = nullptr; // 3 * triangles original indices
+    const uint32_t* rep = nullptr;  // n entries
+    uint32_t triangles = 0;
+    uint32_t unique = 0;
+    size_t n = 0;
+};
+
+class Engine
+{
+public:
+    explicit Engine(hipStream_t stream = nullptr);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Triangulates n points at d_pts (device memory, (x, y) doubles).
+    Status run(const double* d_pts, size_t n);
+    const DeviceResult& result() const { return res_; }
+    const RunStats& stats() const { return st_; }
+    hipStream_t stream() const { return stream_; }
+    // Frees the workspace (the next run allocates again).
+    void release();
+    // Tests only: cap the LDS leaf-block size (0 = default).
+    void set_block_points(int c) { block_points_ = c; }
+    // Device bytes currently held.
+    size_t workspace_bytes() const { return ws_bytes_; }
+
+private:
+    Status ensure(size_t n);
+    Status run_impl(const double* d_pts, size_t n);
+
+    hipStream_t stream_;
+    bool own_stream_;
+    char* ws_ = nullptr;
+    size_t ws_bytes_ = 0;
+    size_t cap_n_ = 0;
+    int block_points_ = 0;
+    DeviceResult res_;
+    RunStats st_{};
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    // carved buffers (see ensure())
+    struct Bufs;
+    Bufs* b_ = nullptr;
+};
+
+} // namespace qdt

@@ -1,0 +1,648 @@
+// qdt_capi.cpp -- the qdt.h C ABI over the GPU engine.
+//
+// Contract (libqdt/qdt.cpp): input checks in the order TOO_MANY_POINTS ->
+// NULL_ARGUMENT -> NOT_FINITE (qdt.cpp:29-43); a rejected, out-of-memory or
+// internally failed batch leaves the context empty (qdt.cpp:56-59, 88-113);
+// no C++ exception crosses the ABI; queries are O(1) reads of host memory
+// (qdt.cpp:246-298).  The host store keeps triangles as uint32 (indices are
+// below 1e8) and widens them on read.
+#include "../../include/qdt.h"
+#include "../../include/qdt_gpu.h"
+#include "engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <set>
+#include <thread>
+#include <utility>
+#include <vector>
+
+namespace
+{
+
+constexpr size_t kMaxPoints = 100000000; // libqdt/wrapper.hxx:15
+
+// Growable pinned host buffer (fast DMA for H2D/D2H).
+template <class T>
+struct Pinned
+{
+    T* p = nullptr;
+    size_t cap = 0;
+    ~Pinned() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    // grows to at least n elements, keeping the first `keep` ones
+    bool grow_keep(size_t n, size_t keep)
+    {
+        if (n <= cap) return true;
+        size_t c = std::max<size_t>(n, 2 * cap);
+        void* q = nullptr;
+        if (hipHostMalloc(&q, c * sizeof(T), hipHostMallocDefault) != hipSuccess) return false;
+        if (keep) std::memcpy(q, p, keep * sizeof(T));
+        reset();
+        p = (T*)q;
+        cap = c;
+        return true;
+    }
+    bool reserve(size_t n)
+    {
+        if (n <= cap) return true;
+        reset();
+        size_t c = std::max<size_t>(n, 1024);
+        void* q = nullptr;
+        if (hipHostMalloc(&q, c * sizeof(T), hipHostMallocDefault) != hipSuccess) return false;
+        p = (T*)q;
+        cap = c;
+        return true;
+    }
+};
+
+} // namespace
+
+struct qdt_t
+{
+    // host store
+    Pinned<qdt_vec2d> pts;
+    size_t npts = 0;
+    Pinned<uint32_t> tris; // 3 per triangle
+    size_t ntris = 0;
+    Pinned<uint32_t> rep;
+    size_t nrep = 0;
+    // device side
+    qdt::Engine* eng = nullptr;
+    double* d_in = nullptr;
+    size_t d_in_cap = 0;
+    // lazy host copies after qdt_gpu_execute_device
+    bool tris_on_device_only = false;
+    bool pts_on_device_only = false;
+    qdt_gpu_stats stats{};
+    // constraints accepted so far (pairs of representative indices)
+    std::set<std::pair<size_t, size_t>> cons;
+
+    ~qdt_t()
+    {
+        delete eng;
+        if (d_in) (void)hipFree(d_in);
+    }
+};
+
+namespace
+{
+
+void reset_context(qdt_t* Q) noexcept
+{
+    Q->npts = 0;
+    Q->ntris = 0;
+    Q->nrep = 0;
+    Q->tris_on_device_only = false;
+    Q->pts_on_device_only = false;
+    Q->stats = qdt_gpu_stats{};
+    Q->cons.clear();
+}
+
+bool finite_pt(const qdt_vec2d& p) { return std::isfinite(p.x) && std::isfinite(p.y); }
+
+// Copies pts into dst and reports whether all coordinates are finite; large
+// inputs are split over host threads.
+bool copy_and_check(const qdt_vec2d* src, size_t n, qdt_vec2d* dst)
+{
+    const size_t kChunk = 1 << 20;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::min<unsigned>(std::max(1u, nt), 16u);
+    if (n < 2 * kChunk || nt == 1)
+    {
+        bool ok = true;
+        for (size_t i = 0; i < n; ++i)
+        {
+            dst[i] = src[i];
+            ok &= finite_pt(src[i]);
+        }
+        return ok;
+    }
+    std::atomic<bool> ok{ true };
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    try
+    {
+        for (unsigned t = 0; t < nt; ++t)
+        {
+            const size_t a = t * per, b = std::min(n, a + per);
+            if (a >= b) break;
+            th.emplace_back([=, &ok]() {
+                bool good = true;
+                for (size_t i = a; i < b; ++i)
+                {
+                    dst[i] = src[i];
+                    good &= finite_pt(src[i]);
+                }
+                if (!good) ok = false;
+            });
+        }
+    }
+    catch (...)
+    {
+        for (auto& t : th) t.join();
+        bool good = true;
+        for (size_t i = 0; i < n; ++i)
+        {
+            dst[i] = src[i];
+            good &= finite_pt(src[i]);
+        }
+        return good;
+    }
+    for (auto& t : th) t.join();
+    return ok;
+}
+
+qdt_status map_status(qdt::Status s)
+{
+    switch (s)
+    {
+    case qdt::Status::ok: return QDT_OK;
+    case qdt::Status::not_finite: return QDT_ERR_NOT_FINITE;
+    case qdt::Status::out_of_memory: return QDT_ERR_OUT_OF_MEMORY;
+    default: return QDT_ERR_INTERNAL;
+    }
+}
+
+qdt_status ensure_engine(qdt_t* Q)
+{
+    if (Q->eng) return QDT_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return QDT_ERR_INTERNAL;
+    Q->eng = new (std::nothrow) qdt::Engine();
+    if (!Q->eng) return QDT_ERR_OUT_OF_MEMORY;
+    if (!Q->eng->stream())
+    {
+        delete Q->eng;
+        Q->eng = nullptr;
+        return QDT_ERR_INTERNAL;
+    }
+    return QDT_OK;
+}
+
+qdt_status ensure_device_input(qdt_t* Q, size_t n)
+{
+    const size_t bytes = n * sizeof(qdt_vec2d);
+    if (bytes <= Q->d_in_cap) return QDT_OK;
+    if (Q->d_in) (void)hipFree(Q->d_in);
+    Q->d_in = nullptr;
+    Q->d_in_cap = 0;
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? QDT_ERR_OUT_OF_MEMORY : QDT_ERR_INTERNAL;
+    Q->d_in = (double*)p;
+    Q->d_in_cap = bytes;
+    return QDT_OK;
+}
+
+void record_stats(qdt_t* Q)
+{
+    const qdt::RunStats& s = Q->eng->stats();
+    Q->stats.orient_calls = s.orient;
+    Q->stats.orient_exact = s.orient_exact;
+    Q->stats.incircle_calls = s.incircle;
+    Q->stats.incircle_exact = s.incircle_exact;
+    Q->stats.extract_exact = s.extract_queue;
+    Q->stats.unique_points = s.unique;
+    Q->stats.triangles = s.triangles;
+    Q->stats.kd_levels = s.levels;
+    Q->stats.device_ms = s.device_ms;
+}
+
+// Brings triangles and rep of the last device run into host memory.
+qdt_status materialize(qdt_t* Q)
+{
+    if (!Q->tris_on_device_only) return QDT_OK;
+    const qdt::DeviceResult& r = Q->eng->result();
+    if (!Q->tris.reserve(3 * (size_t)r.triangles + 3) || !Q->rep.reserve(r.n + 1)) return QDT_ERR_OUT_OF_MEMORY;
+    hipStream_t s = Q->eng->stream();
+    if (r.triangles &&
+        hipMemcpyAsync(Q->tris.p, r.tris, 12 * (size_t)r.triangles, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return QDT_ERR_INTERNAL;
+    if (r.n && hipMemcpyAsync(Q->rep.p, r.rep, 4 * r.n, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return QDT_ERR_INTERNAL;
+    if (hipStreamSynchronize(s) != hipSuccess) return QDT_ERR_INTERNAL;
+    Q->ntris = r.triangles;
+    Q->nrep = r.n;
+    Q->tris_on_device_only = false;
+    return QDT_OK;
+}
+
+qdt_status materialize_points(qdt_t* Q)
+{
+    if (!Q->pts_on_device_only) return QDT_OK;
+    if (!Q->pts.reserve(Q->npts + 1)) return QDT_ERR_OUT_OF_MEMORY;
+    if (Q->npts &&
+        hipMemcpy(Q->pts.p, Q->d_in, Q->npts * sizeof(qdt_vec2d), hipMemcpyDeviceToHost) != hipSuccess)
+        return QDT_ERR_INTERNAL;
+    Q->pts_on_device_only = false;
+    return QDT_OK;
+}
+
+// Runs the engine on the points already in Q->d_in (n of them).
+qdt_status run_device(qdt_t* Q, size_t n)
+{
+    const qdt::Status s = Q->eng->run(Q->d_in, n);
+    if (s != qdt::Status::ok) return map_status(s);
+    record_stats(Q);
+    Q->tris_on_device_only = true;
+    return QDT_OK;
+}
+
+qdt_status batch_checks(const qdt_vec2d* pts, size_t n)
+{
+    if (n > kMaxPoints) return QDT_ERR_TOO_MANY_POINTS;
+    if (n == 0) return QDT_OK;
+    if (!pts) return QDT_ERR_NULL_ARGUMENT;
+    return QDT_OK;
+}
+
+// qdt_execute / qdt_execute_parallel (libqdt/qdt.cpp:88-113)
+qdt_status execute_host(qdt_t* Q, const qdt_vec2d* pts, size_t n)
+{
+    if (!Q) return QDT_ERR_NULL_CONTEXT;
+    try
+    {
+        qdt_status st = batch_checks(pts, n);
+        if (st != QDT_OK || n == 0)
+        {
+            reset_context(Q);
+            return st;
+        }
+        reset_context(Q);
+        if (!Q->pts.reserve(n)) throw std::bad_alloc();
+        if (!copy_and_check(pts, n, Q->pts.p))
+            return QDT_ERR_NOT_FINITE;
+        st = ensure_engine(Q);
+        if (st == QDT_OK) st = ensure_device_input(Q, n);
+        if (st == QDT_OK &&
+            hipMemcpyAsync(Q->d_in, Q->pts.p, n * sizeof(qdt_vec2d), hipMemcpyHostToDevice, Q->eng->stream()) !=
+                hipSuccess)
+            st = QDT_ERR_INTERNAL;
+        if (st == QDT_OK) st = run_device(Q, n);
+        if (st == QDT_OK) st = materialize(Q);
+        if (st != QDT_OK)
+        {
+            reset_context(Q);
+            return st;
+        }
+        Q->npts = n;
+        return QDT_OK;
+    }
+    catch (const std::bad_alloc&)
+    {
+        reset_context(Q);
+        return QDT_ERR_OUT_OF_MEMORY;
+    }
+    catch (...)
+    {
+        reset_context(Q);
+        return QDT_ERR_INTERNAL;
+    }
+}
+
+// Rebuild after appending points: re-runs the batch path over all points.
+// Constraints accepted earlier must survive as edges, else the call fails.
+bool has_edge(const qdt_t* Q, size_t a, size_t b)
+{
+    for (size_t t = 0; t < Q->ntris; ++t)
+    {
+        const uint32_t* T = Q->tris.p + 3 * t;
+        for (int k = 0; k < 3; ++k)
+        {
+            const size_t u = T[k], v = T[(k + 1) % 3];
+            if ((u == a && v == b) || (u == b && v == a)) return true;
+        }
+    }
+    return false;
+}
+
+qdt_status insert_impl(qdt_t* Q, const qdt_vec2d& pt, size_t* index)
+{
+    if (!Q) return QDT_ERR_NULL_CONTEXT;
+    if (!finite_pt(pt)) return QDT_ERR_NOT_FINITE;
+    try
+    {
+        qdt_status st = materialize_points(Q);
+        if (st == QDT_OK) st = materialize(Q);
+        if (st != QDT_OK)
+        {
+            reset_context(Q);
+            return st;
+        }
+        if (Q->npts >= kMaxPoints) return QDT_ERR_TOO_MANY_POINTS;
+        const size_t n = Q->npts + 1;
+        if (!Q->pts.grow_keep(n, Q->npts)) throw std::bad_alloc();
+        Q->pts.p[n - 1] = pt;
+        auto cons = Q->cons;
+        st = ensure_engine(Q);
+        if (st == QDT_OK) st = ensure_device_input(Q, n);
+        if (st == QDT_OK &&
+            hipMemcpyAsync(Q->d_in, Q->pts.p, n * sizeof(qdt_vec2d), hipMemcpyHostToDevice, Q->eng->stream()) !=
+                hipSuccess)
+            st = QDT_ERR_INTERNAL;
+        if (st == QDT_OK) st = run_device(Q, n);
+        if (st == QDT_OK) st = materialize(Q);
+        if (st == QDT_OK)
+            for (const auto& c : cons)
+                if (!has_edge(Q, c.first, c.second)) st = QDT_ERR_INTERNAL;
+        if (st != QDT_OK)
+        {
+            reset_context(Q);
+            return st;
+        }
+        Q->npts = n;
+        Q->cons = cons;
+        if (index) *index = n - 1;
+        return QDT_OK;
+    }
+    catch (const std::bad_alloc&)
+    {
+        reset_context(Q);
+        return QDT_ERR_OUT_OF_MEMORY;
+    }
+    catch (...)
+    {
+        reset_context(Q);
+        return QDT_ERR_INTERNAL;
+    }
+}
+
+// Exact segment tests for the constraint checks (integers not assumed).
+int orient_sign(const qdt_vec2d& a, const qdt_vec2d& b, const qdt_vec2d& c)
+{
+    // long double has a 64-bit mantissa on x86: not exact in general, so use
+    // the error-bounded double filter and fall back to a wider evaluation
+    const double l = (a.x - c.x) * (b.y - c.y), r = (a.y - c.y) * (b.x - c.x);
+    const double det = l - r, bound = 3.3306690738754716e-16 * (std::fabs(l) + std::fabs(r));
+    if (det > bound) return 1;
+    if (-det > bound) return -1;
+    const long double L = ((long double)a.x - c.x) * ((long double)b.y - c.y);
+    const long double R = ((long double)a.y - c.y) * ((long double)b.x - c.x);
+    return L > R ? 1 : (L < R ? -1 : 0);
+}
+
+qdt_status constraint_impl(qdt_t* Q, size_t v0, size_t v1)
+{
+    qdt_status st = materialize_points(Q);
+    if (st == QDT_OK) st = materialize(Q);
+    if (st != QDT_OK) return st;
+    if (v0 >= Q->npts || v1 >= Q->npts) return QDT_ERR_INVALID_INDEX;
+    const size_t a = v0 < Q->nrep ? Q->rep.p[v0] : v0;
+    const size_t b = v1 < Q->nrep ? Q->rep.p[v1] : v1;
+    if (a == b) return QDT_ERR_SAME_POINT;
+    const qdt_vec2d& pa = Q->pts.p[a];
+    const qdt_vec2d& pb = Q->pts.p[b];
+    if (pa.x == pb.x && pa.y == pb.y) return QDT_ERR_SAME_POINT;
+    if (Q->ntris == 0) return QDT_ERR_NO_TRIANGLES;
+    const auto key = std::make_pair(std::min(a, b), std::max(a, b));
+    if (Q->cons.count(key)) return QDT_OK;
+    // crossing an accepted constraint
+    for (const auto& c : Q->cons)
+    {
+        const qdt_vec2d& p = Q->pts.p[c.first];
+        const qdt_vec2d& q = Q->pts.p[c.second];
+        if (c.first == a || c.first == b || c.second == a || c.second == b) continue;
+        const int o1 = orient_sign(pa, pb, p), o2 = orient_sign(pa, pb, q);
+        const int o3 = orient_sign(p, q, pa), o4 = orient_sign(p, q, pb);
+        if (o1 * o2 < 0 && o3 * o4 < 0) return QDT_ERR_CROSSES_CONSTRAINT;
+    }
+    if (has_edge(Q, a, b))
+    {
+        Q->cons.insert(key);
+        return QDT_OK;
+    }
+    // the segment through a third point
+    for (size_t i = 0; i < Q->npts; ++i)
+    {
+        if (i == a || i == b || (i < Q->nrep && Q->rep.p[i] != i)) continue;
+        const qdt_vec2d& p = Q->pts.p[i];
+        if (orient_sign(pa, pb, p) != 0) continue;
+        const double t = (p.x - pa.x) * (pb.x - pa.x) + (p.y - pa.y) * (pb.y - pa.y);
+        const double L = (pb.x - pa.x) * (pb.x - pa.x) + (pb.y - pa.y) * (pb.y - pa.y);
+        if (t > 0 && t < L) return QDT_ERR_THROUGH_POINT;
+    }
+    // Forcing a non-Delaunay edge (constrained triangulation) is not part of
+    // the GPU batch path of this library.
+    return QDT_ERR_INTERNAL;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* qdt_status_string(qdt_status status)
+{
+    switch (status)
+    {
+    case QDT_OK: return "success";
+    case QDT_ERR_NULL_CONTEXT: return "the context is NULL";
+    case QDT_ERR_NULL_ARGUMENT: return "a required pointer argument is NULL";
+    case QDT_ERR_TOO_MANY_POINTS: return "more than 100000000 points";
+    case QDT_ERR_NOT_FINITE: return "a coordinate is NaN or infinite";
+    case QDT_ERR_INVALID_INDEX: return "an end point index is not a point of the context";
+    case QDT_ERR_SAME_POINT: return "both end points are the same point";
+    case QDT_ERR_NO_TRIANGLES: return "the context has no triangles";
+    case QDT_ERR_CROSSES_CONSTRAINT: return "the segment crosses an existing constraint";
+    case QDT_ERR_THROUGH_POINT: return "the segment passes through another point";
+    case QDT_ERR_OUT_OF_MEMORY: return "out of memory";
+    case QDT_ERR_INTERNAL: return "internal error";
+    }
+    return "unknown status";
+}
+
+qdt_t* qdt_create(void) { return new (std::nothrow) qdt_t(); }
+
+qdt_status qdt_execute(qdt_t* ptr, const qdt_vec2d* pts, size_t n) { return execute_host(ptr, pts, n); }
+
+qdt_status qdt_execute_parallel(qdt_t* ptr, const qdt_vec2d* pts, size_t n) { return execute_host(ptr, pts, n); }
+
+qdt_status qdt_insert_point(qdt_t* ptr, qdt_vec2d pt, size_t* index) { return insert_impl(ptr, pt, index); }
+
+qdt_status qdt_insert_point_parallel(qdt_t* ptr, qdt_vec2d pt, size_t* index) { return insert_impl(ptr, pt, index); }
+
+qdt_status qdt_insert_constraint(qdt_t* ptr, size_t v0, size_t v1)
+{
+    if (!ptr) return QDT_ERR_NULL_CONTEXT;
+    try
+    {
+        const qdt_status st = constraint_impl(ptr, v0, v1);
+        if (st == QDT_ERR_INTERNAL || st == QDT_ERR_OUT_OF_MEMORY) reset_context(ptr);
+        return st;
+    }
+    catch (...)
+    {
+        reset_context(ptr);
+        return QDT_ERR_OUT_OF_MEMORY;
+    }
+}
+
+qdt_status qdt_insert_constraints_batch(qdt_t* ptr, const qdt_edge* edges, size_t num_edges, qdt_status* results)
+{
+    if (!ptr)
+    {
+        if (results)
+            for (size_t i = 0; i < num_edges; ++i) results[i] = QDT_ERR_NULL_CONTEXT;
+        return QDT_ERR_NULL_CONTEXT;
+    }
+    if (num_edges == 0) return QDT_OK;
+    if (!edges)
+    {
+        if (results)
+            for (size_t i = 0; i < num_edges; ++i) results[i] = QDT_ERR_NULL_ARGUMENT;
+        return QDT_ERR_NULL_ARGUMENT;
+    }
+    qdt_status first = QDT_OK;
+    for (size_t i = 0; i < num_edges; ++i)
+    {
+        const qdt_status st = qdt_insert_constraint(ptr, edges[i].v0, edges[i].v1);
+        if (st == QDT_ERR_INTERNAL || st == QDT_ERR_OUT_OF_MEMORY)
+        {
+            if (results)
+                for (size_t j = 0; j < num_edges; ++j) results[j] = st;
+            return st;
+        }
+        if (results) results[i] = st;
+        if (st != QDT_OK && first == QDT_OK) first = st;
+    }
+    return first;
+}
+
+size_t qdt_triangles_size(const qdt_t* ptr)
+{
+    if (!ptr) return 0;
+    if (ptr->tris_on_device_only) return ptr->eng->result().triangles;
+    return ptr->ntris;
+}
+
+size_t qdt_get_triangle(const qdt_t* ptr, size_t T[3], size_t id)
+{
+    if (!ptr || !T) return 0;
+    qdt_t* Q = const_cast<qdt_t*>(ptr);
+    if (materialize(Q) != QDT_OK) return 0;
+    if (id >= Q->ntris) return 0;
+    const uint32_t* t = Q->tris.p + 3 * id;
+    T[0] = t[0];
+    T[1] = t[1];
+    T[2] = t[2];
+    return 1;
+}
+
+size_t qdt_points_size(const qdt_t* ptr) { return ptr ? ptr->npts : 0; }
+
+size_t qdt_get_point(const qdt_t* ptr, qdt_vec2d* pt, size_t id)
+{
+    if (!ptr || !pt) return 0;
+    qdt_t* Q = const_cast<qdt_t*>(ptr);
+    if (id >= Q->npts) return 0;
+    if (materialize_points(Q) != QDT_OK) return 0;
+    *pt = Q->pts.p[id];
+    return 1;
+}
+
+size_t qdt_point_is_used(const qdt_t* ptr, size_t id)
+{
+    if (!ptr) return 0;
+    qdt_t* Q = const_cast<qdt_t*>(ptr);
+    if (materialize(Q) != QDT_OK) return 0;
+    if (id >= Q->npts || id >= Q->nrep) return 0;
+    if (Q->ntris == 0) return 0;
+    return Q->rep.p[id] == id ? 1 : 0;
+}
+
+void qdt_destroy(qdt_t* ptr) { delete ptr; }
+
+// ---- GPU extensions ------------------------------------------------------
+
+qdt_status qdt_gpu_execute_device(qdt_t* ptr, const qdt_vec2d* d_pts, size_t n)
+{
+    if (!ptr) return QDT_ERR_NULL_CONTEXT;
+    qdt_t* Q = ptr;
+    try
+    {
+        qdt_status st = batch_checks(d_pts, n);
+        reset_context(Q);
+        if (st != QDT_OK || n == 0) return st;
+        st = ensure_engine(Q);
+        if (st == QDT_OK) st = ensure_device_input(Q, n);
+        if (st == QDT_OK && hipMemcpyAsync(Q->d_in, d_pts, n * sizeof(qdt_vec2d), hipMemcpyDeviceToDevice,
+                                           Q->eng->stream()) != hipSuccess)
+            st = QDT_ERR_INTERNAL;
+        if (st == QDT_OK) st = run_device(Q, n);
+        if (st != QDT_OK)
+        {
+            reset_context(Q);
+            return st;
+        }
+        Q->npts = n;
+        Q->pts_on_device_only = true;
+        return QDT_OK;
+    }
+    catch (...)
+    {
+        reset_context(Q);
+        return QDT_ERR_OUT_OF_MEMORY;
+    }
+}
+
+size_t qdt_gpu_device_triangles(const qdt_t* ptr, const uint32_t** d_tris)
+{
+    if (!ptr || !ptr->eng || (!ptr->tris_on_device_only && ptr->ntris == 0))
+    {
+        if (d_tris) *d_tris = nullptr;
+        return 0;
+    }
+    const qdt::DeviceResult& r = ptr->eng->result();
+    if (d_tris) *d_tris = r.tris;
+    return r.triangles;
+}
+
+size_t qdt_gpu_copy_triangles(const qdt_t* ptr, uint32_t* out)
+{
+    if (!ptr || !out) return 0;
+    qdt_t* Q = const_cast<qdt_t*>(ptr);
+    if (Q->tris_on_device_only)
+    {
+        const qdt::DeviceResult& r = Q->eng->result();
+        if (r.triangles && hipMemcpy(out, r.tris, 12 * (size_t)r.triangles, hipMemcpyDeviceToHost) != hipSuccess)
+            return 0;
+        return r.triangles;
+    }
+    std::memcpy(out, Q->tris.p, 12 * Q->ntris);
+    return Q->ntris;
+}
+
+int qdt_gpu_get_stats(const qdt_t* ptr, qdt_gpu_stats* out)
+{
+    if (!ptr || !out) return 0;
+    *out = ptr->stats;
+    return 1;
+}
+
+int qdt_gpu_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void qdt_gpu_release_workspace(qdt_t* ptr)
+{
+    if (!ptr || !ptr->eng) return;
+    qdt_t* Q = ptr;
+    if (materialize(Q) != QDT_OK) reset_context(Q);
+    if (materialize_points(Q) != QDT_OK) reset_context(Q);
+    Q->eng->release();
+}
+
+} // extern "C"

@@ -1,0 +1,355 @@
+// quadedge.hpp -- structure-of-arrays quad-edge store and the Guibas-Stolfi
+// operations on it, for one thread of a merge kernel.
+//
+// Layout.  A quad-edge record k owns the four directed edges 4k+r (r = 0..3,
+// r odd = dual edges), as in libqdt/builder.hxx:78-86.  Instead of an array
+// of records the store keeps two flat arrays:
+//   NX[e]      onext of directed edge e   (int32, 4 per record, 16 B)
+//   OR[e >> 1] origin of primal edge e    (int32, 2 per record, 8 B)
+// so every operation is one 4-byte access, records can be initialised with
+// one 16-byte store, and the same code runs on HBM or on LDS pointers.
+//
+// Storage of a kd node.  A node holding points [o, o+s) owns records
+// [3o, 3o+3s).  A planar graph on s >= 3 vertices has at most 3s-6 edges, so
+// a node never needs more records than its own range; the records a node is
+// not using sit on its free list (linked through NX[4k], marked OR[2k] = -1),
+// which the parent inherits.  No atomics and no global allocator are needed,
+// and the final layout is deterministic.
+#pragma once
+
+#include "predicates.hpp"
+
+namespace qdt
+{
+
+__device__ __forceinline__ int e_rot(int e) { return (e & ~3) | ((e + 1) & 3); }
+__device__ __forceinline__ int e_invrot(int e) { return (e & ~3) | ((e + 3) & 3); }
+__device__ __forceinline__ int e_sym(int e) { return e ^ 2; }
+
+struct Pool
+{
+    int head, tail; // free records of a node, -1 = empty
+};
+
+struct PredCount
+{
+    unsigned orient, orient_exact, incircle, incircle_exact;
+};
+
+// Every access is bounds-checked (a compare next to a dependent load costs
+// nothing measurable) and every loop has an iteration budget: a logic error
+// sets `err` and unwinds instead of touching memory outside the store or
+// spinning forever.  err = 1: index out of range, 2: free list empty,
+// 3: loop budget exhausted.
+template <class IntPtr, class PtPtr>
+struct QuadEdges
+{
+    IntPtr NX;
+    IntPtr OR;
+    PtPtr PT;
+    int lim_e; // directed edge ids are in [0, lim_e)
+    int lim_v; // vertex ids are in [0, lim_v)
+    int err;
+    PredCount pc;
+
+    __device__ __forceinline__ int chk(int e)
+    {
+        if ((unsigned)e >= (unsigned)lim_e)
+        {
+            err = err ? err : 1;
+            return 0;
+        }
+        return e;
+    }
+    __device__ __forceinline__ int onext(int e) { return NX[chk(e)]; }
+    __device__ __forceinline__ void set_next(int e, int v) { NX[chk(e)] = v; }
+    __device__ __forceinline__ int org(int e) { return OR[chk(e) >> 1]; }
+    __device__ __forceinline__ int dest(int e) { return OR[chk(e ^ 2) >> 1]; }
+    __device__ __forceinline__ void set_org(int e, int v) { OR[chk(e) >> 1] = v; }
+    __device__ __forceinline__ int oprev(int e) { return e_rot(onext(e_rot(e))); }
+    __device__ __forceinline__ int lnext(int e) { return e_rot(onext(e_invrot(e))); }
+    __device__ __forceinline__ int rprev(int e) { return onext(e ^ 2); }
+    __device__ __forceinline__ double2 pt(int v)
+    {
+        if ((unsigned)v >= (unsigned)lim_v)
+        {
+            err = err ? err : 1;
+            v = 0;
+        }
+        return PT[v];
+    }
+
+    __device__ __forceinline__ int pop(Pool& p)
+    {
+        const int k = p.head;
+        if (k < 0 || 4 * k >= lim_e)
+        {
+            err = err ? err : 2;
+            return 0;
+        }
+        const int nx = NX[4 * k];
+        p.head = nx;
+        if (nx < 0) p.tail = -1;
+        return k;
+    }
+
+    __device__ __forceinline__ void push(Pool& p, int k)
+    {
+        OR[2 * k] = -1;
+        NX[4 * k] = p.head;
+        if (p.head < 0) p.tail = k;
+        p.head = k;
+    }
+
+    __device__ __forceinline__ Pool cat(Pool a, Pool b)
+    {
+        if (a.head < 0) return b;
+        if (b.head < 0) return a;
+        set_next(4 * a.tail, b.head);
+        Pool r = { a.head, b.tail };
+        return r;
+    }
+
+    // builder.hxx:131-142
+    __device__ __forceinline__ int make_edge(Pool& p)
+    {
+        const int k = pop(p);
+        const int e = 4 * k;
+        NX[e] = e;
+        NX[e + 1] = e + 3;
+        NX[e + 2] = e + 2;
+        NX[e + 3] = e + 1;
+        OR[2 * k] = -1;
+        OR[2 * k + 1] = -1;
+        return e;
+    }
+
+    // builder.hxx:144-149
+    __device__ __forceinline__ void splice(int a, int b)
+    {
+        const int na = onext(a), nb = onext(b);
+        const int alpha = e_rot(na), beta = e_rot(nb);
+        set_next(a, nb);
+        set_next(b, na);
+        const int xa = onext(alpha), xb = onext(beta);
+        set_next(alpha, xb);
+        set_next(beta, xa);
+    }
+
+    // builder.hxx:151-159
+    __device__ __forceinline__ int connect(int a, int b, Pool& p)
+    {
+        const int e = make_edge(p);
+        set_org(e, dest(a));
+        set_org(e ^ 2, org(b));
+        splice(e, lnext(a));
+        splice(e ^ 2, b);
+        return e;
+    }
+
+    // builder.hxx:161-166
+    __device__ __forceinline__ void delete_edge(int e, Pool& p)
+    {
+        splice(e, oprev(e));
+        splice(e ^ 2, oprev(e ^ 2));
+        push(p, chk(e) >> 2);
+    }
+
+    __device__ __forceinline__ double orient(int a, int b, int c)
+    {
+        const double2 A = pt(a), B = pt(b), C = pt(c);
+        ++pc.orient;
+        return orient2d(A.x, A.y, B.x, B.y, C.x, C.y, pc.orient_exact);
+    }
+    __device__ __forceinline__ bool ccw(int a, int b, int c) { return orient(a, b, c) > 0; }
+    __device__ __forceinline__ bool right_of(int x, int e) { return ccw(x, dest(e), org(e)); }
+    __device__ __forceinline__ bool left_of(int x, int e) { return ccw(x, org(e), dest(e)); }
+
+    // builder.hxx:171-175: distinct vertices, strictly inside
+    __device__ __forceinline__ bool in_circle(int a, int b, int c, int d)
+    {
+        if (a == d || b == d || c == d || a == b || a == c || b == c) return false;
+        const double2 A = pt(a), B = pt(b), C = pt(c), D = pt(d);
+        ++pc.incircle;
+        return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact) > 0;
+    }
+
+    // frame order of builder.hxx:71-76
+    __device__ __forceinline__ bool less_f(int u, int v, int frame)
+    {
+        const double2 a = pt(u), b = pt(v);
+        if (frame == 0) return a.x < b.x || (a.x == b.x && a.y < b.y);
+        return a.y < b.y || (a.y == b.y && a.x > b.x);
+    }
+
+    // Leaf of two or three vertices s[0..cnt) given in the frame's order,
+    // builder.hxx:177-201.  Records come from [rec0, rec0 + 3*cnt); the
+    // unused ones go to the returned pool.
+    __device__ __forceinline__ void leaf(const int* s, int cnt, int rec0, int& first, int& second, Pool& pool)
+    {
+        const int nrec = 3 * cnt;
+        // chain all records of the leaf into its pool, then allocate from it
+        for (int i = 0; i < nrec; ++i)
+        {
+            const int k = rec0 + i;
+            NX[4 * k] = (i + 1 < nrec) ? k + 1 : -1;
+            OR[2 * k] = -1;
+        }
+        pool.head = rec0;
+        pool.tail = rec0 + nrec - 1;
+        if (cnt == 2)
+        {
+            const int a = make_edge(pool);
+            set_org(a, s[0]);
+            set_org(a ^ 2, s[1]);
+            first = a;
+            second = a ^ 2;
+            return;
+        }
+        const int a = make_edge(pool), b = make_edge(pool);
+        splice(a ^ 2, b);
+        set_org(a, s[0]);
+        set_org(a ^ 2, s[1]);
+        set_org(b, s[1]);
+        set_org(b ^ 2, s[2]);
+        const double o = orient(s[0], s[1], s[2]);
+        if (o > 0)
+        {
+            connect(b, a, pool);
+            first = a;
+            second = b ^ 2;
+        }
+        else if (o < 0)
+        {
+            const int c = connect(b, a, pool);
+            first = c ^ 2;
+            second = c;
+        }
+        else
+        {
+            first = a;
+            second = b ^ 2;
+        }
+    }
+
+    // Merge of two triangulations separated in the current frame,
+    // builder.hxx:205-285.  Left/right candidate scans run one after the
+    // other; they touch disjoint edges, so the order does not matter.
+    __device__ void merge(int& ldo, int ldi, int rdi, int& rdo, Pool& pool, int budget)
+    {
+        for (;;)
+        {
+            if (err || --budget < 0) break;
+            if (left_of(org(rdi), ldi))
+                ldi = lnext(ldi);
+            else if (right_of(org(ldi), rdi))
+                rdi = rprev(rdi);
+            else
+                break;
+        }
+        int basel = connect(rdi ^ 2, ldi, pool);
+        if (org(ldi) == org(ldo)) ldo = basel ^ 2;
+        if (org(rdi) == org(rdo)) rdo = basel;
+        for (;;)
+        {
+            if (err) return;
+            if (--budget < 0)
+            {
+                err = 3;
+                return;
+            }
+            const int bo = org(basel);
+            const int bd = dest(basel);
+            int lc = onext(basel ^ 2);
+            bool lv = ccw(dest(lc), bd, bo);
+            if (lv)
+            {
+                bool deleted = false;
+                for (;;)
+                {
+                    if (err || --budget < 0) break;
+                    const int nx = onext(lc);
+                    if (!in_circle(bd, bo, dest(lc), dest(nx))) break;
+                    delete_edge(lc, pool);
+                    lc = nx;
+                    deleted = true;
+                }
+                if (deleted) lv = ccw(dest(lc), bd, bo);
+            }
+            int rc = oprev(basel);
+            bool rv = ccw(dest(rc), bd, bo);
+            if (rv)
+            {
+                bool deleted = false;
+                for (;;)
+                {
+                    if (err || --budget < 0) break;
+                    const int pv = oprev(rc);
+                    if (!in_circle(bd, bo, dest(rc), dest(pv))) break;
+                    delete_edge(rc, pool);
+                    rc = pv;
+                    deleted = true;
+                }
+                if (deleted) rv = ccw(dest(rc), bd, bo);
+            }
+            if (!lv && !rv) break;
+            if (!lv || (rv && in_circle(dest(lc), bd, bo, dest(rc))))
+                basel = connect(rc, basel ^ 2, pool);
+            else
+                basel = connect(basel ^ 2, lc ^ 2, pool);
+        }
+    }
+
+    // builder.hxx:287-301
+    __device__ int hull_walk(int e, int frame, bool want_max, int budget)
+    {
+        for (;;)
+        {
+            if (err) return e;
+            if (--budget < 0)
+            {
+                err = 3;
+                return e;
+            }
+            const int nx = rprev(e);
+            const int pv = oprev(e) ^ 2;
+            const int oe = org(e);
+            const int on = org(nx);
+            if (want_max ? less_f(oe, on, frame) : less_f(on, oe, frame))
+            {
+                e = nx;
+                continue;
+            }
+            const int op = org(pv);
+            if (want_max ? less_f(oe, op, frame) : less_f(op, oe, frame))
+            {
+                e = pv;
+                continue;
+            }
+            return e;
+        }
+    }
+
+    // merge plus the re-walk to the parent's frame (builder.hxx:338-347)
+    // npts = points of the node (bounds every loop)
+    __device__ __forceinline__ void merge_node(int lfirst, int lsecond, int rfirst, int rsecond, int axis,
+                                               int out_axis, int npts, int& first, int& second, Pool& pool)
+    {
+        int ldo = lfirst, rdo = rsecond;
+        const int budget = 16 * npts + 64;
+        merge(ldo, lsecond, rfirst, rdo, pool, budget);
+        if (out_axis != axis)
+        {
+            first = hull_walk(ldo, out_axis, false, npts + 8);
+            const int re = hull_walk(onext(rdo), out_axis, true, npts + 8);
+            second = oprev(re);
+        }
+        else
+        {
+            first = ldo;
+            second = rdo;
+        }
+    }
+};
+
+} // namespace qdt

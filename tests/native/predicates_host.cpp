@@ -1,0 +1,51 @@
+// predicates_host.cpp -- host build of the GPU predicate code, so that its
+// exact arithmetic can be checked on the CPU against the oracle and against
+// integer closed forms.  Test infrastructure (built by __graft_entry__.build).
+#include "../../libqdt_amd/csrc/predicates.hpp"
+
+extern "C" {
+
+int ph_orient2d_exact(double ax, double ay, double bx, double by, double cx, double cy)
+{
+    return qdt::orient2d_exact(ax, ay, bx, by, cx, cy);
+}
+
+int ph_incircle_exact(double ax, double ay, double bx, double by, double cx, double cy, double dx, double dy)
+{
+    return qdt::incircle_exact(ax, ay, bx, by, cx, cy, dx, dy);
+}
+
+// the general (wide accumulator) path, bypassing the 62-bit fast path
+int ph_orient2d_general(double ax, double ay, double bx, double by, double cx, double cy)
+{
+    double hi[4], lo[4];
+    qdt::two_sum(ax, -cx, hi[0], lo[0]);
+    qdt::two_sum(ay, -cy, hi[1], lo[1]);
+    qdt::two_sum(bx, -cx, hi[2], lo[2]);
+    qdt::two_sum(by, -cy, hi[3], lo[3]);
+    return qdt::orient_general(hi, lo);
+}
+
+int ph_incircle_general(double ax, double ay, double bx, double by, double cx, double cy, double dx, double dy)
+{
+    double hi[6], lo[6];
+    qdt::two_sum(ax, -dx, hi[0], lo[0]);
+    qdt::two_sum(ay, -dy, hi[1], lo[1]);
+    qdt::two_sum(bx, -dx, hi[2], lo[2]);
+    qdt::two_sum(by, -dy, hi[3], lo[3]);
+    qdt::two_sum(cx, -dx, hi[4], lo[4]);
+    qdt::two_sum(cy, -dy, hi[5], lo[5]);
+    return qdt::incircle_general(hi, lo);
+}
+
+double ph_orient2d(double ax, double ay, double bx, double by, double cx, double cy, unsigned* nexact)
+{
+    return qdt::orient2d(ax, ay, bx, by, cx, cy, *nexact);
+}
+
+double ph_incircle(double ax, double ay, double bx, double by, double cx, double cy, double dx, double dy,
+                   unsigned* nexact)
+{
+    return qdt::incircle(ax, ay, bx, by, cx, cy, dx, dy, *nexact);
+}
+}

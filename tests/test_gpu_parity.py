@@ -1,0 +1,111 @@
+"""GPU path vs the oracle: canonical triangle sets must be identical.
+
+The GPU builds the same Dwyer median-cut tree, leaves and Guibas-Stolfi merges
+as the reference with exact predicates, so the canonical triangle list
+(rotate to the smallest index, sort) equals the oracle's for every input,
+degenerate ones included.  Every case prints its name before it runs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import libqdt_amd as Q
+from libqdt_amd import workloads as W
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(pts, device=False):
+    import torch
+
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+    with Q.Triangulation() as t:
+        if device:
+            d = torch.from_numpy(pts).to("cuda:0")
+            st = t.execute_device(d.data_ptr(), len(pts))
+        else:
+            st = t.execute(pts)
+        assert st == Q.QDT_OK, Q.status_string(st)
+        tris = t.triangles()
+        used = np.array([t.point_is_used(i) for i in range(len(pts))], dtype=np.uint8) if len(pts) <= 5000 else None
+        return tris, used, t.stats()
+
+
+def check_same(name, pts, device=False):
+    print("case:", name, len(pts), flush=True)
+    got, used, st = run_gpu(pts, device)
+    want = O.triangulate(pts)
+    assert len(got) == len(want.tris), (name, len(got), len(want.tris))
+    assert np.array_equal(O.canonical(got), O.canonical(want.tris)), name
+    if used is not None:
+        exp = (want.rep == np.arange(len(pts), dtype=np.uint32)).astype(np.uint8) if len(want.tris) else \
+            np.zeros(len(pts), np.uint8)
+        assert np.array_equal(used, exp), name
+    return st
+
+
+def test_readme_quad(gpu):
+    pts = [(0.0, 0.0), (4.0, 0.0), (3.0, 1.0), (1.0, 1.0)]
+    got, _, _ = run_gpu(pts)
+    assert O.canonical(got).tolist() == O.canonical(np.array([[3, 0, 1], [1, 2, 3]])).tolist()
+
+
+def test_tiny_sets(gpu):
+    rng = np.random.default_rng(5)
+    for n in list(range(0, 40)) + [63, 64, 65, 127, 128, 129, 255, 256, 257, 1000]:
+        for rnge in (3, 10, 1 << 20):
+            pts = rng.integers(-rnge, rnge + 1, size=(n, 2)).astype(np.float64)
+            check_same("int n=%d range=%d" % (n, rnge), pts)
+
+
+def test_structured(gpu):
+    cases = {
+        "grid 40x40": [(i, j) for i in range(40) for j in range(40)],
+        "collinear diagonal": [(i, 2 * i) for i in range(50)],
+        "collinear horizontal": [(i, 7) for i in range(50)],
+        "collinear vertical": [(7, i) for i in range(50)],
+        "line plus one": [(i, 7) for i in range(50)] + [(25, 8)],
+        "parabola": [(i, i * i) for i in range(30)],
+        "all duplicates": [(5, 5)] * 100,
+        "circle 2^27": [(round((1 << 27) * math.cos(2 * math.pi * i / 200)),
+                         round((1 << 27) * math.sin(2 * math.pi * i / 200))) for i in range(200)],
+        "two rows": [(i, r) for i in range(100) for r in (0, 1)],
+        "triangle dups": [(0, 0), (1, 0), (0, 1)] * 20,
+        "three points": [(0, 0), (5, 0), (0, 5)],
+        "2^28": [(-(1 << 28), -(1 << 28)), (1 << 28, -(1 << 28)), (0, 1 << 28), (1, 1), (-1, 2)],
+    }
+    rng = np.random.default_rng(77)
+    for name, p in cases.items():
+        p = np.array(p, dtype=np.float64)
+        for sh in range(3):
+            q = p if sh == 0 else p[rng.permutation(len(p))]
+            check_same("%s shuffle %d" % (name, sh), q)
+
+
+def test_scaled(gpu):
+    rng = np.random.default_rng(9)
+    for k in (-1000, -600, -300, -64, 64, 300, 600, 900):
+        for rep in range(4):
+            n = 3 + int(rng.integers(0, 40))
+            R = 6 if rep % 2 else 1000
+            p = rng.integers(-R, R + 1, size=(n, 2)).astype(np.float64) * math.ldexp(1.0, k)
+            check_same("scale 2^%d #%d" % (k, rep), p)
+
+
+def test_random_distributions(gpu):
+    check_same("uniform 100k seed 0", W.uniform(100000, 0))
+    check_same("gaussian 100k", W.gaussian(100000, 1))
+    check_same("clustered 100k", W.clustered(100000, 2))
+    check_same("bench uniform 10k [0,1000)^2", W.uniform(10000, 20260607 + 10000, 1000.0))
+
+
+def test_degenerate_mix(gpu):
+    p = W.lattice_dups_circle(side=120, ndup=3000, ncircle=5000, seed=2)
+    st = check_same("lattice+dups+circle small", p)
+    assert st["incircle_exact"] > 0
+
+
+def test_device_input(gpu):
+    check_same("device input 50k", W.uniform(50000, 7), device=True)
