@@ -30,6 +30,10 @@ typedef struct qdt_gpu_stats
     uint64_t triangles;
     uint64_t kd_levels;           /* levels of the median-cut tree */
     double device_ms;             /* time of the device pipeline (events) */
+    /* filled only with profiling on (qdt_gpu_set_profiling): */
+    double stage_ms[8];           /* keys+y sort, x sort, dedupe, frame-1 sort, kd levels, leaves, merges, extract */
+    double merge_launch_ms_sum;   /* sum of the merge kernel launches' durations */
+    uint64_t merge_launches;
 } qdt_gpu_stats;
 
 /* Like qdt_execute(), but d_pts points to n points in device memory.  The
@@ -49,6 +53,9 @@ QDT_API size_t qdt_gpu_copy_triangles(const qdt_t* ptr, uint32_t* out);
 
 /* Counters of the last batch call; returns 1, or 0 for a NULL argument. */
 QDT_API int qdt_gpu_get_stats(const qdt_t* ptr, qdt_gpu_stats* out);
+
+/* Turns per-stage HIP event timing on or off for a context (default off). */
+QDT_API void qdt_gpu_set_profiling(qdt_t* ptr, int enable);
 
 /* Number of HIP devices visible to the library (0 without a GPU). */
 QDT_API int qdt_gpu_device_count(void);

@@ -37,7 +37,7 @@ EXPORTS = [
 ]
 GPU_EXPORTS = [
     "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
-    "qdt_gpu_device_count", "qdt_gpu_release_workspace",
+    "qdt_gpu_set_profiling", "qdt_gpu_device_count", "qdt_gpu_release_workspace",
 ]
 
 
@@ -55,10 +55,15 @@ class GpuStats(ctypes.Structure):
         ("incircle_calls", ctypes.c_uint64), ("incircle_exact", ctypes.c_uint64),
         ("extract_exact", ctypes.c_uint64), ("unique_points", ctypes.c_uint64),
         ("triangles", ctypes.c_uint64), ("kd_levels", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+        ("stage_ms", ctypes.c_double * 8), ("merge_launch_ms_sum", ctypes.c_double),
+        ("merge_launches", ctypes.c_uint64),
     ]
+    STAGES = ("keys_sort_y", "sort_x", "dedupe", "frame1_sort", "kd_levels", "leaves", "merges", "extract")
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stage_ms"}
+        d["stage_ms"] = dict(zip(self.STAGES, list(self.stage_ms)))
+        return d
 
 
 _lib = None
@@ -106,6 +111,8 @@ def lib():
         L.qdt_gpu_copy_triangles.argtypes = [vp, vp]
         L.qdt_gpu_get_stats.restype = ctypes.c_int
         L.qdt_gpu_get_stats.argtypes = [vp, ctypes.POINTER(GpuStats)]
+        L.qdt_gpu_set_profiling.restype = None
+        L.qdt_gpu_set_profiling.argtypes = [vp, ctypes.c_int]
         L.qdt_gpu_device_count.restype = ctypes.c_int
         L.qdt_gpu_device_count.argtypes = []
         L.qdt_gpu_release_workspace.restype = None
@@ -221,6 +228,9 @@ class Triangulation:
         s = GpuStats()
         lib().qdt_gpu_get_stats(self._h, ctypes.byref(s))
         return s.as_dict()
+
+    def set_profiling(self, on=True):
+        lib().qdt_gpu_set_profiling(self._h, 1 if on else 0)
 
     def release_workspace(self):
         lib().qdt_gpu_release_workspace(self._h)

@@ -474,8 +474,20 @@ Engine::Engine(hipStream_t stream) : stream_(stream), own_stream_(stream == null
     std::memset(b_, 0, sizeof(Bufs));
 }
 
+hipEvent_t Engine::prof_event(size_t i)
+{
+    while (evs_.size() <= i)
+    {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        evs_.push_back(e);
+    }
+    return evs_[i];
+}
+
 Engine::~Engine()
 {
+    for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
     release();
     if (b_ && b_->host_scalars) (void)hipHostFree(b_->host_scalars);
     delete b_;
@@ -604,7 +616,17 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     std::vector<uint32_t> ghist_host(8 * 256);
     hghist = ghist_host.data();
 
+    // profiling: event 0..8 = stage boundaries, 9.. = merge launches (start, end)
+    const bool prof = profiling_;
+    auto mark = [&](size_t i) {
+        if (prof)
+        {
+            hipEvent_t e = prof_event(i);
+            if (e) (void)hipEventRecord(e, s);
+        }
+    };
     QDT_HIP(hipEventRecord(ev0_, s));
+    mark(0);
     QDT_HIP(hipMemsetAsync(b.counters, 0, 8 * 8, s));
     QDT_HIP(hipMemsetAsync(b.err, 0, 64, s));
 
@@ -620,11 +642,13 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     QDT_HIP(hipMemcpyAsync(hs, b.err, 4, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
     if (hs[0] & 1) return Status::not_finite;
+    mark(1);
     // 2. stable sort by x -> (x, y, index)
     k_gather_u64<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.kx, n, kA);
     st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
     if (st != Status::ok) return st;
 
+    mark(2);
     // 3. dedupe
     DedupeFlag df{ kA, vA, b.ky };
     scan_u32(df, n, StoreU32{ b.dist }, b.partials, s);
@@ -648,6 +672,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
         return Status::ok;
     }
 
+    mark(3);
     // 4. frame-1 order: (y asc, x desc) = stable y-sort of the reversed frame-0 order
     k_frame1_keys<<<cdiv(m, kBlock), kBlock, 0, s>>>(b.P0, m, b.kA, b.vA);
     kA = b.kA;
@@ -661,6 +686,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     uint32_t* B0 = b.B0;
     k_iota<<<cdiv(m, kBlock), kBlock, 0, s>>>(B0, m);
 
+    mark(4);
     // 5. kd levels
     const int D = kd_depth(m);
     if (D > b.dcap) return Status::internal;
@@ -684,18 +710,23 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     }
     (void)B1spare;
 
+    mark(5);
     // 6. leaves, 7. merges
     {
         const uint32_t cnt = 1u << D;
         k_leaves<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, B0, B1, b.off, D, m, b.res,
                                                                  b.err, b.counters);
     }
+    mark(6);
     for (int d = D - 1; d >= 0; --d)
     {
         const uint32_t cnt = 1u << d;
+        mark(9 + 2 * (size_t)d);
         k_merge<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.off, d, m, b.res, b.err,
                                                                 b.counters);
+        mark(10 + 2 * (size_t)d);
     }
+    mark(7);
 
     // 8. extraction
     const uint32_t nrec = 3 * m;
@@ -703,6 +734,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
                                                           b.counters, b.err);
     k_extract_exact<<<256, kBlock, 0, s>>>(b.NX, b.OR, b.P0, m, b.queue, b.counters, b.qcap, b.face);
     scan_u32(FaceCount{ b.face }, nrec, FaceWrite{ b.face, b.NX, b.OR, b.ID0, b.tris, m }, b.partials, s);
+    mark(8);
     QDT_HIP(hipEventRecord(ev1_, s));
     const uint32_t nbx = cdiv(nrec, kScanTile);
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbx, 4, hipMemcpyDeviceToHost, s));
@@ -720,6 +752,24 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     st_.incircle_exact = cnts[3];
     st_.extract_queue = cnts[4];
     st_.triangles = hs[0];
+    if (prof && evs_.size() >= 9 + 2 * (size_t)D)
+    {
+        for (int i = 0; i < 8; ++i)
+        {
+            float t = 0;
+            (void)hipEventElapsedTime(&t, evs_[i], evs_[i + 1]);
+            st_.stage_ms[i] = t;
+        }
+        double sum = 0;
+        for (int d = 0; d < D; ++d)
+        {
+            float t = 0;
+            (void)hipEventElapsedTime(&t, evs_[9 + 2 * d], evs_[10 + 2 * d]);
+            sum += t;
+        }
+        st_.merge_launch_ms_sum = sum;
+        st_.merge_launches = (uint32_t)D;
+    }
     res_.tris = b.tris;
     res_.triangles = hs[0];
     return Status::ok;

@@ -26,6 +26,10 @@ struct RunStats
     uint32_t leaf_levels; // levels handled inside LDS leaf blocks
     uint32_t block_level; // first level of the LDS blocks (0 = whole set)
     double device_ms;     // first kernel start .. last kernel end
+    // with profiling on: time per stage (ms) and the merge launches
+    double stage_ms[8];   // keys+sort y, sort x, dedupe, frame-1 sort, kd levels, leaves, merges, extract
+    double merge_launch_ms_sum;
+    uint32_t merge_launches;
 };
 
 enum class Status : int
@@ -62,6 +66,8 @@ public:
     void release();
     // Tests only: cap the LDS leaf-block size (0 = default).
     void set_block_points(int c) { block_points_ = c; }
+    // Records HIP events around every stage and every merge launch.
+    void set_profiling(bool on) { profiling_ = on; }
     // Device bytes currently held.
     size_t workspace_bytes() const { return ws_bytes_; }
 
@@ -75,6 +81,9 @@ private:
     size_t ws_bytes_ = 0;
     size_t cap_n_ = 0;
     int block_points_ = 0;
+    bool profiling_ = false;
+    std::vector<hipEvent_t> evs_; // profiling events, created on demand
+    hipEvent_t prof_event(size_t i);
     DeviceResult res_;
     RunStats st_{};
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;

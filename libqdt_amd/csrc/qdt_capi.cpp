@@ -85,6 +85,7 @@ struct qdt_t
     bool tris_on_device_only = false;
     bool pts_on_device_only = false;
     qdt_gpu_stats stats{};
+    bool profiling = false;
     // constraints accepted so far (pairs of representative indices)
     std::set<std::pair<size_t, size_t>> cons;
 
@@ -111,56 +112,55 @@ void reset_context(qdt_t* Q) noexcept
 
 bool finite_pt(const qdt_vec2d& p) { return std::isfinite(p.x) && std::isfinite(p.y); }
 
-// Copies pts into dst and reports whether all coordinates are finite; large
-// inputs are split over host threads.
-bool copy_and_check(const qdt_vec2d* src, size_t n, qdt_vec2d* dst)
+// Runs f(begin, end) over [0, n) on up to 16 host threads (one for small n).
+template <class F>
+void parallel_ranges(size_t n, F f)
 {
     const size_t kChunk = 1 << 20;
     unsigned nt = std::thread::hardware_concurrency();
     nt = std::min<unsigned>(std::max(1u, nt), 16u);
     if (n < 2 * kChunk || nt == 1)
     {
-        bool ok = true;
-        for (size_t i = 0; i < n; ++i)
-        {
-            dst[i] = src[i];
-            ok &= finite_pt(src[i]);
-        }
-        return ok;
+        f((size_t)0, n);
+        return;
     }
-    std::atomic<bool> ok{ true };
     std::vector<std::thread> th;
     const size_t per = (n + nt - 1) / nt;
+    size_t done = 0;
     try
     {
         for (unsigned t = 0; t < nt; ++t)
         {
             const size_t a = t * per, b = std::min(n, a + per);
             if (a >= b) break;
-            th.emplace_back([=, &ok]() {
-                bool good = true;
-                for (size_t i = a; i < b; ++i)
-                {
-                    dst[i] = src[i];
-                    good &= finite_pt(src[i]);
-                }
-                if (!good) ok = false;
-            });
+            th.emplace_back(f, a, b);
+            done = b;
         }
     }
     catch (...)
     {
-        for (auto& t : th) t.join();
-        bool good = true;
-        for (size_t i = 0; i < n; ++i)
-        {
-            dst[i] = src[i];
-            good &= finite_pt(src[i]);
-        }
-        return good;
+        // could not start a thread: do the rest here
     }
     for (auto& t : th) t.join();
+    if (done < n) f(done, n);
+}
+
+// The reference scans every coordinate before doing anything else
+// (qdt.cpp:37-41); this check allocates nothing.
+bool all_finite(const qdt_vec2d* src, size_t n)
+{
+    std::atomic<bool> ok{ true };
+    parallel_ranges(n, [&](size_t a, size_t b) {
+        bool good = true;
+        for (size_t i = a; i < b; ++i) good &= finite_pt(src[i]);
+        if (!good) ok = false;
+    });
     return ok;
+}
+
+void copy_points(const qdt_vec2d* src, size_t n, qdt_vec2d* dst)
+{
+    parallel_ranges(n, [&](size_t a, size_t b) { std::memcpy(dst + a, src + a, (b - a) * sizeof(qdt_vec2d)); });
 }
 
 qdt_status map_status(qdt::Status s)
@@ -181,6 +181,7 @@ qdt_status ensure_engine(qdt_t* Q)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return QDT_ERR_INTERNAL;
     Q->eng = new (std::nothrow) qdt::Engine();
     if (!Q->eng) return QDT_ERR_OUT_OF_MEMORY;
+    Q->eng->set_profiling(Q->profiling);
     if (!Q->eng->stream())
     {
         delete Q->eng;
@@ -217,6 +218,9 @@ void record_stats(qdt_t* Q)
     Q->stats.triangles = s.triangles;
     Q->stats.kd_levels = s.levels;
     Q->stats.device_ms = s.device_ms;
+    for (int i = 0; i < 8; ++i) Q->stats.stage_ms[i] = s.stage_ms[i];
+    Q->stats.merge_launch_ms_sum = s.merge_launch_ms_sum;
+    Q->stats.merge_launches = s.merge_launches;
 }
 
 // Brings triangles and rep of the last device run into host memory.
@@ -280,11 +284,12 @@ qdt_status execute_host(qdt_t* Q, const qdt_vec2d* pts, size_t n)
             return st;
         }
         reset_context(Q);
-        if (!Q->pts.reserve(n)) throw std::bad_alloc();
-        if (!copy_and_check(pts, n, Q->pts.p))
-            return QDT_ERR_NOT_FINITE;
+        if (!all_finite(pts, n)) return QDT_ERR_NOT_FINITE;
         st = ensure_engine(Q);
-        if (st == QDT_OK) st = ensure_device_input(Q, n);
+        if (st != QDT_OK) return st;
+        if (!Q->pts.reserve(n)) throw std::bad_alloc();
+        copy_points(pts, n, Q->pts.p);
+        st = ensure_device_input(Q, n);
         if (st == QDT_OK &&
             hipMemcpyAsync(Q->d_in, Q->pts.p, n * sizeof(qdt_vec2d), hipMemcpyHostToDevice, Q->eng->stream()) !=
                 hipSuccess)
@@ -627,6 +632,13 @@ int qdt_gpu_get_stats(const qdt_t* ptr, qdt_gpu_stats* out)
     if (!ptr || !out) return 0;
     *out = ptr->stats;
     return 1;
+}
+
+void qdt_gpu_set_profiling(qdt_t* ptr, int enable)
+{
+    if (!ptr) return;
+    ptr->profiling = enable != 0;
+    if (ptr->eng) ptr->eng->set_profiling(ptr->profiling);
 }
 
 int qdt_gpu_device_count(void)

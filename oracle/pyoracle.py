@@ -178,6 +178,8 @@ def ref():
         R.ref_orient2d.argtypes = [d] * 6
         R.ref_incircle.restype = d
         R.ref_incircle.argtypes = [d] * 8
+        R.ref_copy_triangles.restype = sz
+        R.ref_copy_triangles.argtypes = [vp, vp]
         _ref = R
     return _ref
 
@@ -197,10 +199,8 @@ def ref_triangulate(pts, parallel=False, timed=False):
             raise RuntimeError("reference returned status %d" % st)
         nt = R.qdt_triangles_size(q)
         out = np.zeros((nt, 3), np.uint32)
-        T = (ctypes.c_size_t * 3)()
-        for i in range(nt):
-            R.qdt_get_triangle(q, T, i)
-            out[i] = (T[0], T[1], T[2])
+        if nt:
+            R.ref_copy_triangles(q, out.ctypes.data)
         used = np.array([R.qdt_point_is_used(q, i) for i in range(n)], np.uint8) if n <= 200000 else None
         return (out, used, dt) if timed else (out, used)
     finally:

@@ -109,3 +109,57 @@ def test_degenerate_mix(gpu):
 
 def test_device_input(gpu):
     check_same("device input 50k", W.uniform(50000, 7), device=True)
+
+
+# ---------------------------------------------------------------------------
+# golden fixtures written by the compiled reference
+# ---------------------------------------------------------------------------
+
+import json  # noqa: E402
+import os  # noqa: E402
+
+import _fixture_inputs  # noqa: E402
+
+FIX = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fixtures.json")))
+
+
+def test_fixtures_on_gpu(gpu):
+    for name, pts in _fixture_inputs.inputs().items():
+        print("fixture:", name, len(pts), flush=True)
+        case = FIX["cases"][name]
+        got, used, _ = run_gpu(pts)
+        assert len(got) == case["triangles"], name
+        assert "%016x" % O.canon_hash(got) == case["fnv"], name
+        if case.get("used") is not None and used is not None:
+            assert used.tolist() == case["used"], name
+
+
+def test_c2_uniform_10M_matches_reference(gpu):
+    name = "C2 uniform 10M seed 1"
+    pts = _fixture_inputs.large_inputs()[name]
+    print("fixture:", name, flush=True)
+    got, _, st = run_gpu(pts, device=True)
+    assert len(got) == FIX["cases"][name]["triangles"]
+    assert "%016x" % O.canon_hash(got) == FIX["cases"][name]["fnv"]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not present")
+def test_c2_gaussian_10M_matches_reference_live(gpu):
+    pts = W.gaussian(10_000_000, 1)
+    print("case: C2 gaussian 10M vs reference threaded", flush=True)
+    got, _, _ = run_gpu(pts, device=True)
+    want, _ = O.ref_triangulate(pts, parallel=True)
+    assert len(got) == len(want)
+    assert O.canon_hash(got) == O.canon_hash(want)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not present")
+def test_c3_degenerate_matches_reference_live(gpu):
+    # 2000x2000 lattice (step 1e-3) + 200k duplicates + 1M jittered circle points
+    pts = W.lattice_dups_circle()
+    print("case: C3 degenerate 5.2M vs reference threaded", len(pts), flush=True)
+    got, _, st = run_gpu(pts, device=True)
+    want, _ = O.ref_triangulate(pts, parallel=True)
+    assert len(got) == len(want)
+    assert O.canon_hash(got) == O.canon_hash(want)
+    assert st["incircle_exact"] > 0 and st["orient_exact"] > 0

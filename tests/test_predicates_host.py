@@ -1,0 +1,87 @@
+"""The GPU predicate code (libqdt_amd/csrc/predicates.hpp) compiled for the
+host: its exact paths (62-bit integer fast path and the wide accumulator)
+against exact rational arithmetic and the oracle, including exponent spreads
+of 2^2000 and subnormals."""
+import ctypes
+import os
+import random
+from fractions import Fraction as F
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "native", "libpredhost.so")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(SO), reason="host predicate build missing")
+
+
+def lib():
+    L = ctypes.CDLL(SO)
+    d = ctypes.c_double
+    for f, na in (("ph_orient2d_exact", 6), ("ph_orient2d_general", 6), ("ph_incircle_exact", 8),
+                  ("ph_incircle_general", 8)):
+        getattr(L, f).restype = ctypes.c_int
+        getattr(L, f).argtypes = [d] * na
+    L.ph_orient2d.restype = d
+    L.ph_orient2d.argtypes = [d] * 6 + [ctypes.POINTER(ctypes.c_uint)]
+    L.ph_incircle.restype = d
+    L.ph_incircle.argtypes = [d] * 8 + [ctypes.POINTER(ctypes.c_uint)]
+    return L
+
+
+def sgn(x):
+    return (x > 0) - (x < 0)
+
+
+def orient_true(a, b, c):
+    ax, ay, bx, by, cx, cy = map(F, (a[0], a[1], b[0], b[1], c[0], c[1]))
+    return sgn((ax - cx) * (by - cy) - (ay - cy) * (bx - cx))
+
+
+def incircle_true(a, b, c, d):
+    ax, ay, bx, by, cx, cy, dx, dy = map(F, (a[0], a[1], b[0], b[1], c[0], c[1], d[0], d[1]))
+    adx, ady, bdx, bdy, cdx, cdy = ax - dx, ay - dy, bx - dx, by - dy, cx - dx, cy - dy
+    return sgn((adx * adx + ady * ady) * (bdx * cdy - cdx * bdy) + (bdx * bdx + bdy * bdy) * (cdx * ady - adx * cdy)
+               + (cdx * cdx + cdy * cdy) * (adx * bdy - bdx * ady))
+
+
+def points(rng, mode):
+    if mode == 0:
+        return [(float(rng.randint(-10, 10)), float(rng.randint(-10, 10))) for _ in range(4)]
+    if mode == 1:
+        return [(rng.random(), rng.random()) for _ in range(4)]
+    if mode == 2:  # wildly different exponents, subnormals included
+        return [(rng.uniform(-1, 1) * 2.0 ** rng.randint(-1074, 1000), rng.uniform(-1, 1) * 2.0 ** rng.randint(-1074, 1000))
+                for _ in range(4)]
+    if mode == 3:  # lattice spacing 1e-3: collinear and cocircular ties
+        return [(rng.randint(0, 2000) * 1e-3, rng.randint(0, 2000) * 1e-3) for _ in range(4)]
+    e = rng.randint(-60, 60)
+    return [(rng.randint(-3, 3) * 2.0 ** e, rng.randint(-3, 3) * 2.0 ** e) for _ in range(4)]
+
+
+def test_exact_paths_against_rationals():
+    L = lib()
+    rng = random.Random(1)
+    for it in range(15000):
+        p = points(rng, it % 5)
+        if it % 7 == 0:
+            p[2] = (p[1][0], p[1][1] + (p[1][1] - p[0][1]))
+        a, b, c, d = p
+        t = orient_true(a, b, c)
+        assert L.ph_orient2d_exact(*a, *b, *c) == t
+        assert L.ph_orient2d_general(*a, *b, *c) == t
+        t = incircle_true(a, b, c, d)
+        assert L.ph_incircle_exact(*a, *b, *c, *d) == t
+        assert L.ph_incircle_general(*a, *b, *c, *d) == t
+
+
+def test_filtered_matches_exact_and_counts():
+    L = lib()
+    rng = random.Random(2)
+    cnt = ctypes.c_uint(0)
+    for it in range(5000):
+        p = points(rng, 0 if it % 2 else 1)
+        a, b, c, d = p
+        assert sgn(L.ph_orient2d(*a, *b, *c, ctypes.byref(cnt))) == orient_true(a, b, c)
+        assert sgn(L.ph_incircle(*a, *b, *c, *d, ctypes.byref(cnt))) == incircle_true(a, b, c, d)
+    assert cnt.value > 0  # the small-integer half sends ties to the exact path
