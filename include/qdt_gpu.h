@@ -34,6 +34,10 @@ typedef struct qdt_gpu_stats
     double stage_ms[8];           /* keys+y sort, x sort, dedupe, frame-1 sort, kd levels, leaves, merges, extract */
     double merge_launch_ms_sum;   /* sum of the merge kernel launches' durations */
     uint64_t merge_launches;
+    uint64_t seam_retries;        /* runs repeated without seam pieces */
+    uint64_t seam_counters[9];    /* anchors certified, inserted, pieces run, ended early, conflicts,
+                                     log overflows, anchors with a clean dry run, anchors selected,
+                                     vertex rings rewritten by anchor insertion */
 } qdt_gpu_stats;
 
 /* Like qdt_execute(), but d_pts points to n points in device memory.  The
@@ -56,6 +60,10 @@ QDT_API int qdt_gpu_get_stats(const qdt_t* ptr, qdt_gpu_stats* out);
 
 /* Turns per-stage HIP event timing on or off for a context (default off). */
 QDT_API void qdt_gpu_set_profiling(qdt_t* ptr, int enable);
+
+/* Cuts large merges into seam pieces that run in parallel (default 1); 0
+ * merges every kd node with one thread.  Results are identical either way. */
+QDT_API void qdt_gpu_set_seams(qdt_t* ptr, int enable);
 
 /* Number of HIP devices visible to the library (0 without a GPU). */
 QDT_API int qdt_gpu_device_count(void);

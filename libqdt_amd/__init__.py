@@ -37,7 +37,7 @@ EXPORTS = [
 ]
 GPU_EXPORTS = [
     "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
-    "qdt_gpu_set_profiling", "qdt_gpu_device_count", "qdt_gpu_release_workspace",
+    "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_device_count", "qdt_gpu_release_workspace",
 ]
 
 
@@ -56,13 +56,17 @@ class GpuStats(ctypes.Structure):
         ("extract_exact", ctypes.c_uint64), ("unique_points", ctypes.c_uint64),
         ("triangles", ctypes.c_uint64), ("kd_levels", ctypes.c_uint64), ("device_ms", ctypes.c_double),
         ("stage_ms", ctypes.c_double * 8), ("merge_launch_ms_sum", ctypes.c_double),
-        ("merge_launches", ctypes.c_uint64),
+        ("merge_launches", ctypes.c_uint64), ("seam_retries", ctypes.c_uint64),
+        ("seam_counters", ctypes.c_uint64 * 9),
     ]
     STAGES = ("keys_sort_y", "sort_x", "dedupe", "frame1_sort", "kd_levels", "leaves", "merges", "extract")
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stage_ms"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("stage_ms", "seam_counters")}
         d["stage_ms"] = dict(zip(self.STAGES, list(self.stage_ms)))
+        d["seam"] = dict(zip(("certified", "inserted", "pieces", "ended_early", "conflicts", "log_overflow",
+                                  "dry_run_ok", "selected", "rings_rebuilt"),
+                             list(self.seam_counters)))
         return d
 
 
@@ -113,6 +117,8 @@ def lib():
         L.qdt_gpu_get_stats.argtypes = [vp, ctypes.POINTER(GpuStats)]
         L.qdt_gpu_set_profiling.restype = None
         L.qdt_gpu_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.qdt_gpu_set_seams.restype = None
+        L.qdt_gpu_set_seams.argtypes = [vp, ctypes.c_int]
         L.qdt_gpu_device_count.restype = ctypes.c_int
         L.qdt_gpu_device_count.argtypes = []
         L.qdt_gpu_release_workspace.restype = None
@@ -231,6 +237,9 @@ class Triangulation:
 
     def set_profiling(self, on=True):
         lib().qdt_gpu_set_profiling(self._h, 1 if on else 0)
+
+    def set_seams(self, on=True):
+        lib().qdt_gpu_set_seams(self._h, 1 if on else 0)
 
     def release_workspace(self):
         lib().qdt_gpu_release_workspace(self._h)

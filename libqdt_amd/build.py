@@ -15,7 +15,7 @@ SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib", "libqdt_amd.so")
 
 SOURCES = ["engine.hip", "qdt_capi.cpp"]
-HEADERS = ["engine.hpp", "predicates.hpp", "primitives.hpp", "quadedge.hpp"]
+HEADERS = ["engine.hpp", "predicates.hpp", "primitives.hpp", "quadedge.hpp", "seam.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

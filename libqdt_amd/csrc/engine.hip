@@ -23,8 +23,10 @@
 #include "engine.hpp"
 #include "primitives.hpp"
 #include "quadedge.hpp"
+#include "seam.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -233,12 +235,14 @@ __global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const 
 
 using GQE = QuadEdges<int*, const double2*>;
 
-__global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const double2* PT, const uint32_t* B0,
-                                                        const uint32_t* B1, const int* off, int D, uint32_t m,
-                                                        int4* res, int* err, unsigned long long* counters)
+__global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const double2* PT, int* VE,
+                                                        const uint32_t* B0, const uint32_t* B1, const int* off, int D,
+                                                        uint32_t m, int lim_e, int4* res, int* err,
+                                                        unsigned long long* counters)
 {
     const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
-    GQE q{ NX, OR, PT, (int)(12 * m), (int)m, 0, {} };
+    GQE q;
+    q.init(NX, OR, PT, VE, lim_e, (int)m);
     if (j < (1u << D))
     {
         const int* oD = off + off_base(D);
@@ -270,11 +274,13 @@ __global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const 
     if (q.err) atomicOr(err, 2 << q.err);
 }
 
-__global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const double2* PT, const int* off, int d,
-                                                       uint32_t m, int4* res, int* err, unsigned long long* counters)
+__global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const double2* PT, int* VE, const int* off,
+                                                       int d, uint32_t m, int lim_e, int4* res, int* err,
+                                                       unsigned long long* counters)
 {
     const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
-    GQE q{ NX, OR, PT, (int)(12 * m), (int)m, 0, {} };
+    GQE q;
+    q.init(NX, OR, PT, VE, lim_e, (int)m);
     if (j < (1u << d))
     {
         const int* od = off + off_base(d);
@@ -293,6 +299,597 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
     }
     flush_counts(counters, q.pc);
     if (q.err) atomicOr(err, 2 << q.err);
+}
+
+
+// ---------------------------------------------------------------------------
+// 7b: large merges cut into seam pieces (see seam.hpp)
+// ---------------------------------------------------------------------------
+
+constexpr int kSegMaxK = 64;           // anchors per node at most
+constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
+constexpr int kSegMin = 8192;          // smaller nodes merge in one piece
+constexpr int kLogCap = 2048;          // vertex log entries per piece
+constexpr int kSegBlock = 4;           // pieces per block (their logs sit in LDS)
+constexpr int kPhases = 3;             // pieces k, k+3, ... run together
+constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit merges
+
+// split vertex of every internal node; anchor heights of segmented levels
+__global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const uint32_t* Bo, const int* off, int d,
+                                                      int K, int* splitv, int* qv)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= (1u << d)) return;
+    const int* od = off + off_base(d);
+    const int o = od[j];
+    const int s = od[j + 1] - o;
+    if (s < 4) return;
+    const size_t ni = node_index(d, j);
+    splitv[ni] = (int)Bf[o + left_size(s)];
+    for (int k = 0; k < K; ++k) qv[ni * kSegMaxK + k] = (int)Bo[o + (int)(((long long)(2 * k + 1) * s) / (2 * K))];
+}
+
+__device__ __forceinline__ bool less_frame_pt(double2 a, double2 b, int frame)
+{
+    if (frame == 0) return a.x < b.x || (a.x == b.x && a.y < b.y);
+    return a.y < b.y || (a.y == b.y && a.x > b.x);
+}
+
+// a vertex of the kd leaf whose cell contains p, below node (dd, jj)
+__device__ int kd_descend(const int* off, const int* splitv, const double2* PT, const uint32_t* B0, int D, int dd,
+                          uint32_t jj, double2 p)
+{
+    while (dd < D)
+    {
+        const int* od = off + off_base(dd);
+        const int s = od[jj + 1] - od[jj];
+        if (s <= 3) break;
+        const double2 sp = PT[splitv[node_index(dd, jj)]];
+        jj = 2 * jj + (less_frame_pt(p, sp, dd & 1) ? 0u : 1u);
+        ++dd;
+    }
+    const int* od = off + off_base(dd);
+    const int o = od[jj];
+    return od[jj + 1] > o ? (int)B0[o] : -1;
+}
+
+constexpr int kCross = 16;                     // crossing edges per side of an anchor at most
+constexpr int kRec = 6 + 2 * kCross;           // ints per anchor record: u v xu xv nl nr crossings
+
+__global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, const double2* PT, int* VE,
+                                                            const int* off, const int* splitv, const int* qv,
+                                                            const uint32_t* B0, int D, int d, int K, uint32_t m,
+                                                            int lim_e, int* arec, double4* acoord, double* akey,
+                                                            int* astate, unsigned long long* counters)
+{
+    const uint32_t t = blockIdx.x * kMergeBlock + threadIdx.x;
+    GQE q;
+    q.init(NX, OR, PT, VE, lim_e, (int)m);
+    if (t < (1u << d) * (uint32_t)K)
+    {
+        const uint32_t j = t / K;
+        const int k = (int)(t % K);
+        int state = 0;
+        const int* od = off + off_base(d);
+        const int s = od[j + 1] - od[j];
+        if (s >= 4)
+        {
+            const int axis = d & 1;
+            const size_t ni = node_index(d, j);
+            const double c = frame_x(PT[splitv[ni]], axis);
+            const double Y = frame_y(PT[qv[ni * kSegMaxK + k]], axis);
+            double2 p = from_frame(c, Y, axis);
+            const int sL = kd_descend(off, splitv, PT, B0, D, d + 1, 2 * j, p);
+            const int sR = kd_descend(off, splitv, PT, B0, D, d + 1, 2 * j + 1, p);
+            int eu = -1, ev = -1;
+            int u = sL >= 0 ? nn_walk(q, sL, p, 512, eu) : -1;
+            int v = sR >= 0 ? nn_walk(q, sR, p, 512, ev) : -1;
+            for (int it = 0; it < 8 && u >= 0 && v >= 0 && !q.err; ++it)
+            {
+                const double2 U = PT[u], V = PT[v];
+                const double ux = frame_x(U, axis), uy = frame_y(U, axis);
+                const double vx = frame_x(V, axis), vy = frame_y(V, axis);
+                if (!(vx > ux)) break;
+                const double X = ((vx - ux) * (vx + ux) + (Y - vy) * (Y - vy) - (Y - uy) * (Y - uy)) / (2.0 * (vx - ux));
+                p = from_frame(X, Y, axis);
+                int eu2 = -1, ev2 = -1;
+                const int u2 = nn_walk(q, u, p, 512, eu2);
+                const int v2 = nn_walk(q, v, p, 512, ev2);
+                const bool same = u2 == u && v2 == v;
+                u = u2;
+                v = v2;
+                eu = eu2;
+                ev = ev2;
+                if (same) break;
+            }
+            if (u >= 0 && v >= 0 && eu >= 0 && ev >= 0 && !q.err && certify(q, u, eu, v, ev))
+            {
+                atomicAdd(&counters[5], 1ull);
+                // dry run of the insertion on the unmerged halves
+                int* r = arec + (size_t)t * kRec;
+                const int xu = find_wedge(q, u, eu, v);
+                const int xv = xu >= 0 ? find_wedge(q, v, ev, u) : -1;
+                const int nl = xv >= 0 ? collect_crossings(q, u, xu, v, r + 6, kCross) : -1;
+                const int nr = nl >= 0 ? collect_crossings(q, v, xv, u, r + 6 + kCross, kCross) : -1;
+                if (nr >= 0 && !q.err)
+                {
+                    r[0] = u;
+                    r[1] = v;
+                    r[2] = xu;
+                    r[3] = xv;
+                    r[4] = nl;
+                    r[5] = nr;
+                    const double2 U = PT[u], V = PT[v];
+                    acoord[t] = make_double4(U.x, U.y, V.x, V.y);
+                    // height where the anchor crosses the cut line: the sort key
+                    const double ux = frame_x(U, axis), uy = frame_y(U, axis);
+                    const double vx = frame_x(V, axis), vy = frame_y(V, axis);
+                    akey[t] = vx > ux ? uy + (c - ux) / (vx - ux) * (vy - uy) : 0.5 * (uy + vy);
+                    state = 1;
+                    atomicAdd(&counters[11], 1ull);
+                }
+            }
+        }
+        astate[t] = state;
+    }
+    flush_counts(counters, q.pc);
+}
+
+// lower common tangent of every segmented node (the first cut)
+__global__ __launch_bounds__(kMergeBlock) void k_seg_tangent(int* NX, int* OR, const double2* PT, int* VE,
+                                                             const int* off, int d, uint32_t m, int lim_e,
+                                                             unsigned* spare, int spare_base, int spare_cap,
+                                                             const int4* res, int* segstart, int4* nstate, int* err,
+                                                             unsigned long long* counters)
+{
+    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    GQE q;
+    q.init(NX, OR, PT, VE, lim_e, (int)m);
+    q.spare = spare;
+    q.spare_base = spare_base;
+    q.spare_cap = spare_cap;
+    if (j < (1u << d))
+    {
+        const int* od = off + off_base(d);
+        const int s = od[j + 1] - od[j];
+        if (s >= 4)
+        {
+            const int4 L = res[node_index(d + 1, 2 * j)];
+            const int4 R = res[node_index(d + 1, 2 * j + 1)];
+            Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
+            int ldo = L.x, rdo = R.y;
+            int budget = 16 * s + 64;
+            segstart[(size_t)j * kSegSlots] = q.tangent(ldo, L.y, R.x, rdo, pool, budget);
+            nstate[j] = make_int4(ldo, rdo, pool.head, pool.tail);
+        }
+    }
+    flush_counts(counters, q.pc);
+    if (q.err) atomicOr(err, 2 << q.err);
+}
+
+// Anchors in chain order, each strictly above the previous cut, and no two
+// cuts with a common end vertex (coordinates are cached in the records: no
+// pointer chasing here).  The tangent's rings changed after the anchors' dry
+// runs, so its two vertices are excluded as anchor ends too.
+__global__ __launch_bounds__(kMergeBlock) void k_seg_select(const int* OR, const double2* PT, const int* off, int d,
+                                                            int K, uint32_t m, int lim_e, const int* segstart,
+                                                            const int* arec,
+                                                            const double4* acoord, const double* akey, int* astate,
+                                                            int* selk, int* nsel, int* alist_n,
+                                                            unsigned long long* counters)
+{
+    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    if (j == 0) *alist_n = 0;
+    if (j >= (1u << d)) return;
+    nsel[j] = 0;
+    const int* od = off + off_base(d);
+    const int s = od[j + 1] - od[j];
+    int cnt = 0;
+    if (s >= 4)
+    {
+        const int b0 = segstart[(size_t)j * kSegSlots];
+        if (b0 < 0 || b0 >= lim_e) return; // the tangent failed: the run's error flag is set
+        int pu = OR[(b0 ^ 2) >> 1], pv = OR[b0 >> 1];
+        if ((unsigned)pu >= m || (unsigned)pv >= m) return;
+        int used[2 * kSegSlots];
+        int nused = 0;
+        used[nused++] = pu;
+        used[nused++] = pv;
+        double2 PU = PT[pu], PV = PT[pv];
+        unsigned dummy = 0;
+        // candidates by crossing height (insertion sort, K is small); the
+        // exact tests below drop any that the rounded keys misorder
+        int order[kSegMaxK];
+        double keys[kSegMaxK];
+        int nc = 0;
+        for (int k = 0; k < K; ++k)
+        {
+            const size_t t = (size_t)j * K + k;
+            if (astate[t] != 1) continue;
+            const double key = akey[t];
+            int i = nc++;
+            while (i > 0 && keys[i - 1] > key)
+            {
+                keys[i] = keys[i - 1];
+                order[i] = order[i - 1];
+                --i;
+            }
+            keys[i] = key;
+            order[i] = (int)t;
+        }
+        for (int i = 0; i < nc; ++i)
+        {
+            const size_t t = (size_t)order[i];
+            const int* r = arec + t * kRec;
+            const int u = r[0], v = r[1];
+            const double4 c = acoord[t];
+            bool ok = true;
+            for (int q = 0; q < nused && ok; ++q) ok = used[q] != u && used[q] != v;
+            ok = ok && orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
+            ok = ok && orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
+            if (!ok)
+            {
+                astate[t] = 0;
+                continue;
+            }
+            astate[t] = 2;
+            used[nused++] = u;
+            used[nused++] = v;
+            selk[(size_t)j * kSegSlots + cnt++] = (int)t;
+            atomicAdd(&counters[12], 1ull);
+            pu = u;
+            pv = v;
+            PU = make_double2(c.x, c.y);
+            PV = make_double2(c.z, c.w);
+        }
+    }
+    nsel[j] = cnt;
+}
+
+// Parallel anchor insertion.  Several anchors may cross the same edge (long
+// edges along a hull are crossed by many), so insertion is done per vertex
+// rather than per anchor: every edge crossed by a selected anchor is marked
+// for deletion (del[record] = the anchor that frees it), every vertex whose
+// ring changes is listed once, and one thread per listed vertex rewrites its
+// ring: the old cyclic order with the marked edges dropped and the new anchor
+// edge placed right after its wedge edge.  A ring fixes the onext of its own
+// directed edges and of their rotations (rot(e).onext = invrot(oprev(e))),
+// so rings of distinct vertices touch disjoint words of NX.
+
+// anchor edge records, vertex marks for u and v (owner[w] = anchor index)
+__global__ __launch_bounds__(kMergeBlock) void k_seg_mark(int* OR, int total, const int* arec, const int* astate,
+                                                          unsigned* spare, int spare_base, int spare_cap, int* owner,
+                                                          int* aedge, int* alist, int* alist_n, int acap, int* err,
+                                                          unsigned long long* counters)
+{
+    const int t = blockIdx.x * kMergeBlock + threadIdx.x;
+    if (t >= total || astate[t] != 2) return;
+    const int* r = arec + (size_t)t * kRec;
+    const unsigned k = atomicAdd(spare, 1u);
+    if ((int)k >= spare_cap)
+    {
+        atomicOr(err, kSeamErr);
+        aedge[t] = -1;
+        return;
+    }
+    const int e = 4 * (spare_base + (int)k); // e runs v -> u
+    OR[e >> 1] = r[1];
+    OR[(e ^ 2) >> 1] = r[0];
+    aedge[t] = e;
+    owner[r[0]] = t;
+    owner[r[1]] = t;
+    const int at = atomicAdd(alist_n, 2);
+    if (at + 2 <= acap)
+    {
+        alist[at] = r[0];
+        alist[at + 1] = r[1];
+    }
+    else
+        atomicOr(err, kSeamErr);
+    atomicAdd(&counters[6], 1ull);
+}
+
+// crossed edges: deletion marks and the rings of their endpoints
+__global__ __launch_bounds__(kMergeBlock) void k_seg_mark_cross(const int* OR, int total, const int* arec,
+                                                                const int* astate, const int* aedge, uint32_t m,
+                                                                int lim_e, int* owner, int* del, int* alist,
+                                                                int* alist_n, int acap, int* err)
+{
+    const int t = blockIdx.x * kMergeBlock + threadIdx.x;
+    if (t >= total || astate[t] != 2 || aedge[t] < 0) return;
+    const int* r = arec + (size_t)t * kRec;
+    for (int side = 0; side < 2; ++side)
+    {
+        const int c = r[4 + side];
+        const int* ce = r + 6 + side * kCross;
+        for (int i = 0; i < c; ++i)
+        {
+            const int e = ce[i];
+            if (e < 0 || e >= lim_e)
+            {
+                atomicOr(err, kSeamErr);
+                continue;
+            }
+            atomicCAS(&del[e >> 2], -1, t);
+            const int ends[2] = { OR[e >> 1], OR[(e ^ 2) >> 1] };
+            for (int w : ends)
+            {
+                if ((unsigned)w >= m) continue;
+                if (atomicCAS(&owner[w], -1, -3) == -1)
+                {
+                    const int at = atomicAdd(alist_n, 1);
+                    if (at < acap)
+                        alist[at] = w;
+                    else
+                        atomicOr(err, kSeamErr);
+                }
+            }
+        }
+    }
+}
+
+// one thread per listed vertex: its new ring
+__global__ __launch_bounds__(kMergeBlock) void k_seg_rebuild(int* NX, const int* OR, int* VE, uint32_t m, int lim_e,
+                                                             const int* arec, const int* aedge, int* owner,
+                                                             const int* del, const int* alist, const int* alist_n,
+                                                             int acap, int* err, unsigned long long* counters)
+{
+    const int n = min(*alist_n, acap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[13], (unsigned long long)n);
+    for (int i = blockIdx.x * kMergeBlock + threadIdx.x; i < n; i += gridDim.x * kMergeBlock)
+    {
+        const int w = alist[i];
+        if ((unsigned)w >= m) continue;
+        const int t = owner[w];
+        int ins = -1, after = -1;
+        if (t >= 0)
+        {
+            const int* r = arec + (size_t)t * kRec;
+            const int e = aedge[t];
+            if (w == r[0])
+            {
+                ins = e ^ 2; // u -> v
+                after = r[2];
+            }
+            else
+            {
+                ins = e; // v -> u
+                after = r[3];
+            }
+        }
+        const int e0 = VE[w];
+        if (e0 < 0 || e0 >= lim_e)
+        {
+            atomicOr(err, kSeamErr);
+            continue;
+        }
+        int first = -1, prev = -1, cur = e0, guard = 0;
+        bool placed = ins < 0;
+        do
+        {
+            const int nxt = NX[cur]; // read before any write to NX[cur]
+            if (del[cur >> 2] == -1)
+            {
+                if (prev >= 0)
+                {
+                    NX[prev] = cur;
+                    NX[e_rot(cur)] = e_invrot(prev);
+                }
+                else
+                    first = cur;
+                prev = cur;
+            }
+            if (cur == after)
+            {
+                if (prev >= 0)
+                {
+                    NX[prev] = ins;
+                    NX[e_rot(ins)] = e_invrot(prev);
+                }
+                else
+                    first = ins;
+                prev = ins;
+                placed = true;
+            }
+            cur = nxt;
+            if (cur < 0 || cur >= lim_e) break;
+        } while (cur != e0 && ++guard < 65536);
+        if (cur != e0 || !placed || first < 0)
+        {
+            atomicOr(err, kSeamErr); // malformed ring, wedge edge gone, or an isolated vertex
+            owner[w] = -1;
+            continue;
+        }
+        NX[prev] = first;
+        NX[e_rot(first)] = e_invrot(prev);
+        VE[w] = first;
+        owner[w] = -1;
+    }
+}
+
+// freed records onto the private list of the anchor that marked them first
+__global__ __launch_bounds__(kMergeBlock) void k_seg_pools(int* NX, int* OR, int total, const int* arec,
+                                                           const int* astate, const int* aedge, int lim_e, int* del,
+                                                           int2* apool)
+{
+    const int t = blockIdx.x * kMergeBlock + threadIdx.x;
+    if (t >= total || astate[t] != 2) return;
+    int head = -1, tail = -1;
+    if (aedge[t] >= 0)
+    {
+        const int* r = arec + (size_t)t * kRec;
+        for (int side = 0; side < 2; ++side)
+        {
+            const int c = r[4 + side];
+            const int* ce = r + 6 + side * kCross;
+            for (int i = 0; i < c; ++i)
+            {
+                const int e = ce[i];
+                if (e < 0 || e >= lim_e) continue;
+                const int k = e >> 2;
+                if (del[k] != t) continue;
+                del[k] = -1;
+                OR[2 * k] = -1;
+                NX[4 * k] = head;
+                if (head < 0) tail = k;
+                head = k;
+            }
+        }
+    }
+    apool[t] = make_int2(head, tail);
+}
+
+// cut list of every node: the tangent, then the inserted anchors in order
+__global__ __launch_bounds__(kMergeBlock) void k_seg_compact(int d, const int* selk, const int* nsel,
+                                                             const int* astate, const int* aedge, int* segstart,
+                                                             int* nseg, const int* off)
+{
+    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    if (j >= (1u << d)) return;
+    const int* od = off + off_base(d);
+    const int s = od[j + 1] - od[j];
+    int cnt = 0;
+    if (s >= 4)
+    {
+        cnt = 1;
+        const int ns = nsel[j];
+        for (int i = 0; i < ns; ++i)
+        {
+            const int t = selk[(size_t)j * kSegSlots + i];
+            if (astate[t] == 2 && aedge[t] >= 0) segstart[(size_t)j * kSegSlots + cnt++] = aedge[t];
+        }
+    }
+    nseg[j] = cnt;
+}
+
+__global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const double2* PT, int* VE, const int* off,
+                                                      int d, uint32_t m, int lim_e, unsigned* spare, int spare_base,
+                                                      int spare_cap, const int* segstart, const int* nseg,
+                                                      int parity, int K, int2* segpool, int* vlog, int* vlog_n,
+                                                      int* err, unsigned long long* counters)
+{
+    __shared__ int logs[kSegBlock][kLogCap];
+    const uint32_t t = blockIdx.x * kSegBlock + threadIdx.x;
+    GQE q;
+    q.init(NX, OR, PT, VE, lim_e, (int)m);
+    q.spare = spare;
+    q.spare_base = spare_base;
+    q.spare_cap = spare_cap;
+    const uint32_t j = t / kSegSlots;
+    const int i = (int)(t % kSegSlots);
+    bool active = false;
+    if (j < (1u << d) && i % kPhases == parity)
+    {
+        const int ns = nseg[j];
+        if (i < ns)
+        {
+            active = true;
+            // with at most one piece per phase nothing can collide: no log
+            if (ns > kPhases)
+            {
+                q.vlog = logs[threadIdx.x];
+                q.logcap = kLogCap;
+            }
+            const int* od = off + off_base(d);
+            const int s = od[j + 1] - od[j];
+            const int* st = segstart + (size_t)j * kSegSlots;
+            const int basel = st[i];
+            const int stop = i + 1 < ns ? st[i + 1] : -1;
+            Pool pool = { -1, -1 };
+            int budget = 16 * s + 64;
+            const int r = q.bubble(basel, stop, pool, budget);
+            if (stop >= 0 && r != 1 && !q.err) q.err = 4;
+            atomicAdd(&counters[7], 1ull);
+            if (q.err == 4) atomicAdd(&counters[8], 1ull);
+            if (q.err == 5) atomicAdd(&counters[10], 1ull);
+            segpool[t] = make_int2(pool.head, pool.tail);
+        }
+    }
+    if (active)
+    {
+        const size_t slot = (size_t)j * (K + 1) + i;
+        int* g = vlog + slot * kLogCap;
+        for (int k = 0; k < q.nlog; ++k) g[k] = q.vlog[k];
+        vlog_n[slot] = q.nlog;
+    }
+    flush_counts(counters, q.pc);
+    if (q.err) atomicOr(err, q.err >= 4 ? kSeamErr : (2 << q.err));
+}
+
+// a vertex logged by two pieces of one phase means they raced: redo the run
+__global__ __launch_bounds__(kMergeBlock) void k_seg_check(const int* nseg, int parity, int K, const int* vlog,
+                                                           const int* vlog_n, int* owner, uint32_t m, int* err,
+                                                           int clear, unsigned long long* counters)
+{
+    const uint32_t t = blockIdx.x; // piece slot j * (K + 1) + i
+    const uint32_t j = t / (K + 1);
+    const int i = (int)(t % (K + 1));
+    if (i % kPhases != parity || i >= nseg[j]) return;
+    const int n = vlog_n[t];
+    const int* g = vlog + (size_t)t * kLogCap;
+    for (int k = threadIdx.x; k < n; k += kMergeBlock)
+    {
+        const int v = g[k];
+        if ((unsigned)v >= m) continue;
+        if (clear)
+            owner[v] = -1;
+        else
+        {
+            const int old = atomicCAS(&owner[v], -1, (int)t);
+            if (old != -1 && old != (int)t)
+            {
+                atomicOr(err, kSeamErr);
+                atomicAdd(&counters[9], 1ull);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kMergeBlock) void k_seg_final(int* NX, int* OR, const double2* PT, int* VE,
+                                                           const int* off, int d, int K, uint32_t m, int lim_e,
+                                                           const int* nseg, const int4* nstate, const int2* segpool,
+                                                           const int* astate, const int2* apool, int4* res, int* err,
+                                                           unsigned long long* counters)
+{
+    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    GQE q;
+    q.init(NX, OR, PT, VE, lim_e, (int)m);
+    if (j < (1u << d))
+    {
+        const int* od = off + off_base(d);
+        const int s = od[j + 1] - od[j];
+        if (s >= 4)
+        {
+            const int4 ns = nstate[j];
+            Pool pool = { ns.z, ns.w };
+            const int cnt = nseg[j];
+            for (int i = 0; i < cnt; ++i)
+            {
+                const int2 sp = segpool[(size_t)j * kSegSlots + i];
+                pool = q.cat(pool, Pool{ sp.x, sp.y });
+            }
+            for (int k = 0; k < K; ++k)
+            {
+                const size_t t = (size_t)j * K + k;
+                if (astate[t] != 2) continue;
+                const int2 ap = apool[t];
+                pool = q.cat(pool, Pool{ ap.x, ap.y });
+            }
+            int first, second;
+            q.rewalk(ns.x, ns.y, d & 1, d > 0 ? ((d - 1) & 1) : 0, s, first, second);
+            res[node_index(d, j)] = make_int4(first, second, pool.head, pool.tail);
+        }
+    }
+    flush_counts(counters, q.pc);
+    if (q.err) atomicOr(err, 2 << q.err);
+}
+
+// number of anchors for a level whose nodes hold about s points (0: one piece)
+int seam_anchors(double s)
+{
+    if (s < kSegMin) return 0;
+    int k = (int)(std::sqrt(s) / 20.0);
+    return k < 2 ? 2 : (k > kSegMaxK ? kSegMaxK : k);
 }
 
 // ---------------------------------------------------------------------------
@@ -318,20 +915,22 @@ __device__ __forceinline__ bool face_of(const int* NX, const int* OR, int e, int
 }
 
 __global__ __launch_bounds__(kBlock) void k_extract_test(const int* NX, const int* OR, const double2* PT,
-                                                         uint32_t nrec, uint32_t m, uint32_t* face, uint32_t* queue,
+                                                         uint32_t nrec, uint32_t m, int lim_e, const unsigned* spare,
+                                                         uint32_t spare_cap, uint32_t* face, uint32_t* queue,
                                                          uint32_t qcap, unsigned long long* counters, int* err)
 {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= nrec) return;
+    const uint32_t used = 3 * m + min(*spare, spare_cap);
     uint32_t fl = 0;
-    if (OR[2 * k] >= 0)
+    if (k < used && OR[2 * k] >= 0)
     {
 #pragma unroll
         for (int r = 0; r < 4; r += 2)
         {
             const int e = 4 * (int)k + r;
             int f, g, a, b, c;
-            if (!face_of(NX, OR, e, (int)(12 * m), (int)m, f, g, a, b, c)) continue;
+            if (!face_of(NX, OR, e, lim_e, (int)m, f, g, a, b, c)) continue;
             if (e > f || e > g) continue;
             const double2 A = PT[a], B = PT[b], C = PT[c];
             const double detleft = (A.x - C.x) * (B.y - C.y);
@@ -358,7 +957,7 @@ __global__ __launch_bounds__(kBlock) void k_extract_test(const int* NX, const in
 }
 
 __global__ __launch_bounds__(kBlock) void k_extract_exact(const int* NX, const int* OR, const double2* PT,
-                                                          uint32_t m, const uint32_t* queue,
+                                                          uint32_t m, int lim_e, const uint32_t* queue,
                                                           const unsigned long long* counters, uint32_t qcap,
                                                           uint32_t* face)
 {
@@ -367,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void k_extract_exact(const int* NX, const i
     {
         const int e = (int)queue[t];
         int f, g, a, b, c;
-        if (!face_of(NX, OR, e, (int)(12 * m), (int)m, f, g, a, b, c)) continue;
+        if (!face_of(NX, OR, e, lim_e, (int)m, f, g, a, b, c)) continue;
         const double2 A = PT[a], B = PT[b], C = PT[c];
         if (orient2d_exact(A.x, A.y, B.x, B.y, C.x, C.y) > 0) atomicOr(&face[e >> 2], 1u << ((e & 2) >> 1));
     }
@@ -387,6 +986,7 @@ struct FaceWrite
     const uint32_t* ID0;
     uint32_t* tris;
     uint32_t m;
+    int lim_e;
     __device__ void operator()(uint32_t k, uint32_t pre) const
     {
         const uint32_t fl = face[k] & 3u;
@@ -396,7 +996,7 @@ struct FaceWrite
             if (!(fl & (1u << (r >> 1)))) continue;
             const int e = 4 * (int)k + r;
             int f, g, a, b, c;
-            if (!face_of(NX, OR, e, (int)(12 * m), (int)m, f, g, a, b, c)) a = b = c = 0;
+            if (!face_of(NX, OR, e, lim_e, (int)m, f, g, a, b, c)) a = b = c = 0;
             tris[3 * pre] = ID0[a];
             tris[3 * pre + 1] = ID0[b];
             tris[3 * pre + 2] = ID0[c];
@@ -459,6 +1059,17 @@ struct Engine::Bufs
     int4* res;
     unsigned long long* counters;
     int* err;
+    // seam pieces
+    int *VE, *splitv, *qv, *segstart, *nseg, *vlog, *vlog_n, *owner;
+    int *arec, *astate, *aedge, *selk, *nsel, *alist, *alist_n;
+    double4* acoord;
+    double* akey;
+    int2 *apool, *segpool;
+    int4* nstate;
+    unsigned* spare;
+    size_t rec_cap;   // quad-edge records allocated (3N + spare)
+    size_t seg_nodes; // node slots of the segmented levels
+    size_t anchor_cap, log_slots;
     uint32_t qcap;
     int dcap;
     uint32_t* host_scalars; // pinned
@@ -526,8 +1137,21 @@ Status Engine::ensure(size_t n)
     const size_t nodes = ((size_t)2 << D) + 64;
     const size_t offs = ((size_t)2 << D) + (size_t)D + 64;
     const uint32_t nb_sort = cdiv(N, kSortTile);
-    const uint32_t nb_scan = cdiv(3 * N, kScanTile) + 1;
+    const size_t spare_recs = N / 8 + 65536;
+    const size_t R = 3 * N + spare_recs;
+    const uint32_t nb_scan = cdiv(R, kScanTile) + 1;
     const size_t qcap = std::max<size_t>(4096, N / 4);
+    // segmented levels: nodes of at least kSegMin points
+    size_t seg_nodes = 1, anchor_cap = 1, log_slots = 1;
+    for (int d = 0; d < 31; ++d)
+    {
+        const double sz = (double)N / (double)(1u << d);
+        const int K = seam_anchors(sz);
+        if (!K) break;
+        seg_nodes = (size_t)2 << d;
+        anchor_cap = std::max(anchor_cap, ((size_t)1 << d) * K);
+        log_slots = std::max(log_slots, ((size_t)1 << d) * (K + 1));
+    }
     struct Item
     {
         void** p;
@@ -540,13 +1164,26 @@ Status Engine::ensure(size_t n)
         { (void**)&b.dist, 4 * N },     { (void**)&b.ID0, 4 * N },      { (void**)&b.rep, 4 * N },
         { (void**)&b.B0, 4 * N },       { (void**)&b.B1, 4 * N },       { (void**)&b.Bt, 4 * N },
         { (void**)&b.nodeA, 4 * N },    { (void**)&b.nodeB, 4 * N },    { (void**)&b.Z, 4 * N },
-        { (void**)&b.tris, 4 * 6 * N }, { (void**)&b.face, 4 * 3 * N }, { (void**)&b.queue, 4 * qcap },
+        { (void**)&b.tris, 4 * 6 * N }, { (void**)&b.face, 4 * R },     { (void**)&b.queue, 4 * qcap },
         { (void**)&b.partials, 4 * (size_t)(nb_scan + 1 + 256 * nb_sort) },
         { (void**)&b.hist, 4 * 256 * (size_t)nb_sort },
         { (void**)&b.ghist, 4 * 8 * 256 },
-        { (void**)&b.P0, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.NX, 4 * 12 * N },
-        { (void**)&b.OR, 4 * 6 * N },   { (void**)&b.off, 4 * offs },   { (void**)&b.res, 16 * nodes },
-        { (void**)&b.counters, 8 * 8 }, { (void**)&b.err, 64 },
+        { (void**)&b.P0, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.NX, 4 * 4 * R },
+        { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.off, 4 * offs },   { (void**)&b.res, 16 * nodes },
+        { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
+        { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },    { (void**)&b.splitv, 4 * nodes },
+        { (void**)&b.qv, 4 * (size_t)kSegMaxK * seg_nodes },
+        { (void**)&b.segstart, 4 * (size_t)kSegSlots * seg_nodes },
+        { (void**)&b.nseg, 4 * seg_nodes }, { (void**)&b.nstate, 16 * seg_nodes },
+        { (void**)&b.segpool, 8 * (size_t)kSegSlots * seg_nodes },
+        { (void**)&b.arec, 4 * (size_t)kRec * anchor_cap }, { (void**)&b.acoord, 32 * anchor_cap },
+        { (void**)&b.akey, 8 * anchor_cap },
+        { (void**)&b.astate, 4 * anchor_cap }, { (void**)&b.aedge, 4 * anchor_cap },
+        { (void**)&b.apool, 8 * anchor_cap }, { (void**)&b.selk, 4 * (size_t)kSegSlots * seg_nodes },
+        { (void**)&b.nsel, 4 * seg_nodes },
+        { (void**)&b.alist, 4 * (2 + 4 * (size_t)kCross) * anchor_cap }, { (void**)&b.alist_n, 64 },
+        { (void**)&b.vlog, 4 * (size_t)kLogCap * log_slots }, { (void**)&b.vlog_n, 4 * log_slots },
+        { (void**)&b.spare, 64 },
     };
     size_t total = 0;
     for (const Item& it : items) total += (it.bytes + 255) & ~(size_t)255;
@@ -563,6 +1200,10 @@ Status Engine::ensure(size_t n)
     }
     b.qcap = (uint32_t)qcap;
     b.dcap = D;
+    b.rec_cap = R;
+    b.seg_nodes = seg_nodes;
+    b.anchor_cap = anchor_cap;
+    b.log_slots = log_slots;
     cap_n_ = N;
     return Status::ok;
 }
@@ -600,13 +1241,26 @@ Status Engine::run(const double* d_pts, size_t n)
     if (n == 0) return Status::ok;
     Status s = ensure(n);
     if (s != Status::ok) return s;
-    s = run_impl(d_pts, n);
+    bool redo = false;
+    s = run_impl(d_pts, n, seams_, redo);
+    if (s == Status::ok && redo)
+    {
+        // two seam pieces touched the same vertex: repeat with unsplit merges
+        const RunStats first = st_;
+        st_ = RunStats();
+        s = run_impl(d_pts, n, false, redo);
+        st_.seam_retries = 1;
+        st_.device_ms += first.device_ms;
+        for (int i = 0; i < 9; ++i) st_.seam[i] = first.seam[i];
+        st_.err_bits = first.err_bits;
+    }
     if (s != Status::ok) res_ = DeviceResult();
     return s;
 }
 
-Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
+Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& redo)
 {
+    redo = false;
     Bufs& b = *b_;
     hipStream_t s = stream_;
     const uint32_t n = (uint32_t)n_sz;
@@ -627,7 +1281,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     };
     QDT_HIP(hipEventRecord(ev0_, s));
     mark(0);
-    QDT_HIP(hipMemsetAsync(b.counters, 0, 8 * 8, s));
+    QDT_HIP(hipMemsetAsync(b.counters, 0, 8 * 16, s));
     QDT_HIP(hipMemsetAsync(b.err, 0, 64, s));
 
     // 1. keys; the first sort orders by y and carries the input index
@@ -673,6 +1327,15 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     }
 
     mark(3);
+    QDT_HIP(hipMemsetAsync(b.VE, 0xff, 4 * (size_t)m, s));
+    QDT_HIP(hipMemsetAsync(b.owner, 0xff, 4 * (size_t)m, s));
+    // the face flags of the extraction double as the deletion marks of the
+    // seam anchors (-1 = kept) until the merges end
+    QDT_HIP(hipMemsetAsync(b.face, 0xff, 4 * b.rec_cap, s));
+    QDT_HIP(hipMemsetAsync(b.spare, 0, 64, s));
+    const int lim_e = (int)(4 * b.rec_cap);
+    const int spare_base = (int)(3 * (size_t)m);
+    const int spare_cap = (int)(b.rec_cap - 3 * (size_t)m);
     // 4. frame-1 order: (y asc, x desc) = stable y-sort of the reversed frame-0 order
     k_frame1_keys<<<cdiv(m, kBlock), kBlock, 0, s>>>(b.P0, m, b.kA, b.vA);
     kA = b.kA;
@@ -702,6 +1365,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
         k_children<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(b.off, d);
         uint32_t*& Bf = (d & 1) ? B1 : B0;
         uint32_t*& Bo = (d & 1) ? B0 : B1;
+        const int Kd = seams ? seam_anchors((double)(m >> d)) : 0;
+        k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, b.off, d, Kd, b.splitv, b.qv);
         k_split<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bf, nodeA, b.off, d, m, nodeB, b.bits);
         scan_u32(ZeroFlag{ Bo, b.bits }, m, StoreU32{ b.Z }, b.partials, s);
         k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, nodeA, b.off, d, b.bits, b.Z, m, Bt);
@@ -714,34 +1379,83 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     // 6. leaves, 7. merges
     {
         const uint32_t cnt = 1u << D;
-        k_leaves<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, B0, B1, b.off, D, m, b.res,
-                                                                 b.err, b.counters);
+        k_leaves<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, D, m,
+                                                                 lim_e, b.res, b.err, b.counters);
     }
     mark(6);
     for (int d = D - 1; d >= 0; --d)
     {
         const uint32_t cnt = 1u << d;
         mark(9 + 2 * (size_t)d);
-        k_merge<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.off, d, m, b.res, b.err,
-                                                                b.counters);
+        const int K = seams ? seam_anchors((double)(m >> d)) : 0;
+        if (!K)
+            k_merge<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res,
+                                                                    b.err, b.counters);
+        else
+        {
+            const int total = (int)(cnt * (uint32_t)K);
+            k_seg_anchor<<<cdiv((size_t)total, kMergeBlock), kMergeBlock, 0, s>>>(
+                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, D, d, K, m, lim_e, b.arec, b.acoord, b.akey,
+                b.astate, b.counters);
+            k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
+                                                                          b.spare, spare_base, spare_cap, b.res,
+                                                                          b.segstart, b.nstate, b.err, b.counters);
+            k_seg_select<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.OR, b.P0, b.off, d, K, m, lim_e, b.segstart,
+                                                                         b.arec, b.acoord, b.akey, b.astate, b.selk,
+                                                                         b.nsel, b.alist_n, b.counters);
+            const int acap = (int)(b.anchor_cap * (2 + 4 * kCross));
+            const uint32_t ga = cdiv((size_t)total, kMergeBlock);
+            k_seg_mark<<<ga, kMergeBlock, 0, s>>>(b.OR, total, b.arec, b.astate, b.spare, spare_base, spare_cap,
+                                                  b.owner, b.aedge, b.alist, b.alist_n, acap, b.err, b.counters);
+            k_seg_mark_cross<<<ga, kMergeBlock, 0, s>>>(b.OR, total, b.arec, b.astate, b.aedge, m, lim_e, b.owner,
+                                                        (int*)b.face, b.alist, b.alist_n, acap, b.err);
+            k_seg_rebuild<<<cdiv((size_t)total * 8, kMergeBlock), kMergeBlock, 0, s>>>(
+                b.NX, b.OR, b.VE, m, lim_e, b.arec, b.aedge, b.owner, (const int*)b.face, b.alist, b.alist_n, acap,
+                b.err, b.counters);
+            k_seg_pools<<<ga, kMergeBlock, 0, s>>>(b.NX, b.OR, total, b.arec, b.astate, b.aedge, lim_e, (int*)b.face,
+                                                   b.apool);
+            k_seg_compact<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge,
+                                                                          b.segstart, b.nseg, b.off);
+            for (int parity = 0; parity < kPhases; ++parity)
+            {
+                k_seg_run<<<cdiv((size_t)cnt * kSegSlots, kSegBlock), kSegBlock, 0, s>>>(
+                    b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.spare, spare_base, spare_cap, b.segstart, b.nseg,
+                    parity, K, b.segpool, b.vlog, b.vlog_n, b.err, b.counters);
+                k_seg_check<<<cnt * (K + 1), kMergeBlock, 0, s>>>(b.nseg, parity, K, b.vlog, b.vlog_n, b.owner,
+                                                                   m, b.err, 0, b.counters);
+                k_seg_check<<<cnt * (K + 1), kMergeBlock, 0, s>>>(b.nseg, parity, K, b.vlog, b.vlog_n, b.owner,
+                                                                   m, b.err, 1, b.counters);
+            }
+            k_seg_final<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m,
+                                                                        lim_e, b.nseg, b.nstate, b.segpool,
+                                                                        b.astate, b.apool, b.res, b.err, b.counters);
+        }
         mark(10 + 2 * (size_t)d);
     }
     mark(7);
 
     // 8. extraction
-    const uint32_t nrec = 3 * m;
-    k_extract_test<<<cdiv(nrec, kBlock), kBlock, 0, s>>>(b.NX, b.OR, b.P0, nrec, m, b.face, b.queue, b.qcap,
-                                                          b.counters, b.err);
-    k_extract_exact<<<256, kBlock, 0, s>>>(b.NX, b.OR, b.P0, m, b.queue, b.counters, b.qcap, b.face);
-    scan_u32(FaceCount{ b.face }, nrec, FaceWrite{ b.face, b.NX, b.OR, b.ID0, b.tris, m }, b.partials, s);
+    const uint32_t nrec = (uint32_t)(3 * (size_t)m + spare_cap);
+    k_extract_test<<<cdiv(nrec, kBlock), kBlock, 0, s>>>(b.NX, b.OR, b.P0, nrec, m, lim_e, b.spare,
+                                                          (uint32_t)spare_cap, b.face, b.queue, b.qcap, b.counters,
+                                                          b.err);
+    k_extract_exact<<<256, kBlock, 0, s>>>(b.NX, b.OR, b.P0, m, lim_e, b.queue, b.counters, b.qcap, b.face);
+    scan_u32(FaceCount{ b.face }, nrec, FaceWrite{ b.face, b.NX, b.OR, b.ID0, b.tris, m, lim_e }, b.partials, s);
     mark(8);
     QDT_HIP(hipEventRecord(ev1_, s));
     const uint32_t nbx = cdiv(nrec, kScanTile);
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbx, 4, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipMemcpyAsync(hs + 1, b.err, 4, hipMemcpyDeviceToHost, s));
-    unsigned long long cnts[8];
-    QDT_HIP(hipMemcpyAsync(cnts, b.counters, 8 * 8, hipMemcpyDeviceToHost, s));
+    unsigned long long cnts[16];
+    QDT_HIP(hipMemcpyAsync(cnts, b.counters, 8 * 16, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
+    if (hs[1] & kSeamErr)
+    {
+        for (int i = 0; i < 9; ++i) st_.seam[i] = cnts[5 + i];
+        st_.err_bits = hs[1];
+        redo = true;
+        return Status::ok;
+    }
     if (hs[1] & ~1) return Status::internal;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev0_, ev1_);
@@ -751,6 +1465,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz)
     st_.incircle = cnts[2];
     st_.incircle_exact = cnts[3];
     st_.extract_queue = cnts[4];
+    for (int i = 0; i < 9; ++i) st_.seam[i] = cnts[5 + i];
+    st_.err_bits = hs[1];
     st_.triangles = hs[0];
     if (prof && evs_.size() >= 9 + 2 * (size_t)D)
     {

@@ -30,6 +30,11 @@ struct RunStats
     double stage_ms[8];   // keys+sort y, sort x, dedupe, frame-1 sort, kd levels, leaves, merges, extract
     double merge_launch_ms_sum;
     uint32_t merge_launches;
+    uint32_t seam_retries; // runs repeated with unsplit merges
+    // seam pieces: anchors certified, anchors inserted, pieces run, pieces
+    // that ended early, vertex conflicts, log overflows (of the first run)
+    uint64_t seam[9];
+    uint32_t err_bits;
 };
 
 enum class Status : int
@@ -68,12 +73,14 @@ public:
     void set_block_points(int c) { block_points_ = c; }
     // Records HIP events around every stage and every merge launch.
     void set_profiling(bool on) { profiling_ = on; }
+    // Cuts large merges into seam pieces (default on).
+    void set_seams(bool on) { seams_ = on; }
     // Device bytes currently held.
     size_t workspace_bytes() const { return ws_bytes_; }
 
 private:
     Status ensure(size_t n);
-    Status run_impl(const double* d_pts, size_t n);
+    Status run_impl(const double* d_pts, size_t n, bool seams, bool& redo);
 
     hipStream_t stream_;
     bool own_stream_;
@@ -82,6 +89,7 @@ private:
     size_t cap_n_ = 0;
     int block_points_ = 0;
     bool profiling_ = false;
+    bool seams_ = true;
     std::vector<hipEvent_t> evs_; // profiling events, created on demand
     hipEvent_t prof_event(size_t i);
     DeviceResult res_;

@@ -86,6 +86,7 @@ struct qdt_t
     bool pts_on_device_only = false;
     qdt_gpu_stats stats{};
     bool profiling = false;
+    bool seams = true;
     // constraints accepted so far (pairs of representative indices)
     std::set<std::pair<size_t, size_t>> cons;
 
@@ -182,6 +183,7 @@ qdt_status ensure_engine(qdt_t* Q)
     Q->eng = new (std::nothrow) qdt::Engine();
     if (!Q->eng) return QDT_ERR_OUT_OF_MEMORY;
     Q->eng->set_profiling(Q->profiling);
+    Q->eng->set_seams(Q->seams);
     if (!Q->eng->stream())
     {
         delete Q->eng;
@@ -221,6 +223,8 @@ void record_stats(qdt_t* Q)
     for (int i = 0; i < 8; ++i) Q->stats.stage_ms[i] = s.stage_ms[i];
     Q->stats.merge_launch_ms_sum = s.merge_launch_ms_sum;
     Q->stats.merge_launches = s.merge_launches;
+    Q->stats.seam_retries = s.seam_retries;
+    for (int i = 0; i < 9; ++i) Q->stats.seam_counters[i] = s.seam[i];
 }
 
 // Brings triangles and rep of the last device run into host memory.
@@ -639,6 +643,13 @@ void qdt_gpu_set_profiling(qdt_t* ptr, int enable)
     if (!ptr) return;
     ptr->profiling = enable != 0;
     if (ptr->eng) ptr->eng->set_profiling(ptr->profiling);
+}
+
+void qdt_gpu_set_seams(qdt_t* ptr, int enable)
+{
+    if (!ptr) return;
+    ptr->seams = enable != 0;
+    if (ptr->eng) ptr->eng->set_seams(ptr->seams);
 }
 
 int qdt_gpu_device_count(void)

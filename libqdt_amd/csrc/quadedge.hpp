@@ -8,13 +8,16 @@
 //   OR[e >> 1] origin of primal edge e    (int32, 2 per record, 8 B)
 // so every operation is one 4-byte access, records can be initialised with
 // one 16-byte store, and the same code runs on HBM or on LDS pointers.
+// VE[v] keeps one edge out of every vertex (-1 while it has none), updated by
+// every connect and delete; the seam kernels start their walks from it.
 //
 // Storage of a kd node.  A node holding points [o, o+s) owns records
 // [3o, 3o+3s).  A planar graph on s >= 3 vertices has at most 3s-6 edges, so
 // a node never needs more records than its own range; the records a node is
 // not using sit on its free list (linked through NX[4k], marked OR[2k] = -1),
-// which the parent inherits.  No atomics and no global allocator are needed,
-// and the final layout is deterministic.
+// which the parent inherits.  Seam segments that run concurrently inside one
+// merge take extra records from a spare region above 3m with an atomic
+// counter when their private lists run dry.
 #pragma once
 
 #include "predicates.hpp"
@@ -36,21 +39,44 @@ struct PredCount
     unsigned orient, orient_exact, incircle, incircle_exact;
 };
 
-// Every access is bounds-checked (a compare next to a dependent load costs
-// nothing measurable) and every loop has an iteration budget: a logic error
-// sets `err` and unwinds instead of touching memory outside the store or
-// spinning forever.  err = 1: index out of range, 2: free list empty,
-// 3: loop budget exhausted.
+// Error codes (`err`): 1 index out of range, 2 no free record, 3 loop
+// budget exhausted, 4 a seam segment ended before its stop edge, 5 vertex
+// log overflow.  Every access is bounds-checked (a compare next to a
+// dependent load costs nothing measurable) and every loop has a budget: a
+// logic error unwinds instead of touching memory outside the store or
+// spinning forever.
 template <class IntPtr, class PtPtr>
 struct QuadEdges
 {
     IntPtr NX;
     IntPtr OR;
     PtPtr PT;
-    int lim_e; // directed edge ids are in [0, lim_e)
-    int lim_v; // vertex ids are in [0, lim_v)
+    int* VE;            // vertex -> an edge out of it (may be null)
+    int lim_e;          // directed edge ids are in [0, lim_e)
+    int lim_v;          // vertex ids are in [0, lim_v)
+    unsigned* spare;    // spare record counter (null: no spare region)
+    int spare_base;     // first spare record
+    int spare_cap;      // spare records available
+    int* vlog;          // vertex log for conflict checks (null: off)
+    int nlog, logcap;
     int err;
     PredCount pc;
+
+    __device__ __forceinline__ void init(IntPtr nx, IntPtr orr, PtPtr pt, int* ve, int le, int lv)
+    {
+        NX = nx;
+        OR = orr;
+        PT = pt;
+        VE = ve;
+        lim_e = le;
+        lim_v = lv;
+        spare = nullptr;
+        spare_base = spare_cap = 0;
+        vlog = nullptr;
+        nlog = logcap = 0;
+        err = 0;
+        pc = PredCount{ 0, 0, 0, 0 };
+    }
 
     __device__ __forceinline__ int chk(int e)
     {
@@ -78,13 +104,38 @@ struct QuadEdges
         }
         return PT[v];
     }
+    __device__ __forceinline__ void set_ve(int v, int e)
+    {
+        if (VE && (unsigned)v < (unsigned)lim_v) VE[v] = e;
+    }
+    __device__ __forceinline__ void log_v(int v)
+    {
+        if (!vlog || (unsigned)v >= (unsigned)lim_v) return;
+        if (nlog > 0 && vlog[nlog - 1] == v) return;
+        if (nlog >= logcap)
+        {
+            err = err ? err : 5;
+            return;
+        }
+        vlog[nlog++] = v;
+    }
 
     __device__ __forceinline__ int pop(Pool& p)
     {
         const int k = p.head;
-        if (k < 0 || 4 * k >= lim_e)
+        if (k < 0)
         {
+            if (spare)
+            {
+                const unsigned s = atomicAdd(spare, 1u);
+                if ((int)s < spare_cap) return spare_base + (int)s;
+            }
             err = err ? err : 2;
+            return 0;
+        }
+        if (4 * k >= lim_e)
+        {
+            err = err ? err : 1;
             return 0;
         }
         const int nx = NX[4 * k];
@@ -140,18 +191,26 @@ struct QuadEdges
     __device__ __forceinline__ int connect(int a, int b, Pool& p)
     {
         const int e = make_edge(p);
-        set_org(e, dest(a));
-        set_org(e ^ 2, org(b));
+        const int o = dest(a), d = org(b);
+        set_org(e, o);
+        set_org(e ^ 2, d);
         splice(e, lnext(a));
         splice(e ^ 2, b);
+        set_ve(o, e);
+        set_ve(d, e ^ 2);
         return e;
     }
 
     // builder.hxx:161-166
     __device__ __forceinline__ void delete_edge(int e, Pool& p)
     {
-        splice(e, oprev(e));
-        splice(e ^ 2, oprev(e ^ 2));
+        const int o = org(e), d = dest(e);
+        const int po = oprev(e);
+        splice(e, po);
+        const int pd = oprev(e ^ 2);
+        splice(e ^ 2, pd);
+        set_ve(o, po != e ? po : -1);
+        set_ve(d, pd != (e ^ 2) ? pd : -1);
         push(p, chk(e) >> 2);
     }
 
@@ -172,6 +231,12 @@ struct QuadEdges
         const double2 A = pt(a), B = pt(b), C = pt(c), D = pt(d);
         ++pc.incircle;
         return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact) > 0;
+    }
+    __device__ __forceinline__ double incircle_v(int a, int b, int c, int d)
+    {
+        const double2 A = pt(a), B = pt(b), C = pt(c), D = pt(d);
+        ++pc.incircle;
+        return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact);
     }
 
     // frame order of builder.hxx:71-76
@@ -202,6 +267,8 @@ struct QuadEdges
             const int a = make_edge(pool);
             set_org(a, s[0]);
             set_org(a ^ 2, s[1]);
+            set_ve(s[0], a);
+            set_ve(s[1], a ^ 2);
             first = a;
             second = a ^ 2;
             return;
@@ -212,6 +279,9 @@ struct QuadEdges
         set_org(a ^ 2, s[1]);
         set_org(b, s[1]);
         set_org(b ^ 2, s[2]);
+        set_ve(s[0], a);
+        set_ve(s[1], b);
+        set_ve(s[2], b ^ 2);
         const double o = orient(s[0], s[1], s[2]);
         if (o > 0)
         {
@@ -232,10 +302,8 @@ struct QuadEdges
         }
     }
 
-    // Merge of two triangulations separated in the current frame,
-    // builder.hxx:205-285.  Left/right candidate scans run one after the
-    // other; they touch disjoint edges, so the order does not matter.
-    __device__ void merge(int& ldo, int ldi, int rdi, int& rdo, Pool& pool, int budget)
+    // Lower common tangent and first cross edge of a merge (builder.hxx:253-261).
+    __device__ __forceinline__ int tangent(int& ldo, int ldi, int rdi, int& rdo, Pool& pool, int& budget)
     {
         for (;;)
         {
@@ -247,57 +315,106 @@ struct QuadEdges
             else
                 break;
         }
-        int basel = connect(rdi ^ 2, ldi, pool);
+        const int basel = connect(rdi ^ 2, ldi, pool);
         if (org(ldi) == org(ldo)) ldo = basel ^ 2;
         if (org(rdi) == org(rdo)) rdo = basel;
+        return basel;
+    }
+
+    // The rising bubble of builder.hxx:263-283 from cross edge `basel`.  With
+    // stop < 0 it runs to the upper common tangent.  With a stop edge (a
+    // cross edge already in the store, directed right -> left like basel) it
+    // ends when the next cross edge to connect is the stop edge.  Where the
+    // stop edge is the first candidate around a base vertex, that side has no
+    // candidate inside this piece (between two cross edges sharing a vertex
+    // only further cross edges can lie: an edge of one side there would cross
+    // an edge of the other side).  Returns 1 at the stop edge, 0 at the top.
+    __device__ int bubble(int basel, int stop, Pool& pool, int& budget)
+    {
+        const int srec = stop >= 0 ? (stop >> 2) : -1;
+        const int so = stop >= 0 ? org(stop) : -1;
+        const int sd = stop >= 0 ? dest(stop) : -1;
         for (;;)
         {
-            if (err) return;
+            if (err) return 0;
             if (--budget < 0)
             {
                 err = 3;
-                return;
+                return 0;
             }
             const int bo = org(basel);
             const int bd = dest(basel);
+            log_v(bo);
+            log_v(bd);
             int lc = onext(basel ^ 2);
-            bool lv = ccw(dest(lc), bd, bo);
-            if (lv)
+            bool lv = false;
+            if ((lc >> 2) != srec)
             {
-                bool deleted = false;
-                for (;;)
+                log_v(dest(lc));
+                lv = ccw(dest(lc), bd, bo);
+                if (lv)
                 {
-                    if (err || --budget < 0) break;
-                    const int nx = onext(lc);
-                    if (!in_circle(bd, bo, dest(lc), dest(nx))) break;
-                    delete_edge(lc, pool);
-                    lc = nx;
-                    deleted = true;
+                    bool deleted = false;
+                    for (;;)
+                    {
+                        if (err || --budget < 0) break;
+                        const int nx = onext(lc);
+                        if ((nx >> 2) == srec) break;
+                        log_v(dest(nx));
+                        if (!in_circle(bd, bo, dest(lc), dest(nx))) break;
+                        delete_edge(lc, pool);
+                        lc = nx;
+                        deleted = true;
+                    }
+                    if (deleted) lv = ccw(dest(lc), bd, bo);
                 }
-                if (deleted) lv = ccw(dest(lc), bd, bo);
             }
             int rc = oprev(basel);
-            bool rv = ccw(dest(rc), bd, bo);
-            if (rv)
+            bool rv = false;
+            if ((rc >> 2) != srec)
             {
-                bool deleted = false;
-                for (;;)
+                log_v(dest(rc));
+                rv = ccw(dest(rc), bd, bo);
+                if (rv)
                 {
-                    if (err || --budget < 0) break;
-                    const int pv = oprev(rc);
-                    if (!in_circle(bd, bo, dest(rc), dest(pv))) break;
-                    delete_edge(rc, pool);
-                    rc = pv;
-                    deleted = true;
+                    bool deleted = false;
+                    for (;;)
+                    {
+                        if (err || --budget < 0) break;
+                        const int pv = oprev(rc);
+                        if ((pv >> 2) == srec) break;
+                        log_v(dest(pv));
+                        if (!in_circle(bd, bo, dest(rc), dest(pv))) break;
+                        delete_edge(rc, pool);
+                        rc = pv;
+                        deleted = true;
+                    }
+                    if (deleted) rv = ccw(dest(rc), bd, bo);
                 }
-                if (deleted) rv = ccw(dest(rc), bd, bo);
             }
-            if (!lv && !rv) break;
-            if (!lv || (rv && in_circle(dest(lc), bd, bo, dest(rc))))
+            if (!lv && !rv) return 0;
+            const bool right = !lv || (rv && in_circle(dest(lc), bd, bo, dest(rc)));
+            // the new cross edge would run dest(rc) -> bd or bo -> dest(lc)
+            if (stop >= 0)
+            {
+                const int no = right ? dest(rc) : bo;
+                const int nd = right ? bd : dest(lc);
+                if (no == so && nd == sd) return 1;
+            }
+            if (right)
                 basel = connect(rc, basel ^ 2, pool);
             else
                 basel = connect(basel ^ 2, lc ^ 2, pool);
         }
+    }
+
+    // Merge of two triangulations separated in the current frame,
+    // builder.hxx:205-285.  Left/right candidate scans run one after the
+    // other; they touch disjoint edges, so the order does not matter.
+    __device__ void merge(int& ldo, int ldi, int rdi, int& rdo, Pool& pool, int budget)
+    {
+        const int basel = tangent(ldo, ldi, rdi, rdo, pool, budget);
+        bubble(basel, -1, pool, budget);
     }
 
     // builder.hxx:287-301
@@ -330,14 +447,11 @@ struct QuadEdges
         }
     }
 
-    // merge plus the re-walk to the parent's frame (builder.hxx:338-347)
-    // npts = points of the node (bounds every loop)
-    __device__ __forceinline__ void merge_node(int lfirst, int lsecond, int rfirst, int rsecond, int axis,
-                                               int out_axis, int npts, int& first, int& second, Pool& pool)
+    // re-walk of a merged node's extreme hull edges to the parent's frame
+    // (builder.hxx:340-346)
+    __device__ __forceinline__ void rewalk(int ldo, int rdo, int axis, int out_axis, int npts, int& first,
+                                           int& second)
     {
-        int ldo = lfirst, rdo = rsecond;
-        const int budget = 16 * npts + 64;
-        merge(ldo, lsecond, rfirst, rdo, pool, budget);
         if (out_axis != axis)
         {
             first = hull_walk(ldo, out_axis, false, npts + 8);
@@ -349,6 +463,16 @@ struct QuadEdges
             first = ldo;
             second = rdo;
         }
+    }
+
+    // npts = points of the node (bounds every loop)
+    __device__ __forceinline__ void merge_node(int lfirst, int lsecond, int rfirst, int rsecond, int axis,
+                                               int out_axis, int npts, int& first, int& second, Pool& pool)
+    {
+        int ldo = lfirst, rdo = rsecond;
+        const int budget = 16 * npts + 64;
+        merge(ldo, lsecond, rfirst, rdo, pool, budget);
+        rewalk(ldo, rdo, axis, out_axis, npts, first, second);
     }
 };
 

@@ -1,0 +1,248 @@
+// seam.hpp -- cutting one large Guibas-Stolfi merge into independent pieces.
+//
+// A merge of two triangulations L and R (separated in the node's frame)
+// produces a chain of cross edges from the lower to the upper common tangent.
+// The rising bubble walks that chain one edge at a time, so a top-level merge
+// of millions of points is a single thread chasing ~sqrt(n) pointers.  Any
+// edge that is in *every* Delaunay triangulation of L u R cuts the chain: the
+// bubble below it and the bubble above it touch disjoint parts of the store.
+//
+//  1. anchors   one thread per (node, k): a point q on the cut line at the
+//               k-th height; greedy walks find q's nearest vertices u in L and
+//               v in R (floating point is fine here); q moves to where the
+//               bisector of u and v meets its height until (u, v) is stable.
+//  2. certify   exactly: circles through u and v are parametrised by their
+//               centre on the bisector; each Delaunay neighbour w of u (in L)
+//               or of v (in R) forbids the centres on one side of the
+//               circumcentre of (u, v, w).  (u, v) is in every Delaunay
+//               triangulation of L u R iff every neighbour left of u->v and
+//               every neighbour right of it are pairwise strictly separated:
+//               incircle(u, v, wl, wr) < 0 for all such pairs (u, v, wl
+//               counter-clockwise), and no neighbour is collinear with u, v.
+//               (Vor_L(u) is cut out by u's Delaunay neighbours in L, Vor_R(v)
+//               by v's in R, so these constraints are all there are.)
+//  3. insert    one thread per node: the lower tangent, then every anchor in
+//               order along the chain: the edges of L and of R crossing the
+//               anchor segment are found by a straight walk (dry run first,
+//               any degeneracy drops the anchor), deleted, and the anchor is
+//               spliced into the rings of u and v.
+//  4. segments  one thread per piece, in three phases (pieces k, k+3, ...
+//               together): the rising bubble from anchor k stops when anchor
+//               k+1 is the next cross edge (QuadEdges::bubble).  Pieces of one
+//               phase must not share vertices; each logs the vertices whose
+//               rings it reads or splices, and a check kernel flags any vertex
+//               logged twice.  A flagged run is repeated with unsplit merges.
+#pragma once
+
+#include "quadedge.hpp"
+
+namespace qdt
+{
+
+// frame coordinates: frame 0 is (x, y); frame 1 is (y, -x), a rotation that
+// turns the frame-1 order of builder.hxx:75 into lexicographic (X, Y) order
+__device__ __forceinline__ double frame_x(double2 p, int axis) { return axis ? p.y : p.x; }
+__device__ __forceinline__ double frame_y(double2 p, int axis) { return axis ? -p.x : p.y; }
+__device__ __forceinline__ double2 from_frame(double X, double Y, int axis)
+{
+    return axis ? make_double2(-Y, X) : make_double2(X, Y);
+}
+
+template <class Q>
+__device__ __forceinline__ double dist2(Q& q, int v, double2 p)
+{
+    const double2 a = q.pt(v);
+    const double dx = a.x - p.x, dy = a.y - p.y;
+    return dx * dx + dy * dy;
+}
+
+// Greedy walk to the vertex nearest to p (floating point).  Returns the
+// vertex and, in e_out, an edge out of it; -1 when the start has no edge.
+template <class Q>
+__device__ int nn_walk(Q& q, int s, double2 p, int budget, int& e_out)
+{
+    int e0 = q.VE[s];
+    if (e0 < 0) return -1;
+    int best = s;
+    double bd = dist2(q, s, p);
+    for (int step = 0; step < budget && !q.err; ++step)
+    {
+        int e = e0, nb = -1, ne = -1;
+        double nbd = bd;
+        int guard = 0;
+        do
+        {
+            const int w = q.dest(e);
+            const double d = dist2(q, w, p);
+            if (d < nbd)
+            {
+                nbd = d;
+                nb = w;
+                ne = e ^ 2;
+            }
+            e = q.onext(e);
+        } while (e != e0 && ++guard < 4096 && !q.err);
+        if (nb < 0)
+        {
+            e_out = e0;
+            return best;
+        }
+        best = nb;
+        bd = nbd;
+        e0 = ne;
+    }
+    e_out = e0;
+    return best;
+}
+
+// Exact certificate that (u, v) is an edge of every Delaunay triangulation of
+// L u R (u in L, v in R, eu / ev edges out of u / v).
+template <class Q>
+__device__ bool certify(Q& q, int u, int eu, int v, int ev)
+{
+    constexpr int kMax = 48;
+    int left[kMax], right[kMax];
+    int nl = 0, nr = 0;
+    for (int side = 0; side < 2; ++side)
+    {
+        const int e0 = side ? ev : eu;
+        int e = e0, guard = 0;
+        do
+        {
+            const int w = q.dest(e);
+            if (w != u && w != v)
+            {
+                const double o = q.orient(u, v, w);
+                if (o == 0) return false;
+                if (o > 0)
+                {
+                    if (nl >= kMax) return false;
+                    left[nl++] = w;
+                }
+                else
+                {
+                    if (nr >= kMax) return false;
+                    right[nr++] = w;
+                }
+            }
+            e = q.onext(e);
+        } while (e != e0 && ++guard < kMax && !q.err);
+        if (e != e0) return false;
+    }
+    for (int i = 0; i < nl; ++i)
+        for (int j = 0; j < nr; ++j)
+            if (!(q.incircle_v(u, v, left[i], right[j]) < 0)) return false;
+    return !q.err;
+}
+
+// The edge x out of u such that direction u->p lies strictly inside the
+// counter-clockwise wedge from x to onext(x); -1 when p's direction
+// coincides with an edge or the ring is malformed.
+template <class Q>
+__device__ int find_wedge(Q& q, int u, int e0, int p)
+{
+    const int n1 = q.onext(e0);
+    if (n1 == e0) return e0; // single edge: every other direction is inside
+    int e = e0, guard = 0;
+    do
+    {
+        const int n = q.onext(e);
+        const int a = q.dest(e), b = q.dest(n);
+        const double oap = q.orient(u, a, p);
+        const double opb = q.orient(u, p, b);
+        if (oap == 0 && q.orient(u, a, b) != 0)
+        {
+            // collinear with edge e: same direction is a coincidence, the
+            // opposite direction lies in some wedge (handled below)
+            const double2 U = q.pt(u), A = q.pt(a), P = q.pt(p);
+            if ((A.x - U.x) * (P.x - U.x) + (A.y - U.y) * (P.y - U.y) > 0) return -1;
+        }
+        const double oab = q.orient(u, a, b);
+        bool inside;
+        if (oab > 0)
+            inside = oap > 0 && opb > 0;
+        else
+            inside = oap > 0 || opb > 0; // reflex or straight wedge
+        if (inside) return e;
+        e = n;
+    } while (e != e0 && ++guard < 256 && !q.err);
+    return -1;
+}
+
+// Kind of the face left of e: 1 a counter-clockwise triangle, 0 the outer
+// face (its boundary runs clockwise, or is longer than any face the anchor
+// insertions can create), -1 a larger inner face (left by an earlier
+// anchor's deletions) or a degenerate one.
+template <class Q>
+__device__ int face_kind(Q& q, int e)
+{
+    constexpr int kLimit = 48;
+    const int f = q.lnext(e), g = q.lnext(f);
+    if (q.lnext(g) == e)
+    {
+        const double o = q.orient(q.org(e), q.org(f), q.org(g));
+        return o > 0 ? 1 : (o < 0 ? 0 : -1);
+    }
+    // shoelace area of the boundary, relative to its first vertex
+    const double2 p0 = q.pt(q.org(e));
+    double area = 0;
+    int h = f, n = 1;
+    double2 prev = q.pt(q.org(f));
+    while (h != e && n < kLimit && !q.err)
+    {
+        h = q.lnext(h);
+        const double2 cur = q.pt(q.org(h));
+        area += (prev.x - p0.x) * (cur.y - p0.y) - (prev.y - p0.y) * (cur.x - p0.x);
+        prev = cur;
+        ++n;
+    }
+    if (h != e) return 0; // longer than any inner face an insertion leaves
+    return area < 0 ? 0 : -1;
+}
+
+// Edges of one triangulation crossed by the segment u -> p (u a vertex of it,
+// p outside its hull).  x is u's wedge edge for direction p.  Dry run: fills
+// out[] and returns the count, or -1 on any degeneracy (the segment through a
+// vertex, p inside a face, a face enlarged by an earlier anchor, too many
+// crossings).
+template <class Q>
+__device__ int collect_crossings(Q& q, int u, int x, int p, int* out, int cap)
+{
+    int n = 0;
+    if (q.onext(x) == x) return 0; // u has one edge: nothing around it to cross
+    // the face left of x, between x and onext(x)
+    const int k0 = face_kind(q, x);
+    if (k0 == 0) return 0;  // the segment leaves u into the outer face
+    if (k0 < 0) return -1;
+    int e;
+    {
+        const int f = q.lnext(x), g = q.lnext(f);
+        if (q.onext(x) != (g ^ 2)) return -1;
+        // f runs a -> b opposite u; the segment leaves through it
+        const double o = q.orient(q.org(f), q.dest(f), p);
+        if (o >= 0) return -1; // p inside the triangle or on its edge
+        e = f;
+    }
+    for (int guard = 0; guard < cap + 4; ++guard)
+    {
+        if (n >= cap) return -1;
+        out[n++] = e; // e runs a -> b with a right of u->p and b left of it
+        const int h = e ^ 2; // b -> a
+        const int k = face_kind(q, h);
+        if (k == 0) return n; // crossed a hull edge: done
+        if (k < 0) return -1;
+        const int f = q.lnext(h), g = q.lnext(f);
+        const int c = q.org(g); // triangle (b, a, c)
+        const double oc = q.orient(u, p, c);
+        if (oc == 0) return -1; // the segment passes through c
+        if (oc > 0)
+            e = f; // a -> c crosses (a right, c left)
+        else
+            e = g; // c -> b crosses (c right, b left)
+        const double op = q.orient(q.org(e), q.dest(e), p);
+        if (op >= 0) return -1; // p inside this triangle
+    }
+    return -1;
+}
+
+} // namespace qdt
