@@ -35,9 +35,9 @@ typedef struct qdt_gpu_stats
     double merge_launch_ms_sum;   /* sum of the merge kernel launches' durations */
     uint64_t merge_launches;
     uint64_t seam_retries;        /* runs repeated without seam pieces */
-    uint64_t seam_counters[9];    /* anchors certified, inserted, pieces run, ended early, conflicts,
-                                     log overflows, anchors with a clean dry run, anchors selected,
-                                     vertex rings rewritten by anchor insertion */
+    uint64_t seam_counters[8];    /* anchors certified, anchors inserted, pieces run, pieces
+                                     resumed, pieces stopped by a claimed vertex, anchors with a
+                                     clean dry run, anchors selected, vertex rings rewritten */
 } qdt_gpu_stats;
 
 /* Like qdt_execute(), but d_pts points to n points in device memory.  The

@@ -57,15 +57,15 @@ class GpuStats(ctypes.Structure):
         ("triangles", ctypes.c_uint64), ("kd_levels", ctypes.c_uint64), ("device_ms", ctypes.c_double),
         ("stage_ms", ctypes.c_double * 8), ("merge_launch_ms_sum", ctypes.c_double),
         ("merge_launches", ctypes.c_uint64), ("seam_retries", ctypes.c_uint64),
-        ("seam_counters", ctypes.c_uint64 * 9),
+        ("seam_counters", ctypes.c_uint64 * 8),
     ]
     STAGES = ("keys_sort_y", "sort_x", "dedupe", "frame1_sort", "kd_levels", "leaves", "merges", "extract")
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("stage_ms", "seam_counters")}
         d["stage_ms"] = dict(zip(self.STAGES, list(self.stage_ms)))
-        d["seam"] = dict(zip(("certified", "inserted", "pieces", "ended_early", "conflicts", "log_overflow",
-                                  "dry_run_ok", "selected", "rings_rebuilt"),
+        d["seam"] = dict(zip(("certified", "inserted", "pieces", "resumed", "stopped", "dry_run_ok", "selected",
+                                  "rings_rebuilt"),
                              list(self.seam_counters)))
         return d
 

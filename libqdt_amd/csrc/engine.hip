@@ -309,9 +309,9 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
 constexpr int kSegMaxK = 64;           // anchors per node at most
 constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
 constexpr int kSegMin = 8192;          // smaller nodes merge in one piece
-constexpr int kLogCap = 2048;          // vertex log entries per piece
-constexpr int kSegBlock = 4;           // pieces per block (their logs sit in LDS)
-constexpr int kPhases = 3;             // pieces k, k+3, ... run together
+constexpr int kSegBlock = 64;          // pieces per block
+constexpr int kPhases = 2;             // pieces k, k+2, ... run together
+constexpr int kClaimBits = 20;         // piece id bits of a claim (the rest: launch epoch)
 constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit merges
 
 // split vertex of every internal node; anchor heights of segmented levels
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
                     const double vx = frame_x(V, axis), vy = frame_y(V, axis);
                     akey[t] = vx > ux ? uy + (c - ux) / (vx - ux) * (vy - uy) : 0.5 * (uy + vy);
                     state = 1;
-                    atomicAdd(&counters[11], 1ull);
+                    atomicAdd(&counters[10], 1ull);
                 }
             }
         }
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_select(const int* OR, const
             used[nused++] = u;
             used[nused++] = v;
             selk[(size_t)j * kSegSlots + cnt++] = (int)t;
-            atomicAdd(&counters[12], 1ull);
+            atomicAdd(&counters[11], 1ull);
             pu = u;
             pv = v;
             PU = make_double2(c.x, c.y);
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_rebuild(int* NX, const int*
                                                              int acap, int* err, unsigned long long* counters)
 {
     const int n = min(*alist_n, acap);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[13], (unsigned long long)n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[12], (unsigned long long)n);
     for (int i = blockIdx.x * kMergeBlock + threadIdx.x; i < n; i += gridDim.x * kMergeBlock)
     {
         const int w = alist[i];
@@ -762,13 +762,15 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_compact(int d, const int* s
     nseg[j] = cnt;
 }
 
+// One thread per piece of one phase: the rising bubble from cut i to cut
+// i+1.  Pieces of one phase share no cut vertex; a piece that needs a vertex
+// another piece holds stops and leaves its resume edge in segresume.
 __global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const double2* PT, int* VE, const int* off,
                                                       int d, uint32_t m, int lim_e, unsigned* spare, int spare_base,
                                                       int spare_cap, const int* segstart, const int* nseg,
-                                                      int parity, int K, int2* segpool, int* vlog, int* vlog_n,
-                                                      int* err, unsigned long long* counters)
+                                                      int parity, int epoch, int* claim, int2* segpool,
+                                                      int* segresume, int* err, unsigned long long* counters)
 {
-    __shared__ int logs[kSegBlock][kLogCap];
     const uint32_t t = blockIdx.x * kSegBlock + threadIdx.x;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
@@ -777,72 +779,77 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const d
     q.spare_cap = spare_cap;
     const uint32_t j = t / kSegSlots;
     const int i = (int)(t % kSegSlots);
-    bool active = false;
     if (j < (1u << d) && i % kPhases == parity)
     {
         const int ns = nseg[j];
         if (i < ns)
         {
-            active = true;
-            // with at most one piece per phase nothing can collide: no log
-            if (ns > kPhases)
-            {
-                q.vlog = logs[threadIdx.x];
-                q.logcap = kLogCap;
-            }
+            q.claim = claim;
+            q.claim_base = epoch << kClaimBits;
+            q.claim_me = q.claim_base | (int)t;
             const int* od = off + off_base(d);
             const int s = od[j + 1] - od[j];
             const int* st = segstart + (size_t)j * kSegSlots;
-            const int basel = st[i];
+            int basel = st[i];
             const int stop = i + 1 < ns ? st[i + 1] : -1;
             Pool pool = { -1, -1 };
             int budget = 16 * s + 64;
             const int r = q.bubble(basel, stop, pool, budget);
-            if (stop >= 0 && r != 1 && !q.err) q.err = 4;
             atomicAdd(&counters[7], 1ull);
-            if (q.err == 4) atomicAdd(&counters[8], 1ull);
-            if (q.err == 5) atomicAdd(&counters[10], 1ull);
+            if (r == 2 && !q.err)
+            {
+                segresume[t] = basel;
+                atomicAdd(&counters[9], 1ull);
+            }
+            else
+            {
+                segresume[t] = -1;
+                if (stop >= 0 && r != 1 && !q.err) q.err = 4;
+            }
             segpool[t] = make_int2(pool.head, pool.tail);
         }
-    }
-    if (active)
-    {
-        const size_t slot = (size_t)j * (K + 1) + i;
-        int* g = vlog + slot * kLogCap;
-        for (int k = 0; k < q.nlog; ++k) g[k] = q.vlog[k];
-        vlog_n[slot] = q.nlog;
     }
     flush_counts(counters, q.pc);
     if (q.err) atomicOr(err, q.err >= 4 ? kSeamErr : (2 << q.err));
 }
 
-// a vertex logged by two pieces of one phase means they raced: redo the run
-__global__ __launch_bounds__(kMergeBlock) void k_seg_check(const int* nseg, int parity, int K, const int* vlog,
-                                                           const int* vlog_n, int* owner, uint32_t m, int* err,
-                                                           int clear, unsigned long long* counters)
+// one thread per node: the stopped pieces, bottom to top, with no other
+// piece of the node running
+__global__ __launch_bounds__(kMergeBlock) void k_seg_resume(int* NX, int* OR, const double2* PT, int* VE,
+                                                            const int* off, int d, uint32_t m, int lim_e,
+                                                            unsigned* spare, int spare_base, int spare_cap,
+                                                            const int* segstart, const int* nseg, int2* segpool,
+                                                            const int* segresume, int* err,
+                                                            unsigned long long* counters)
 {
-    const uint32_t t = blockIdx.x; // piece slot j * (K + 1) + i
-    const uint32_t j = t / (K + 1);
-    const int i = (int)(t % (K + 1));
-    if (i % kPhases != parity || i >= nseg[j]) return;
-    const int n = vlog_n[t];
-    const int* g = vlog + (size_t)t * kLogCap;
-    for (int k = threadIdx.x; k < n; k += kMergeBlock)
+    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    GQE q;
+    q.init(NX, OR, PT, VE, lim_e, (int)m);
+    q.spare = spare;
+    q.spare_base = spare_base;
+    q.spare_cap = spare_cap;
+    if (j < (1u << d))
     {
-        const int v = g[k];
-        if ((unsigned)v >= m) continue;
-        if (clear)
-            owner[v] = -1;
-        else
+        const int ns = nseg[j];
+        const int* od = off + off_base(d);
+        const int s = od[j + 1] - od[j];
+        int budget = 16 * s + 64;
+        for (int i = 0; i < ns && !q.err; ++i)
         {
-            const int old = atomicCAS(&owner[v], -1, (int)t);
-            if (old != -1 && old != (int)t)
-            {
-                atomicOr(err, kSeamErr);
-                atomicAdd(&counters[9], 1ull);
-            }
+            const size_t t = (size_t)j * kSegSlots + i;
+            int basel = segresume[t];
+            if (basel < 0) continue;
+            const int stop = i + 1 < ns ? segstart[(size_t)j * kSegSlots + i + 1] : -1;
+            const int2 sp = segpool[t];
+            Pool pool = { sp.x, sp.y };
+            const int r = q.bubble(basel, stop, pool, budget);
+            if (stop >= 0 && r != 1 && !q.err) q.err = 4;
+            segpool[t] = make_int2(pool.head, pool.tail);
+            atomicAdd(&counters[8], 1ull);
         }
     }
+    flush_counts(counters, q.pc);
+    if (q.err) atomicOr(err, q.err >= 4 ? kSeamErr : (2 << q.err));
 }
 
 __global__ __launch_bounds__(kMergeBlock) void k_seg_final(int* NX, int* OR, const double2* PT, int* VE,
@@ -1060,7 +1067,7 @@ struct Engine::Bufs
     unsigned long long* counters;
     int* err;
     // seam pieces
-    int *VE, *splitv, *qv, *segstart, *nseg, *vlog, *vlog_n, *owner;
+    int *VE, *splitv, *qv, *segstart, *nseg, *segresume, *claim, *owner;
     int *arec, *astate, *aedge, *selk, *nsel, *alist, *alist_n;
     double4* acoord;
     double* akey;
@@ -1069,7 +1076,7 @@ struct Engine::Bufs
     unsigned* spare;
     size_t rec_cap;   // quad-edge records allocated (3N + spare)
     size_t seg_nodes; // node slots of the segmented levels
-    size_t anchor_cap, log_slots;
+    size_t anchor_cap;
     uint32_t qcap;
     int dcap;
     uint32_t* host_scalars; // pinned
@@ -1142,7 +1149,7 @@ Status Engine::ensure(size_t n)
     const uint32_t nb_scan = cdiv(R, kScanTile) + 1;
     const size_t qcap = std::max<size_t>(4096, N / 4);
     // segmented levels: nodes of at least kSegMin points
-    size_t seg_nodes = 1, anchor_cap = 1, log_slots = 1;
+    size_t seg_nodes = 1, anchor_cap = 1;
     for (int d = 0; d < 31; ++d)
     {
         const double sz = (double)N / (double)(1u << d);
@@ -1150,7 +1157,6 @@ Status Engine::ensure(size_t n)
         if (!K) break;
         seg_nodes = (size_t)2 << d;
         anchor_cap = std::max(anchor_cap, ((size_t)1 << d) * K);
-        log_slots = std::max(log_slots, ((size_t)1 << d) * (K + 1));
     }
     struct Item
     {
@@ -1182,7 +1188,7 @@ Status Engine::ensure(size_t n)
         { (void**)&b.apool, 8 * anchor_cap }, { (void**)&b.selk, 4 * (size_t)kSegSlots * seg_nodes },
         { (void**)&b.nsel, 4 * seg_nodes },
         { (void**)&b.alist, 4 * (2 + 4 * (size_t)kCross) * anchor_cap }, { (void**)&b.alist_n, 64 },
-        { (void**)&b.vlog, 4 * (size_t)kLogCap * log_slots }, { (void**)&b.vlog_n, 4 * log_slots },
+        { (void**)&b.segresume, 4 * (size_t)kSegSlots * seg_nodes }, { (void**)&b.claim, 4 * N },
         { (void**)&b.spare, 64 },
     };
     size_t total = 0;
@@ -1203,7 +1209,6 @@ Status Engine::ensure(size_t n)
     b.rec_cap = R;
     b.seg_nodes = seg_nodes;
     b.anchor_cap = anchor_cap;
-    b.log_slots = log_slots;
     cap_n_ = N;
     return Status::ok;
 }
@@ -1251,7 +1256,7 @@ Status Engine::run(const double* d_pts, size_t n)
         s = run_impl(d_pts, n, false, redo);
         st_.seam_retries = 1;
         st_.device_ms += first.device_ms;
-        for (int i = 0; i < 9; ++i) st_.seam[i] = first.seam[i];
+        for (int i = 0; i < 8; ++i) st_.seam[i] = first.seam[i];
         st_.err_bits = first.err_bits;
     }
     if (s != Status::ok) res_ = DeviceResult();
@@ -1329,6 +1334,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     mark(3);
     QDT_HIP(hipMemsetAsync(b.VE, 0xff, 4 * (size_t)m, s));
     QDT_HIP(hipMemsetAsync(b.owner, 0xff, 4 * (size_t)m, s));
+    QDT_HIP(hipMemsetAsync(b.claim, 0xff, 4 * (size_t)m, s));
     // the face flags of the extraction double as the deletion marks of the
     // seam anchors (-1 = kept) until the merges end
     QDT_HIP(hipMemsetAsync(b.face, 0xff, 4 * b.rec_cap, s));
@@ -1383,11 +1389,14 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
                                                                  lim_e, b.res, b.err, b.counters);
     }
     mark(6);
+    int epoch = 0; // claim epochs: one per piece launch
     for (int d = D - 1; d >= 0; --d)
     {
         const uint32_t cnt = 1u << d;
         mark(9 + 2 * (size_t)d);
-        const int K = seams ? seam_anchors((double)(m >> d)) : 0;
+        // piece ids of a level must fit the claim id bits
+        const bool fits = (size_t)cnt * kSegSlots < ((size_t)1 << kClaimBits);
+        const int K = seams && fits ? seam_anchors((double)(m >> d)) : 0;
         if (!K)
             k_merge<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res,
                                                                     b.err, b.counters);
@@ -1417,15 +1426,13 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
             k_seg_compact<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge,
                                                                           b.segstart, b.nseg, b.off);
             for (int parity = 0; parity < kPhases; ++parity)
-            {
                 k_seg_run<<<cdiv((size_t)cnt * kSegSlots, kSegBlock), kSegBlock, 0, s>>>(
                     b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.spare, spare_base, spare_cap, b.segstart, b.nseg,
-                    parity, K, b.segpool, b.vlog, b.vlog_n, b.err, b.counters);
-                k_seg_check<<<cnt * (K + 1), kMergeBlock, 0, s>>>(b.nseg, parity, K, b.vlog, b.vlog_n, b.owner,
-                                                                   m, b.err, 0, b.counters);
-                k_seg_check<<<cnt * (K + 1), kMergeBlock, 0, s>>>(b.nseg, parity, K, b.vlog, b.vlog_n, b.owner,
-                                                                   m, b.err, 1, b.counters);
-            }
+                    parity, epoch++, b.claim, b.segpool, b.segresume, b.err, b.counters);
+            k_seg_resume<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
+                                                                         b.spare, spare_base, spare_cap, b.segstart,
+                                                                         b.nseg, b.segpool, b.segresume, b.err,
+                                                                         b.counters);
             k_seg_final<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m,
                                                                         lim_e, b.nseg, b.nstate, b.segpool,
                                                                         b.astate, b.apool, b.res, b.err, b.counters);
@@ -1451,7 +1458,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     QDT_HIP(hipStreamSynchronize(s));
     if (hs[1] & kSeamErr)
     {
-        for (int i = 0; i < 9; ++i) st_.seam[i] = cnts[5 + i];
+        for (int i = 0; i < 8; ++i) st_.seam[i] = cnts[5 + i];
         st_.err_bits = hs[1];
         redo = true;
         return Status::ok;
@@ -1465,7 +1472,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     st_.incircle = cnts[2];
     st_.incircle_exact = cnts[3];
     st_.extract_queue = cnts[4];
-    for (int i = 0; i < 9; ++i) st_.seam[i] = cnts[5 + i];
+    for (int i = 0; i < 8; ++i) st_.seam[i] = cnts[5 + i];
     st_.err_bits = hs[1];
     st_.triangles = hs[0];
     if (prof && evs_.size() >= 9 + 2 * (size_t)D)

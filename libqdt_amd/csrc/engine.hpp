@@ -31,9 +31,10 @@ struct RunStats
     double merge_launch_ms_sum;
     uint32_t merge_launches;
     uint32_t seam_retries; // runs repeated with unsplit merges
-    // seam pieces: anchors certified, anchors inserted, pieces run, pieces
-    // that ended early, vertex conflicts, log overflows (of the first run)
-    uint64_t seam[9];
+    // seam pieces (of the first run): anchors certified, anchors inserted,
+    // pieces run, pieces resumed, pieces stopped by a claimed vertex,
+    // anchors with a clean dry run, anchors selected, rings rewritten
+    uint64_t seam[8];
     uint32_t err_bits;
 };
 

@@ -224,7 +224,7 @@ void record_stats(qdt_t* Q)
     Q->stats.merge_launch_ms_sum = s.merge_launch_ms_sum;
     Q->stats.merge_launches = s.merge_launches;
     Q->stats.seam_retries = s.seam_retries;
-    for (int i = 0; i < 9; ++i) Q->stats.seam_counters[i] = s.seam[i];
+    for (int i = 0; i < 8; ++i) Q->stats.seam_counters[i] = s.seam[i];
 }
 
 // Brings triangles and rep of the last device run into host memory.

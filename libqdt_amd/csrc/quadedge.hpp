@@ -18,6 +18,14 @@
 // which the parent inherits.  Seam segments that run concurrently inside one
 // merge take extra records from a spare region above 3m with an atomic
 // counter when their private lists run dry.
+//
+// Claims.  Concurrent segments must never touch the ring of one vertex at
+// the same time.  A segment claims every vertex whose ring it is about to
+// read or change (claim[v] = its id, ids of one launch >= claim_base, larger
+// than every id of earlier launches, so old claims expire by themselves).
+// A segment that finds a vertex held by another one stops before touching
+// it; the operation sequence is a pure function of the store, so the
+// segment can be resumed later from the cross edge where it stopped.
 #pragma once
 
 #include "predicates.hpp"
@@ -40,8 +48,7 @@ struct PredCount
 };
 
 // Error codes (`err`): 1 index out of range, 2 no free record, 3 loop
-// budget exhausted, 4 a seam segment ended before its stop edge, 5 vertex
-// log overflow.  Every access is bounds-checked (a compare next to a
+// budget exhausted, 4 a seam segment ended before its stop edge.  Every access is bounds-checked (a compare next to a
 // dependent load costs nothing measurable) and every loop has a budget: a
 // logic error unwinds instead of touching memory outside the store or
 // spinning forever.
@@ -57,8 +64,10 @@ struct QuadEdges
     unsigned* spare;    // spare record counter (null: no spare region)
     int spare_base;     // first spare record
     int spare_cap;      // spare records available
-    int* vlog;          // vertex log for conflict checks (null: off)
-    int nlog, logcap;
+    int* claim;         // vertex claims (null: single owner, no claims)
+    int claim_me, claim_base;
+    int h0, h1, h2, h3; // recently claimed vertices (named registers: no
+                        // indexed array, which would live in scratch memory)
     int err;
     PredCount pc;
 
@@ -72,8 +81,9 @@ struct QuadEdges
         lim_v = lv;
         spare = nullptr;
         spare_base = spare_cap = 0;
-        vlog = nullptr;
-        nlog = logcap = 0;
+        claim = nullptr;
+        claim_me = claim_base = 0;
+        h0 = h1 = h2 = h3 = -1;
         err = 0;
         pc = PredCount{ 0, 0, 0, 0 };
     }
@@ -108,16 +118,23 @@ struct QuadEdges
     {
         if (VE && (unsigned)v < (unsigned)lim_v) VE[v] = e;
     }
-    __device__ __forceinline__ void log_v(int v)
+    // true when this segment may touch v's ring (claims it on first use)
+    __device__ __forceinline__ bool take(int v)
     {
-        if (!vlog || (unsigned)v >= (unsigned)lim_v) return;
-        if (nlog > 0 && vlog[nlog - 1] == v) return;
-        if (nlog >= logcap)
+        if (!claim) return true;
+        if (v == h0 || v == h1 || v == h2 || v == h3) return true;
+        if ((unsigned)v >= (unsigned)lim_v)
         {
-            err = err ? err : 5;
-            return;
+            err = err ? err : 1;
+            return false;
         }
-        vlog[nlog++] = v;
+        const int r = atomicMax(&claim[v], claim_me);
+        if (r >= claim_base && r != claim_me) return false;
+        h3 = h2;
+        h2 = h1;
+        h1 = h0;
+        h0 = v;
+        return true;
     }
 
     __device__ __forceinline__ int pop(Pool& p)
@@ -328,8 +345,10 @@ struct QuadEdges
     // stop edge is the first candidate around a base vertex, that side has no
     // candidate inside this piece (between two cross edges sharing a vertex
     // only further cross edges can lie: an edge of one side there would cross
-    // an edge of the other side).  Returns 1 at the stop edge, 0 at the top.
-    __device__ int bubble(int basel, int stop, Pool& pool, int& budget)
+    // an edge of the other side).  Returns 1 at the stop edge, 0 at the top,
+    // 2 when a vertex it needs is claimed by another segment: `basel` is then
+    // the cross edge to resume from.
+    __device__ int bubble(int& basel, int stop, Pool& pool, int& budget)
     {
         const int srec = stop >= 0 ? (stop >> 2) : -1;
         const int so = stop >= 0 ? org(stop) : -1;
@@ -344,13 +363,11 @@ struct QuadEdges
             }
             const int bo = org(basel);
             const int bd = dest(basel);
-            log_v(bo);
-            log_v(bd);
+            if (!take(bo) || !take(bd)) return 2;
             int lc = onext(basel ^ 2);
             bool lv = false;
             if ((lc >> 2) != srec)
             {
-                log_v(dest(lc));
                 lv = ccw(dest(lc), bd, bo);
                 if (lv)
                 {
@@ -360,8 +377,8 @@ struct QuadEdges
                         if (err || --budget < 0) break;
                         const int nx = onext(lc);
                         if ((nx >> 2) == srec) break;
-                        log_v(dest(nx));
                         if (!in_circle(bd, bo, dest(lc), dest(nx))) break;
+                        if (!take(dest(lc))) return 2;
                         delete_edge(lc, pool);
                         lc = nx;
                         deleted = true;
@@ -373,7 +390,6 @@ struct QuadEdges
             bool rv = false;
             if ((rc >> 2) != srec)
             {
-                log_v(dest(rc));
                 rv = ccw(dest(rc), bd, bo);
                 if (rv)
                 {
@@ -383,8 +399,8 @@ struct QuadEdges
                         if (err || --budget < 0) break;
                         const int pv = oprev(rc);
                         if ((pv >> 2) == srec) break;
-                        log_v(dest(pv));
                         if (!in_circle(bd, bo, dest(rc), dest(pv))) break;
+                        if (!take(dest(rc))) return 2;
                         delete_edge(rc, pool);
                         rc = pv;
                         deleted = true;
@@ -401,6 +417,7 @@ struct QuadEdges
                 const int nd = right ? bd : dest(lc);
                 if (no == so && nd == sd) return 1;
             }
+            if (!take(right ? dest(rc) : dest(lc))) return 2;
             if (right)
                 basel = connect(rc, basel ^ 2, pool);
             else
@@ -413,7 +430,7 @@ struct QuadEdges
     // other; they touch disjoint edges, so the order does not matter.
     __device__ void merge(int& ldo, int ldi, int rdi, int& rdo, Pool& pool, int budget)
     {
-        const int basel = tangent(ldo, ldi, rdi, rdo, pool, budget);
+        int basel = tangent(ldo, ldi, rdi, rdo, pool, budget);
         bubble(basel, -1, pool, budget);
     }
 
