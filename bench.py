@@ -6,10 +6,13 @@ merges + triangle extraction, results left in HBM.  `value` is points per
 second over the K timed steps.
 
 Extra fields:
-  roofline      the dominant kernel (the Guibas-Stolfi merge kernel, one
-                launch per kd level): algorithmic quad-edge bytes per launch
-                over its average launch time, measured with HIP events on the
-                engine's stream around every merge launch of the timed steps.
+  roofline      the dominant kernel, k_merge (the Guibas-Stolfi merge of every
+                node of one kd level, one thread per node; one launch per level
+                whose nodes are merged in one piece): algorithmic bytes per
+                launch = 48 B (one 24 B quad-edge record written once and read
+                once) per edge the launch creates, counted on the device, over
+                the launch's average duration, measured with HIP events on the
+                engine's stream around every k_merge launch of the timed steps.
   cpu_baseline  the reference's threaded qdt_execute_parallel, compiled from
                 the reference sources into oracle/_ref/, on the same 10M input,
                 on this host's cores (median of --cpu-reps runs).
@@ -29,6 +32,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def traffic_per_launch():
+    """HBM bytes per k_merge launch from the PMC runs summarised in
+    profiles/k_merge_10M_uniform.md (FETCH_SIZE and WRITE_SIZE, each scaled
+    by a calibration copy); None when that summary is absent."""
+    p = os.path.join(ROOT, "profiles", "k_merge_10M_uniform.md")
+    try:
+        for line in open(p):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -76,7 +93,7 @@ def main():
         assert st == Q.QDT_OK, Q.status_string(st)
     torch.cuda.synchronize()
 
-    step_ms, merge_ms, merge_launches, stats = [], [], [], None
+    step_ms, merge_ms, merge_launches, merge_edges, stats = [], [], [], [], None
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
@@ -86,6 +103,7 @@ def main():
         stats = ctx.stats()
         merge_ms.append(stats["merge_launch_ms_sum"])
         merge_launches.append(stats["merge_launches"])
+        merge_edges.append(stats["merge_edges"])
     torch.cuda.synchronize()
     total_s = time.perf_counter() - t0
     ms_per_step = total_s * 1e3 / args.steps
@@ -94,17 +112,15 @@ def main():
     T = int(stats["triangles"])
     m = int(stats["unique_points"])
     E = 3 * m - 3 - (2 * m - 2 - T)  # Euler: E = 3m - 3 - b with b = 2m - 2 - T
-    # Merge kernel roofline: every final quad-edge record (24 B: four onext +
-    # two origins) is written once and read once by the merges -> 48 B per
-    # edge over the whole merge stage; one launch = one kd level, so the
-    # per-launch algorithmic bytes are 48 * E / launches.
+    # k_merge roofline: each edge a launch creates is a 24 B record (four
+    # onext + two origins) written once and read once -> 48 B per edge.
     launches = merge_launches[-1] or 1
     avg_launch_ms = statistics.mean(merge_ms) / launches
-    bytes_per_launch = 48.0 * E / launches
+    bytes_per_launch = 48.0 * statistics.mean(merge_edges) / launches
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
 
     line = {
-        "metric": "Mpts/s triangulated (10M uniform f64)",
+        "metric": "Mpts/s triangulated (10M uniform f64); batched sets/s; serve p50/p99 ms",
         "value": round(value, 3),
         "unit": "Mpts/s",
         "n_gpus": 1,
@@ -131,16 +147,18 @@ def main():
             },
         },
         "roofline": {
-            "kernel": "k_merge (one launch per kd level)",
+            "kernel": "k_merge (one launch per kd level merged in one piece per node)",
             "bound": "hbm",
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_per_launch(),
             "avg_launch_ms": round(avg_launch_ms, 4),
             "launches_per_step": launches,
+            "edges_per_step": int(merge_edges[-1]),
             "bytes_per_launch": bytes_per_launch,
+            "seam_levels_ms": round(stats["seam_level_ms_sum"], 3),
             "pipeline_device_ms": round(stats["device_ms"], 3),
             "pipeline_alg_bytes": 36 * n + 40 * m + 48 * E + 12 * T,
         },

@@ -32,8 +32,11 @@ typedef struct qdt_gpu_stats
     double device_ms;             /* time of the device pipeline (events) */
     /* filled only with profiling on (qdt_gpu_set_profiling): */
     double stage_ms[8];           /* keys+y sort, x sort, dedupe, frame-1 sort, kd levels, leaves, merges, extract */
-    double merge_launch_ms_sum;   /* sum of the merge kernel launches' durations */
+    double merge_launch_ms_sum;   /* sum of the merge kernel launches' durations (one launch per
+                                     kd level merged in one piece per node) */
     uint64_t merge_launches;
+    uint64_t merge_edges;         /* quad-edge records those launches created */
+    double seam_level_ms_sum;     /* kd levels cut into seam pieces, all their kernels */
     uint64_t seam_retries;        /* runs repeated without seam pieces */
     uint64_t seam_counters[8];    /* anchors certified, anchors inserted, pieces run, pieces
                                      resumed, pieces stopped by a claimed vertex, anchors with a

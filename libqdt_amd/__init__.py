@@ -56,7 +56,8 @@ class GpuStats(ctypes.Structure):
         ("extract_exact", ctypes.c_uint64), ("unique_points", ctypes.c_uint64),
         ("triangles", ctypes.c_uint64), ("kd_levels", ctypes.c_uint64), ("device_ms", ctypes.c_double),
         ("stage_ms", ctypes.c_double * 8), ("merge_launch_ms_sum", ctypes.c_double),
-        ("merge_launches", ctypes.c_uint64), ("seam_retries", ctypes.c_uint64),
+        ("merge_launches", ctypes.c_uint64), ("merge_edges", ctypes.c_uint64),
+        ("seam_level_ms_sum", ctypes.c_double), ("seam_retries", ctypes.c_uint64),
         ("seam_counters", ctypes.c_uint64 * 8),
     ]
     STAGES = ("keys_sort_y", "sort_x", "dedupe", "frame1_sort", "kd_levels", "leaves", "merges", "extract")

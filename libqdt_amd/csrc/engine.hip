@@ -69,6 +69,15 @@ __device__ __forceinline__ void flush_counts(unsigned long long* counters, const
     }
 }
 
+// edges created by the merge kernel (for its algorithmic bytes)
+__device__ __forceinline__ void flush_connects(unsigned long long* counters, const PredCount& pc)
+{
+    unsigned long long x = pc.connects;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    if (__lane_id() == 0 && x) atomicAdd(&counters[13], x);
+}
+
 // ---------------------------------------------------------------------------
 // 1-4: keys, sorts, dedupe, frame-1 order
 // ---------------------------------------------------------------------------
@@ -298,6 +307,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
         }
     }
     flush_counts(counters, q.pc);
+    flush_connects(counters, q.pc);
     if (q.err) atomicOr(err, 2 << q.err);
 }
 
@@ -1390,6 +1400,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     }
     mark(6);
     int epoch = 0; // claim epochs: one per piece launch
+    uint64_t kmerge_levels = 0;
     for (int d = D - 1; d >= 0; --d)
     {
         const uint32_t cnt = 1u << d;
@@ -1397,6 +1408,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
         // piece ids of a level must fit the claim id bits
         const bool fits = (size_t)cnt * kSegSlots < ((size_t)1 << kClaimBits);
         const int K = seams && fits ? seam_anchors((double)(m >> d)) : 0;
+        if (!K)
+            kmerge_levels |= 1ull << d;
         if (!K)
             k_merge<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res,
                                                                     b.err, b.counters);
@@ -1473,6 +1486,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     st_.incircle_exact = cnts[3];
     st_.extract_queue = cnts[4];
     for (int i = 0; i < 8; ++i) st_.seam[i] = cnts[5 + i];
+    st_.merge_edges = cnts[13];
     st_.err_bits = hs[1];
     st_.triangles = hs[0];
     if (prof && evs_.size() >= 9 + 2 * (size_t)D)
@@ -1483,15 +1497,23 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
             (void)hipEventElapsedTime(&t, evs_[i], evs_[i + 1]);
             st_.stage_ms[i] = t;
         }
-        double sum = 0;
+        double sum = 0, seam_sum = 0;
+        uint32_t launches = 0;
         for (int d = 0; d < D; ++d)
         {
             float t = 0;
             (void)hipEventElapsedTime(&t, evs_[9 + 2 * d], evs_[10 + 2 * d]);
-            sum += t;
+            if (kmerge_levels >> d & 1)
+            {
+                sum += t;
+                ++launches;
+            }
+            else
+                seam_sum += t;
         }
         st_.merge_launch_ms_sum = sum;
-        st_.merge_launches = (uint32_t)D;
+        st_.merge_launches = launches;
+        st_.seam_level_ms_sum = seam_sum;
     }
     res_.tris = b.tris;
     res_.triangles = hs[0];

@@ -28,8 +28,10 @@ struct RunStats
     double device_ms;     // first kernel start .. last kernel end
     // with profiling on: time per stage (ms) and the merge launches
     double stage_ms[8];   // keys+sort y, sort x, dedupe, frame-1 sort, kd levels, leaves, merges, extract
-    double merge_launch_ms_sum;
+    double merge_launch_ms_sum; // the merge kernel's launches (levels merged in one piece)
     uint32_t merge_launches;
+    uint64_t merge_edges;       // edges those launches created
+    double seam_level_ms_sum;   // levels cut into seam pieces (all their kernels)
     uint32_t seam_retries; // runs repeated with unsplit merges
     // seam pieces (of the first run): anchors certified, anchors inserted,
     // pieces run, pieces resumed, pieces stopped by a claimed vertex,

@@ -223,6 +223,8 @@ void record_stats(qdt_t* Q)
     for (int i = 0; i < 8; ++i) Q->stats.stage_ms[i] = s.stage_ms[i];
     Q->stats.merge_launch_ms_sum = s.merge_launch_ms_sum;
     Q->stats.merge_launches = s.merge_launches;
+    Q->stats.merge_edges = s.merge_edges;
+    Q->stats.seam_level_ms_sum = s.seam_level_ms_sum;
     Q->stats.seam_retries = s.seam_retries;
     for (int i = 0; i < 8; ++i) Q->stats.seam_counters[i] = s.seam[i];
 }

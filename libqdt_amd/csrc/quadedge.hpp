@@ -45,6 +45,7 @@ struct Pool
 struct PredCount
 {
     unsigned orient, orient_exact, incircle, incircle_exact;
+    unsigned connects; // edges created
 };
 
 // Error codes (`err`): 1 index out of range, 2 no free record, 3 loop
@@ -85,7 +86,7 @@ struct QuadEdges
         claim_me = claim_base = 0;
         h0 = h1 = h2 = h3 = -1;
         err = 0;
-        pc = PredCount{ 0, 0, 0, 0 };
+        pc = PredCount{ 0, 0, 0, 0, 0 };
     }
 
     __device__ __forceinline__ int chk(int e)
@@ -208,6 +209,7 @@ struct QuadEdges
     __device__ __forceinline__ int connect(int a, int b, Pool& p)
     {
         const int e = make_edge(p);
+        ++pc.connects;
         const int o = dest(a), d = org(b);
         set_org(e, o);
         set_org(e ^ 2, d);
