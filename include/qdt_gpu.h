@@ -74,6 +74,55 @@ QDT_API int qdt_gpu_device_count(void);
 /* Frees the device workspace a context keeps between calls. */
 QDT_API void qdt_gpu_release_workspace(qdt_t* ptr);
 
+/* ---- Batched form: many independent point sets in one pipeline pass ----
+ *
+ * No counterpart in the reference (whose callers run one qdt_execute per
+ * set, each on its own context).  Set i is points [offsets[i], offsets[i+1])
+ * of the input; every set is triangulated exactly as qdt_execute would do
+ * it alone (same triangle set, same duplicate rule), but all sets share one
+ * sort, one kd forest and one merge launch per level. */
+typedef struct qdt_gpu_batch qdt_gpu_batch;
+
+QDT_API qdt_gpu_batch* qdt_gpu_batch_create(void);
+QDT_API void qdt_gpu_batch_destroy(qdt_gpu_batch* b);
+
+/* Host input.  offsets holds k + 1 non-decreasing values starting at 0.
+ * Returns QDT_ERR_NULL_CONTEXT (b NULL), QDT_ERR_NULL_ARGUMENT (offsets NULL
+ * with k > 0, or pts NULL with points), QDT_ERR_INVALID_INDEX (offsets not
+ * starting at 0 or decreasing), QDT_ERR_TOO_MANY_POINTS (more than 1e8 points
+ * in all), or the out-of-memory / internal codes.  A set holding a NaN or
+ * infinite coordinate does not fail the call: its own status is
+ * QDT_ERR_NOT_FINITE and it has no triangles.  A failed call empties b. */
+QDT_API qdt_status qdt_gpu_batch_execute(qdt_gpu_batch* b, const qdt_vec2d* pts, const size_t* offsets, size_t k);
+
+/* Device input (d_pts in device memory, offsets on the host). */
+QDT_API qdt_status qdt_gpu_batch_execute_device(qdt_gpu_batch* b, const qdt_vec2d* d_pts, const size_t* offsets,
+                                                size_t k);
+
+/* Number of sets of the last call. */
+QDT_API size_t qdt_gpu_batch_sets(const qdt_gpu_batch* b);
+/* Status of one set: QDT_OK or QDT_ERR_NOT_FINITE (QDT_ERR_INVALID_INDEX
+ * for a set index out of range). */
+QDT_API qdt_status qdt_gpu_batch_set_status(const qdt_gpu_batch* b, size_t set);
+QDT_API size_t qdt_gpu_batch_triangles_size(const qdt_gpu_batch* b, size_t set);
+QDT_API size_t qdt_gpu_batch_unique_points(const qdt_gpu_batch* b, size_t set);
+/* Triangle `id` of a set, as set-local input indices (the set's first point
+ * is 0), counter-clockwise; 1 on success, 0 (T untouched) otherwise. */
+QDT_API size_t qdt_gpu_batch_get_triangle(const qdt_gpu_batch* b, size_t set, size_t T[3], size_t id);
+/* Where a set's triangles lie in the bulk array: first triangle and count. */
+QDT_API int qdt_gpu_batch_set_range(const qdt_gpu_batch* b, size_t set, size_t* first, size_t* count);
+/* All triangles (3 * total uint32, set-local indices, each set's triangles
+ * contiguous at its range), to host memory; returns the total. */
+QDT_API size_t qdt_gpu_batch_copy_triangles(const qdt_gpu_batch* b, uint32_t* out);
+/* The same array on the device, valid until the next call on b. */
+QDT_API size_t qdt_gpu_batch_device_triangles(const qdt_gpu_batch* b, const uint32_t** d_tris);
+/* Lowest set-local index with the same coordinates, for every input point
+ * (n uint32, in input order); returns n. */
+QDT_API size_t qdt_gpu_batch_copy_rep(const qdt_gpu_batch* b, uint32_t* out);
+QDT_API int qdt_gpu_batch_get_stats(const qdt_gpu_batch* b, qdt_gpu_stats* out);
+QDT_API void qdt_gpu_batch_set_profiling(qdt_gpu_batch* b, int enable);
+QDT_API void qdt_gpu_batch_set_seams(qdt_gpu_batch* b, int enable);
+
 #ifdef __cplusplus
 }
 #endif

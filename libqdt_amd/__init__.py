@@ -38,6 +38,11 @@ EXPORTS = [
 GPU_EXPORTS = [
     "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
     "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_device_count", "qdt_gpu_release_workspace",
+    "qdt_gpu_batch_create", "qdt_gpu_batch_destroy", "qdt_gpu_batch_execute", "qdt_gpu_batch_execute_device",
+    "qdt_gpu_batch_sets", "qdt_gpu_batch_set_status", "qdt_gpu_batch_triangles_size",
+    "qdt_gpu_batch_unique_points", "qdt_gpu_batch_get_triangle", "qdt_gpu_batch_set_range",
+    "qdt_gpu_batch_copy_triangles", "qdt_gpu_batch_device_triangles", "qdt_gpu_batch_copy_rep",
+    "qdt_gpu_batch_get_stats", "qdt_gpu_batch_set_profiling", "qdt_gpu_batch_set_seams",
 ]
 
 
@@ -124,6 +129,35 @@ def lib():
         L.qdt_gpu_device_count.argtypes = []
         L.qdt_gpu_release_workspace.restype = None
         L.qdt_gpu_release_workspace.argtypes = [vp]
+        # batched form
+        L.qdt_gpu_batch_create.restype = vp
+        L.qdt_gpu_batch_create.argtypes = []
+        L.qdt_gpu_batch_destroy.restype = None
+        L.qdt_gpu_batch_destroy.argtypes = [vp]
+        for f in ("qdt_gpu_batch_execute", "qdt_gpu_batch_execute_device"):
+            getattr(L, f).restype = st
+            getattr(L, f).argtypes = [vp, vp, ctypes.POINTER(sz), sz]
+        L.qdt_gpu_batch_sets.restype = sz
+        L.qdt_gpu_batch_sets.argtypes = [vp]
+        L.qdt_gpu_batch_set_status.restype = st
+        L.qdt_gpu_batch_set_status.argtypes = [vp, sz]
+        for f in ("qdt_gpu_batch_triangles_size", "qdt_gpu_batch_unique_points"):
+            getattr(L, f).restype = sz
+            getattr(L, f).argtypes = [vp, sz]
+        L.qdt_gpu_batch_get_triangle.restype = sz
+        L.qdt_gpu_batch_get_triangle.argtypes = [vp, sz, ctypes.POINTER(sz), sz]
+        L.qdt_gpu_batch_set_range.restype = ctypes.c_int
+        L.qdt_gpu_batch_set_range.argtypes = [vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        for f in ("qdt_gpu_batch_copy_triangles", "qdt_gpu_batch_copy_rep"):
+            getattr(L, f).restype = sz
+            getattr(L, f).argtypes = [vp, vp]
+        L.qdt_gpu_batch_device_triangles.restype = sz
+        L.qdt_gpu_batch_device_triangles.argtypes = [vp, ctypes.POINTER(vp)]
+        L.qdt_gpu_batch_get_stats.restype = ctypes.c_int
+        L.qdt_gpu_batch_get_stats.argtypes = [vp, ctypes.POINTER(GpuStats)]
+        for f in ("qdt_gpu_batch_set_profiling", "qdt_gpu_batch_set_seams"):
+            getattr(L, f).restype = None
+            getattr(L, f).argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -244,6 +278,111 @@ class Triangulation:
 
     def release_workspace(self):
         lib().qdt_gpu_release_workspace(self._h)
+
+
+class Batch:
+    """Many independent point sets in one GPU pass (include/qdt_gpu.h,
+    qdt_gpu_batch_*).  Set i is points [offsets[i], offsets[i+1])."""
+
+    def __init__(self):
+        self._h = lib().qdt_gpu_batch_create()
+        if not self._h:
+            raise MemoryError("qdt_gpu_batch_create returned NULL")
+        self._k = 0
+
+    def close(self):
+        if self._h:
+            lib().qdt_gpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def _offsets(offsets):
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        return o, o.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)), max(len(o) - 1, 0)
+
+    def execute(self, pts, offsets):
+        """Host points (n, 2) and k + 1 offsets."""
+        a = _as_points(pts)
+        o, op, k = self._offsets(offsets)
+        st = int(lib().qdt_gpu_batch_execute(self._h, a.ctypes.data if len(a) else None, op, k))
+        self._k = k if st == QDT_OK else 0
+        return st
+
+    def execute_device(self, dev_ptr, offsets):
+        """Points already in GPU memory; offsets on the host."""
+        o, op, k = self._offsets(offsets)
+        st = int(lib().qdt_gpu_batch_execute_device(self._h, ctypes.c_void_p(int(dev_ptr)), op, k))
+        self._k = k if st == QDT_OK else 0
+        return st
+
+    def sets(self):
+        return int(lib().qdt_gpu_batch_sets(self._h))
+
+    def status(self, i):
+        return int(lib().qdt_gpu_batch_set_status(self._h, i))
+
+    def triangles_size(self, i):
+        return int(lib().qdt_gpu_batch_triangles_size(self._h, i))
+
+    def unique_points(self, i):
+        return int(lib().qdt_gpu_batch_unique_points(self._h, i))
+
+    def get_triangle(self, i, t):
+        T = (ctypes.c_size_t * 3)()
+        ok = lib().qdt_gpu_batch_get_triangle(self._h, i, T, t)
+        return (T[0], T[1], T[2]) if ok else None
+
+    def set_range(self, i):
+        a, c = ctypes.c_size_t(), ctypes.c_size_t()
+        ok = lib().qdt_gpu_batch_set_range(self._h, i, ctypes.byref(a), ctypes.byref(c))
+        return (a.value, c.value) if ok else None
+
+    def all_triangles(self):
+        """(total, 3) uint32: every set's triangles at its set_range()."""
+        p = ctypes.c_void_p()
+        total = int(lib().qdt_gpu_batch_device_triangles(self._h, ctypes.byref(p)))
+        out = np.zeros((total, 3), dtype=np.uint32)
+        if total:
+            got = lib().qdt_gpu_batch_copy_triangles(self._h, out.ctypes.data)
+            assert got == total
+        return out
+
+    def triangles(self, i, all_tris=None):
+        """(T, 3) uint32 triangles of set i, set-local indices."""
+        r = self.set_range(i)
+        if r is None:
+            return None
+        a = self.all_triangles() if all_tris is None else all_tris
+        return a[r[0]:r[0] + r[1]]
+
+    def rep(self, n):
+        out = np.zeros(n, dtype=np.uint32)
+        if n:
+            lib().qdt_gpu_batch_copy_rep(self._h, out.ctypes.data)
+        return out
+
+    def stats(self):
+        s = GpuStats()
+        lib().qdt_gpu_batch_get_stats(self._h, ctypes.byref(s))
+        return s.as_dict()
+
+    def set_profiling(self, on=True):
+        lib().qdt_gpu_batch_set_profiling(self._h, 1 if on else 0)
+
+    def set_seams(self, on=True):
+        lib().qdt_gpu_batch_set_seams(self._h, 1 if on else 0)
 
 
 def triangulate(pts):

@@ -101,15 +101,21 @@ __global__ __launch_bounds__(kBlock) void k_gather_u64(const uint32_t* idx, cons
     if (i < n) dst[i] = src[idx[i]];
 }
 
+// Sorted position p starts a new distinct point.  One set: the sort's last
+// keys are x keys.  Several sets: the last keys are set ranks and x keys are
+// read by input index.
 struct DedupeFlag
 {
-    const uint64_t* kx_sorted;
+    const uint64_t* k_sorted;
     const uint32_t* idx;
     const uint64_t* ky;
+    const uint64_t* kx; // by input index, when k_sorted holds ranks (else null)
     __device__ uint32_t operator()(uint32_t p) const
     {
         if (p == 0) return 1;
-        return (kx_sorted[p] != kx_sorted[p - 1] || ky[idx[p]] != ky[idx[p - 1]]) ? 1 : 0;
+        const uint32_t i = idx[p], h = idx[p - 1];
+        if (k_sorted[p] != k_sorted[p - 1] || ky[i] != ky[h]) return 1;
+        return (kx && kx[i] != kx[h]) ? 1 : 0;
     }
 };
 
@@ -127,7 +133,7 @@ struct StoreU32
 
 __global__ __launch_bounds__(kBlock) void k_dedupe_write(const double2* pts, const uint32_t* idx, DedupeFlag f,
                                                          const uint32_t* dist, uint32_t n, double2* P0,
-                                                         uint32_t* ID0)
+                                                         uint32_t* ID0, uint32_t* vrank)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
@@ -137,7 +143,131 @@ __global__ __launch_bounds__(kBlock) void k_dedupe_write(const double2* pts, con
         const uint32_t i = idx[p];
         P0[d] = pts[i];
         ID0[d] = i;
+        if (f.kx) vrank[d] = (uint32_t)f.k_sorted[p];
     }
+}
+
+// ---- several point sets in one run ----
+
+// set of input index i: the last s with soff[s] <= i
+__device__ __forceinline__ int set_of(const uint32_t* soff, int k, uint32_t i)
+{
+    int lo = 0, hi = k - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (soff[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_sets(const double2* pts, uint32_t n, const uint32_t* soff, int k,
+                                                       uint32_t* setbad)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double2 p = pts[i];
+    if (!isfinite(p.x) || !isfinite(p.y)) atomicOr(&setbad[set_of(soff, k, i)], 1u);
+}
+
+// rank of every input point's set (the sets' order in the kd forest)
+__global__ __launch_bounds__(kBlock) void k_input_rank(uint32_t n, const uint32_t* soff, int k, const uint32_t* srank,
+                                                       uint32_t* rin)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) rin[i] = srank[set_of(soff, k, i)];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_rank(const uint32_t* idx, const uint32_t* rank, uint32_t n,
+                                                        uint64_t* key)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) key[i] = rank[idx[i]];
+}
+
+// first distinct point of every rank; rpos[r] = first sorted position of rank r
+__global__ __launch_bounds__(kBlock) void k_vstart(const uint32_t* dist, const uint32_t* rpos, int nr, uint32_t n,
+                                                   const uint32_t* m_total, uint32_t* vstart)
+{
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r > nr) return;
+    const uint32_t p = rpos[r];
+    vstart[r] = p < n ? dist[p] : *m_total;
+}
+
+// rank of every triangle (by its first vertex's input index)
+__global__ __launch_bounds__(kBlock) void k_tri_rank(const uint32_t* tris, uint32_t T, const uint32_t* rin,
+                                                     uint32_t* trank)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < T) trank[i] = rin[tris[3 * i]];
+}
+
+// lower bounds of ranks 0..nr in a rank-sorted key array
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_rank_bounds(const K* keys, uint32_t len, int nr, uint32_t* out)
+{
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r > nr) return;
+    uint32_t lo = 0, hi = len;
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)keys[mid] < (uint64_t)r)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[r] = lo;
+}
+
+// triangles grouped by rank, with set-local indices.  Records below 3m gave
+// their triangles in rank order already (records of a set lie in its points'
+// record range); the spare records' triangles were sorted by rank.
+__global__ __launch_bounds__(kBlock) void k_batch_tris(const uint32_t* tris, uint32_t T, uint32_t tmain,
+                                                       const uint32_t* trank, const uint64_t* skey,
+                                                       const uint32_t* sidx, const uint32_t* mstart,
+                                                       const uint32_t* sstart, const uint32_t* toff,
+                                                       const uint32_t* in_off, uint32_t* out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= T) return;
+    uint32_t src, dst, r;
+    if (i < tmain)
+    {
+        r = trank[i];
+        src = i;
+        dst = toff[r] + (i - mstart[r]);
+    }
+    else
+    {
+        const uint32_t j = i - tmain;
+        r = (uint32_t)skey[j];
+        src = tmain + sidx[j];
+        dst = toff[r] + (mstart[r + 1] - mstart[r]) + (j - sstart[r]);
+    }
+    const uint32_t base = in_off[r];
+    out[3 * dst] = tris[3 * src] - base;
+    out[3 * dst + 1] = tris[3 * src + 1] - base;
+    out[3 * dst + 2] = tris[3 * src + 2] - base;
+}
+
+__global__ __launch_bounds__(kBlock) void k_spare_keys(const uint32_t* trank, uint32_t c, uint64_t* key,
+                                                       uint32_t* idx)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= c) return;
+    key[j] = trank[j];
+    idx[j] = j;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rep_local(uint32_t* rep, uint32_t n, const uint32_t* soff, int k)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) rep[i] -= soff[set_of(soff, k, i)];
 }
 
 __global__ __launch_bounds__(kBlock) void k_rep(const uint32_t* idx, DedupeFlag f, const uint32_t* dist,
@@ -175,23 +305,43 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* a, uint32_t n, ui
 // 5: kd levels
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ size_t off_base(int d) { return ((size_t)1 << d) - 1 + (size_t)d; }
-__device__ __forceinline__ size_t node_index(int d, uint32_t j) { return ((size_t)1 << d) - 1 + j; }
+// The kd forest.  Level d holds the children (2j, 2j + 1) of level d-1's
+// nodes, followed by the roots of the point sets whose trees start at level
+// d (a single set: one root at level 0).  The offsets buffer starts with a
+// level table, int4 per level: node count, offset of the level's node
+// offsets (count + 1 of them) after the table, index of its first node.
+constexpr int kMaxLevels = 64;
+constexpr int kLvlHdr = 4 * kMaxLevels;
+__device__ __forceinline__ uint32_t lcount(const int* off, int d) { return (uint32_t)off[4 * d]; }
+__device__ __forceinline__ size_t off_base(const int* off, int d) { return kLvlHdr + (size_t)off[4 * d + 1]; }
+__device__ __forceinline__ size_t node_index(const int* off, int d, uint32_t j) { return (size_t)off[4 * d + 2] + j; }
 __device__ __forceinline__ int left_size(int s) { return s >= 4 ? s / 2 : s; }
 
-__global__ void k_root(int* off, uint32_t m)
+// node offsets of the roots at level d: the point sets of ranks [ra, rb),
+// whose distinct points start at vstart[rank]; they follow the children of
+// level d-1 (or start the level)
+__global__ __launch_bounds__(kBlock) void k_roots(int* off, int d, int ra, int rb, const uint32_t* vstart)
 {
-    off[0] = 0;
-    off[1] = (int)m;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int nr = rb - ra;
+    if (i > nr) return;
+    int* od = off + off_base(off, d);
+    const uint32_t first = d > 0 ? 2 * lcount(off, d - 1) : 0;
+    if (i == 0)
+    {
+        if (first == 0) od[0] = (int)vstart[ra];
+        return;
+    }
+    od[first + i] = (int)vstart[ra + i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_children(int* off, int d)
 {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t cnt = 1u << d;
+    const uint32_t cnt = lcount(off, d);
     if (j >= cnt) return;
-    const int* od = off + off_base(d);
-    int* oc = off + off_base(d + 1);
+    const int* od = off + off_base(off, d);
+    int* oc = off + off_base(off, d + 1);
     const int o = od[j];
     const int s = od[j + 1] - o;
     oc[2 * j] = o;
@@ -199,13 +349,35 @@ __global__ __launch_bounds__(kBlock) void k_children(int* off, int d)
     if (j == cnt - 1) oc[2 * cnt] = od[cnt];
 }
 
+// node ids at level d+1 of the points of the sets rooted there (ranks
+// [ra, rb), positions [vstart[ra], vstart[rb])); until then they belong to
+// no node
+__global__ __launch_bounds__(kBlock) void k_root_nodes(const int* off, int d1, int ra, int rb,
+                                                       const uint32_t* vstart, uint32_t first, uint32_t* node)
+{
+    const uint32_t p = vstart[ra] + blockIdx.x * kBlock + threadIdx.x;
+    if (p >= vstart[rb]) return;
+    int lo = ra, hi = rb - 1; // the rank whose range holds p
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (vstart[mid] <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    node[p] = first + (uint32_t)(lo - ra);
+    (void)off;
+    (void)d1;
+}
+
 // positional split of the array sorted in the cut's frame
 __global__ __launch_bounds__(kBlock) void k_split(const uint32_t* Bf, const uint32_t* node, const int* off, int d,
                                                   uint32_t m, uint32_t* node_next, uint8_t* bits)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= m) return;
-    const int* od = off + off_base(d);
+    const int* od = off + off_base(off, d);
+    if (p >= (uint32_t)od[lcount(off, d)]) return; // points of sets rooted deeper
     const uint32_t j = node[p];
     const int o = od[j];
     const int s = od[j + 1] - o;
@@ -228,11 +400,16 @@ __global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const 
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= m) return;
-    const int* od = off + off_base(d);
+    const int* od = off + off_base(off, d);
+    const uint32_t v = Bo[p];
+    if (p >= (uint32_t)od[lcount(off, d)])
+    {
+        Bout[p] = v; // sets rooted deeper keep their order
+        return;
+    }
     const uint32_t j = node[p];
     const int o = od[j];
     const int s = od[j + 1] - o;
-    const uint32_t v = Bo[p];
     const int z = (int)(Z[p] - Z[o]);
     const int pos = bits[v] ? o + left_size(s) + ((int)p - o - z) : o + z;
     Bout[pos] = v;
@@ -252,19 +429,19 @@ __global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const 
     const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
-    if (j < (1u << D))
+    if (j < lcount(off, D))
     {
-        const int* oD = off + off_base(D);
+        const int* oD = off + off_base(off, D);
         const int o = oD[j];
         const int s = oD[j + 1] - o;
-        if (s >= 2)
+        if (s >= 1)
         {
             int db = D;
             uint32_t jb = j;
-            while (db > 0)
+            while (db > 0 && jb < 2 * lcount(off, db - 1)) // roots have no parent
             {
                 const uint32_t pj = jb >> 1;
-                const int* op = off + off_base(db - 1);
+                const int* op = off + off_base(off, db - 1);
                 if (op[pj + 1] - op[pj] >= 4) break;
                 --db;
                 jb = pj;
@@ -276,7 +453,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const 
             int first, second;
             Pool pool;
             q.leaf(v, s, 3 * o, first, second, pool);
-            res[node_index(db, jb)] = make_int4(first, second, pool.head, pool.tail);
+            res[node_index(off, db, jb)] = make_int4(first, second, pool.head, pool.tail);
         }
     }
     flush_counts(counters, q.pc);
@@ -290,20 +467,20 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
     const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
-    if (j < (1u << d))
+    if (j < lcount(off, d))
     {
-        const int* od = off + off_base(d);
+        const int* od = off + off_base(off, d);
         const int s = od[j + 1] - od[j];
         if (s >= 4)
         {
-            const int4 L = res[node_index(d + 1, 2 * j)];
-            const int4 R = res[node_index(d + 1, 2 * j + 1)];
+            const int4 L = res[node_index(off, d + 1, 2 * j)];
+            const int4 R = res[node_index(off, d + 1, 2 * j + 1)];
             Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
             const int axis = d & 1;
             const int out_axis = d > 0 ? ((d - 1) & 1) : 0;
             int first, second;
             q.merge_node(L.x, L.y, R.x, R.y, axis, out_axis, s, first, second, pool);
-            res[node_index(d, j)] = make_int4(first, second, pool.head, pool.tail);
+            res[node_index(off, d, j)] = make_int4(first, second, pool.head, pool.tail);
         }
     }
     flush_counts(counters, q.pc);
@@ -324,19 +501,28 @@ constexpr int kPhases = 2;             // pieces k, k+2, ... run together
 constexpr int kClaimBits = 20;         // piece id bits of a claim (the rest: launch epoch)
 constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit merges
 
+// number of anchors for a node of s points (0: one piece)
+__host__ __device__ inline int seam_anchors(double s)
+{
+    if (s < kSegMin) return 0;
+    int k = (int)(std::sqrt(s) / 20.0);
+    return k < 2 ? 2 : (k > kSegMaxK ? kSegMaxK : k);
+}
+
 // split vertex of every internal node; anchor heights of segmented levels
 __global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const uint32_t* Bo, const int* off, int d,
                                                       int K, int* splitv, int* qv)
 {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= (1u << d)) return;
-    const int* od = off + off_base(d);
+    if (j >= lcount(off, d)) return;
+    const int* od = off + off_base(off, d);
     const int o = od[j];
     const int s = od[j + 1] - o;
     if (s < 4) return;
-    const size_t ni = node_index(d, j);
+    const size_t ni = node_index(off, d, j);
     splitv[ni] = (int)Bf[o + left_size(s)];
-    for (int k = 0; k < K; ++k) qv[ni * kSegMaxK + k] = (int)Bo[o + (int)(((long long)(2 * k + 1) * s) / (2 * K))];
+    const int Kn = K ? min(K, seam_anchors((double)s)) : 0;
+    for (int k = 0; k < Kn; ++k) qv[ni * kSegMaxK + k] = (int)Bo[o + (int)(((long long)(2 * k + 1) * s) / (2 * Kn))];
 }
 
 __device__ __forceinline__ bool less_frame_pt(double2 a, double2 b, int frame)
@@ -351,14 +537,14 @@ __device__ int kd_descend(const int* off, const int* splitv, const double2* PT, 
 {
     while (dd < D)
     {
-        const int* od = off + off_base(dd);
+        const int* od = off + off_base(off, dd);
         const int s = od[jj + 1] - od[jj];
         if (s <= 3) break;
-        const double2 sp = PT[splitv[node_index(dd, jj)]];
+        const double2 sp = PT[splitv[node_index(off, dd, jj)]];
         jj = 2 * jj + (less_frame_pt(p, sp, dd & 1) ? 0u : 1u);
         ++dd;
     }
-    const int* od = off + off_base(dd);
+    const int* od = off + off_base(off, dd);
     const int o = od[jj];
     return od[jj + 1] > o ? (int)B0[o] : -1;
 }
@@ -375,17 +561,17 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
     const uint32_t t = blockIdx.x * kMergeBlock + threadIdx.x;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
-    if (t < (1u << d) * (uint32_t)K)
+    if (t < lcount(off, d) * (uint32_t)K)
     {
         const uint32_t j = t / K;
         const int k = (int)(t % K);
         int state = 0;
-        const int* od = off + off_base(d);
+        const int* od = off + off_base(off, d);
         const int s = od[j + 1] - od[j];
-        if (s >= 4)
+        if (s >= 4 && k < seam_anchors((double)s))
         {
             const int axis = d & 1;
-            const size_t ni = node_index(d, j);
+            const size_t ni = node_index(off, d, j);
             const double c = frame_x(PT[splitv[ni]], axis);
             const double Y = frame_y(PT[qv[ni * kSegMaxK + k]], axis);
             double2 p = from_frame(c, Y, axis);
@@ -458,14 +644,14 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_tangent(int* NX, int* OR, c
     q.spare = spare;
     q.spare_base = spare_base;
     q.spare_cap = spare_cap;
-    if (j < (1u << d))
+    if (j < lcount(off, d))
     {
-        const int* od = off + off_base(d);
+        const int* od = off + off_base(off, d);
         const int s = od[j + 1] - od[j];
         if (s >= 4)
         {
-            const int4 L = res[node_index(d + 1, 2 * j)];
-            const int4 R = res[node_index(d + 1, 2 * j + 1)];
+            const int4 L = res[node_index(off, d + 1, 2 * j)];
+            const int4 R = res[node_index(off, d + 1, 2 * j + 1)];
             Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
             int ldo = L.x, rdo = R.y;
             int budget = 16 * s + 64;
@@ -490,9 +676,9 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_select(const int* OR, const
 {
     const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
     if (j == 0) *alist_n = 0;
-    if (j >= (1u << d)) return;
+    if (j >= lcount(off, d)) return;
     nsel[j] = 0;
-    const int* od = off + off_base(d);
+    const int* od = off + off_base(off, d);
     const int s = od[j + 1] - od[j];
     int cnt = 0;
     if (s >= 4)
@@ -755,8 +941,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_compact(int d, const int* s
                                                              int* nseg, const int* off)
 {
     const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
-    if (j >= (1u << d)) return;
-    const int* od = off + off_base(d);
+    if (j >= lcount(off, d)) return;
+    const int* od = off + off_base(off, d);
     const int s = od[j + 1] - od[j];
     int cnt = 0;
     if (s >= 4)
@@ -789,7 +975,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const d
     q.spare_cap = spare_cap;
     const uint32_t j = t / kSegSlots;
     const int i = (int)(t % kSegSlots);
-    if (j < (1u << d) && i % kPhases == parity)
+    if (j < lcount(off, d) && i % kPhases == parity)
     {
         const int ns = nseg[j];
         if (i < ns)
@@ -797,7 +983,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const d
             q.claim = claim;
             q.claim_base = epoch << kClaimBits;
             q.claim_me = q.claim_base | (int)t;
-            const int* od = off + off_base(d);
+            const int* od = off + off_base(off, d);
             const int s = od[j + 1] - od[j];
             const int* st = segstart + (size_t)j * kSegSlots;
             int basel = st[i];
@@ -838,10 +1024,10 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_resume(int* NX, int* OR, co
     q.spare = spare;
     q.spare_base = spare_base;
     q.spare_cap = spare_cap;
-    if (j < (1u << d))
+    if (j < lcount(off, d))
     {
         const int ns = nseg[j];
-        const int* od = off + off_base(d);
+        const int* od = off + off_base(off, d);
         const int s = od[j + 1] - od[j];
         int budget = 16 * s + 64;
         for (int i = 0; i < ns && !q.err; ++i)
@@ -871,9 +1057,9 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_final(int* NX, int* OR, con
     const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
-    if (j < (1u << d))
+    if (j < lcount(off, d))
     {
-        const int* od = off + off_base(d);
+        const int* od = off + off_base(off, d);
         const int s = od[j + 1] - od[j];
         if (s >= 4)
         {
@@ -894,20 +1080,13 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_final(int* NX, int* OR, con
             }
             int first, second;
             q.rewalk(ns.x, ns.y, d & 1, d > 0 ? ((d - 1) & 1) : 0, s, first, second);
-            res[node_index(d, j)] = make_int4(first, second, pool.head, pool.tail);
+            res[node_index(off, d, j)] = make_int4(first, second, pool.head, pool.tail);
         }
     }
     flush_counts(counters, q.pc);
     if (q.err) atomicOr(err, 2 << q.err);
 }
 
-// number of anchors for a level whose nodes hold about s points (0: one piece)
-int seam_anchors(double s)
-{
-    if (s < kSegMin) return 0;
-    int k = (int)(std::sqrt(s) / 20.0);
-    return k < 2 ? 2 : (k > kSegMaxK ? kSegMaxK : k);
-}
 
 // ---------------------------------------------------------------------------
 // 8: extraction
@@ -1004,8 +1183,11 @@ struct FaceWrite
     uint32_t* tris;
     uint32_t m;
     int lim_e;
+    uint32_t split;  // first spare record: where the set-grouped triangles end
+    uint32_t* tmain; // receives the count of triangles before `split`
     __device__ void operator()(uint32_t k, uint32_t pre) const
     {
+        if (k == split) *tmain = pre;
         const uint32_t fl = face[k] & 3u;
         if (!fl) return;
         for (int r = 0; r < 4; r += 2)
@@ -1067,30 +1249,128 @@ int kd_depth(uint32_t m)
 
 struct Engine::Bufs
 {
+    // point-proportional (ensure)
     uint64_t *kA, *kB, *kx, *ky;
     uint32_t *vA, *vB, *dist, *ID0, *rep, *B0, *B1, *Bt, *nodeA, *nodeB, *Z, *tris, *face, *queue;
-    uint32_t *partials, *hist, *ghist;
+    uint32_t *partials, *hist, *ghist, *vrank, *rin, *one_set;
     double2* P0;
     uint8_t* bits;
-    int *NX, *OR, *off;
-    int4* res;
+    int *NX, *OR;
     unsigned long long* counters;
     int* err;
-    // seam pieces
-    int *VE, *splitv, *qv, *segstart, *nseg, *segresume, *claim, *owner;
+    int *VE, *claim, *owner;
+    unsigned* spare;
+    size_t rec_cap; // quad-edge records allocated (3N + spare)
+    uint32_t qcap;
+    // node-proportional (ensure_nodes)
+    int* off; // level table, then every level's node offsets
+    int4* res;
+    int *splitv, *qv, *segstart, *nseg, *segresume;
     int *arec, *astate, *aedge, *selk, *nsel, *alist, *alist_n;
     double4* acoord;
     double* akey;
     int2 *apool, *segpool;
     int4* nstate;
-    unsigned* spare;
-    size_t rec_cap;   // quad-edge records allocated (3N + spare)
-    size_t seg_nodes; // node slots of the segmented levels
-    size_t anchor_cap;
-    uint32_t qcap;
-    int dcap;
-    uint32_t* host_scalars; // pinned
+    size_t nodes_cap, offs_cap, seg_nodes, anchor_cap;
+    // per set (ensure_sets)
+    uint32_t *soff, *srank, *rpos, *vstart, *setbad, *in_off, *mstart, *sstart, *toff, *trank, *btris, *tmain;
+    size_t sets_cap, tris_cap;
+    // pinned host staging
+    uint32_t* host_scalars;
+    int* host_table;
+    uint32_t* host_sets;
+    size_t host_sets_cap;
 };
+
+// kd forest plan: every level's node count and the ranks rooted there
+struct Engine::Plan
+{
+    int D = 0;                // leaf level
+    std::vector<int> count;   // nodes per level
+    std::vector<int> offb;    // first node offset of the level (after the table)
+    std::vector<int> nodeb;   // first node index of the level
+    std::vector<int> ra;      // ranks [ra[d], ra[d+1]) are rooted at level d
+    std::vector<double> smax; // largest node of the level (bound)
+    std::vector<int> K;       // anchors per node at the level (0: not cut into pieces)
+    size_t nodes = 0, offs = 0, seg_nodes = 1, anchor_cap = 1;
+};
+
+namespace
+{
+
+// sizes[r], roots[r]: size bound and root level of the set of rank r (roots
+// ascending); D: leaf level
+void make_plan(Engine::Plan& P, const std::vector<uint32_t>& sizes, const std::vector<int>& roots, int D, bool seams)
+{
+    P.D = D;
+    P.count.assign(D + 1, 0);
+    P.offb.assign(D + 1, 0);
+    P.nodeb.assign(D + 1, 0);
+    P.ra.assign(D + 2, (int)sizes.size());
+    P.smax.assign(D + 1, 0.0);
+    P.K.assign(D + 1, 0);
+    for (int r = (int)sizes.size() - 1; r >= 0; --r) P.ra[roots[r]] = r;
+    for (int d = D; d >= 0; --d) P.ra[d] = std::min(P.ra[d], P.ra[d + 1]);
+    size_t offs = 0, nodes = 0;
+    for (int d = 0; d <= D; ++d)
+    {
+        P.count[d] = (d ? 2 * P.count[d - 1] : 0) + (P.ra[d + 1] - P.ra[d]);
+        P.offb[d] = (int)offs;
+        P.nodeb[d] = (int)nodes;
+        offs += (size_t)P.count[d] + 1;
+        nodes += (size_t)P.count[d];
+    }
+    for (size_t r = 0; r < sizes.size(); ++r)
+    {
+        double sz = sizes[r];
+        for (int d = roots[r]; d <= D; ++d)
+        {
+            P.smax[d] = std::max(P.smax[d], sz);
+            if (sz >= 4) sz = std::ceil(sz / 2);
+        }
+    }
+    P.nodes = nodes;
+    P.offs = offs;
+    P.seg_nodes = 1;
+    P.anchor_cap = 1;
+    for (int d = 0; d < D; ++d)
+    {
+        const bool fits = (size_t)P.count[d] * kSegSlots < ((size_t)1 << kClaimBits);
+        P.K[d] = seams && fits ? seam_anchors(P.smax[d]) : 0;
+        if (!P.K[d]) continue;
+        P.seg_nodes = std::max(P.seg_nodes, (size_t)P.count[d]);
+        P.anchor_cap = std::max(P.anchor_cap, (size_t)P.count[d] * P.K[d]);
+    }
+}
+
+struct Item
+{
+    void** p;
+    size_t bytes;
+};
+
+constexpr size_t kHostSetArrays = 8;
+
+template <size_t N>
+Status carve(const Item (&items)[N], char*& region, size_t& bytes)
+{
+    size_t total = 0;
+    for (const Item& it : items) total += (it.bytes + 255) & ~(size_t)255;
+    void* ws = nullptr;
+    const hipError_t e = hipMalloc(&ws, total);
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal;
+    region = (char*)ws;
+    bytes = total;
+    size_t at = 0;
+    for (const Item& it : items)
+    {
+        *it.p = region + at;
+        at += (it.bytes + 255) & ~(size_t)255;
+    }
+    return Status::ok;
+}
+
+} // namespace
 
 Engine::Engine(hipStream_t stream) : stream_(stream), own_stream_(stream == nullptr)
 {
@@ -1117,7 +1397,12 @@ Engine::~Engine()
 {
     for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
     release();
-    if (b_ && b_->host_scalars) (void)hipHostFree(b_->host_scalars);
+    if (b_)
+    {
+        if (b_->host_scalars) (void)hipHostFree(b_->host_scalars);
+        if (b_->host_table) (void)hipHostFree(b_->host_table);
+        if (b_->host_sets) (void)hipHostFree(b_->host_sets);
+    }
     delete b_;
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -1126,12 +1411,20 @@ Engine::~Engine()
 
 void Engine::release()
 {
-    if (ws_) (void)hipFree(ws_);
-    ws_ = nullptr;
-    ws_bytes_ = 0;
+    for (char** r : { &ws_, &ws_nodes_, &ws_sets_ })
+    {
+        if (*r) (void)hipFree(*r);
+        *r = nullptr;
+    }
+    ws_bytes_ = ws_nodes_bytes_ = ws_sets_bytes_ = 0;
     cap_n_ = 0;
+    b_->nodes_cap = b_->offs_cap = b_->seg_nodes = b_->anchor_cap = 0;
+    b_->sets_cap = b_->tris_cap = 0;
     res_ = DeviceResult();
+    batch_ = BatchResult();
 }
+
+size_t Engine::workspace_bytes() const { return ws_bytes_ + ws_nodes_bytes_ + ws_sets_bytes_; }
 
 Status Engine::ensure(size_t n)
 {
@@ -1141,6 +1434,8 @@ Status Engine::ensure(size_t n)
         void* h = nullptr;
         QDT_HIP(hipHostMalloc(&h, 64, hipHostMallocDefault));
         b_->host_scalars = (uint32_t*)h;
+        QDT_HIP(hipHostMalloc(&h, 4 * kLvlHdr, hipHostMallocDefault));
+        b_->host_table = (int*)h;
     }
     if (!ev0_)
     {
@@ -1148,31 +1443,16 @@ Status Engine::ensure(size_t n)
         QDT_HIP(hipEventCreate(&ev1_));
     }
     if (n <= cap_n_ && ws_) return Status::ok;
-    release();
+    if (ws_) (void)hipFree(ws_);
+    ws_ = nullptr;
+    ws_bytes_ = 0;
+    cap_n_ = 0;
     const size_t N = std::max<size_t>(n, 64);
-    const int D = kd_depth((uint32_t)N);
-    const size_t nodes = ((size_t)2 << D) + 64;
-    const size_t offs = ((size_t)2 << D) + (size_t)D + 64;
     const uint32_t nb_sort = cdiv(N, kSortTile);
     const size_t spare_recs = N / 8 + 65536;
     const size_t R = 3 * N + spare_recs;
     const uint32_t nb_scan = cdiv(R, kScanTile) + 1;
     const size_t qcap = std::max<size_t>(4096, N / 4);
-    // segmented levels: nodes of at least kSegMin points
-    size_t seg_nodes = 1, anchor_cap = 1;
-    for (int d = 0; d < 31; ++d)
-    {
-        const double sz = (double)N / (double)(1u << d);
-        const int K = seam_anchors(sz);
-        if (!K) break;
-        seg_nodes = (size_t)2 << d;
-        anchor_cap = std::max(anchor_cap, ((size_t)1 << d) * K);
-    }
-    struct Item
-    {
-        void** p;
-        size_t bytes;
-    };
     Bufs& b = *b_;
     const Item items[] = {
         { (void**)&b.kA, 8 * N },       { (void**)&b.kB, 8 * N },       { (void**)&b.kx, 8 * N },
@@ -1184,42 +1464,87 @@ Status Engine::ensure(size_t n)
         { (void**)&b.partials, 4 * (size_t)(nb_scan + 1 + 256 * nb_sort) },
         { (void**)&b.hist, 4 * 256 * (size_t)nb_sort },
         { (void**)&b.ghist, 4 * 8 * 256 },
+        { (void**)&b.vrank, 4 * N },    { (void**)&b.rin, 4 * N },
         { (void**)&b.P0, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.NX, 4 * 4 * R },
-        { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.off, 4 * offs },   { (void**)&b.res, 16 * nodes },
-        { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
-        { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },    { (void**)&b.splitv, 4 * nodes },
+        { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
+        { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },    { (void**)&b.claim, 4 * N },
+        { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },
+    };
+    const Status st = carve(items, ws_, ws_bytes_);
+    if (st != Status::ok) return st;
+    b.qcap = (uint32_t)qcap;
+    b.rec_cap = R;
+    cap_n_ = N;
+    return Status::ok;
+}
+
+Status Engine::ensure_nodes(const Plan& P)
+{
+    Bufs& b = *b_;
+    if (ws_nodes_ && P.nodes <= b.nodes_cap && P.offs <= b.offs_cap && P.seg_nodes <= b.seg_nodes &&
+        P.anchor_cap <= b.anchor_cap)
+        return Status::ok;
+    if (ws_nodes_) (void)hipFree(ws_nodes_);
+    ws_nodes_ = nullptr;
+    ws_nodes_bytes_ = 0;
+    // grow with headroom so that similar batches reuse the allocation
+    const size_t nodes = std::max(P.nodes, b.nodes_cap) + 64;
+    const size_t offs = std::max(P.offs, b.offs_cap) + 64;
+    const size_t seg_nodes = std::max(P.seg_nodes, b.seg_nodes);
+    const size_t anchor_cap = std::max(P.anchor_cap, b.anchor_cap);
+    const Item items[] = {
+        { (void**)&b.off, 4 * (kLvlHdr + offs) }, { (void**)&b.res, 16 * nodes }, { (void**)&b.splitv, 4 * nodes },
         { (void**)&b.qv, 4 * (size_t)kSegMaxK * seg_nodes },
         { (void**)&b.segstart, 4 * (size_t)kSegSlots * seg_nodes },
         { (void**)&b.nseg, 4 * seg_nodes }, { (void**)&b.nstate, 16 * seg_nodes },
         { (void**)&b.segpool, 8 * (size_t)kSegSlots * seg_nodes },
+        { (void**)&b.segresume, 4 * (size_t)kSegSlots * seg_nodes },
         { (void**)&b.arec, 4 * (size_t)kRec * anchor_cap }, { (void**)&b.acoord, 32 * anchor_cap },
         { (void**)&b.akey, 8 * anchor_cap },
         { (void**)&b.astate, 4 * anchor_cap }, { (void**)&b.aedge, 4 * anchor_cap },
         { (void**)&b.apool, 8 * anchor_cap }, { (void**)&b.selk, 4 * (size_t)kSegSlots * seg_nodes },
         { (void**)&b.nsel, 4 * seg_nodes },
         { (void**)&b.alist, 4 * (2 + 4 * (size_t)kCross) * anchor_cap }, { (void**)&b.alist_n, 64 },
-        { (void**)&b.segresume, 4 * (size_t)kSegSlots * seg_nodes }, { (void**)&b.claim, 4 * N },
-        { (void**)&b.spare, 64 },
     };
-    size_t total = 0;
-    for (const Item& it : items) total += (it.bytes + 255) & ~(size_t)255;
-    void* ws = nullptr;
-    const hipError_t e = hipMalloc(&ws, total);
-    if (e != hipSuccess) return e == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal;
-    ws_ = (char*)ws;
-    ws_bytes_ = total;
-    size_t at = 0;
-    for (const Item& it : items)
-    {
-        *it.p = ws_ + at;
-        at += (it.bytes + 255) & ~(size_t)255;
-    }
-    b.qcap = (uint32_t)qcap;
-    b.dcap = D;
-    b.rec_cap = R;
+    const Status st = carve(items, ws_nodes_, ws_nodes_bytes_);
+    if (st != Status::ok) return st;
+    b.nodes_cap = nodes;
+    b.offs_cap = offs;
     b.seg_nodes = seg_nodes;
     b.anchor_cap = anchor_cap;
-    cap_n_ = N;
+    return Status::ok;
+}
+
+Status Engine::ensure_sets(size_t k, size_t n)
+{
+    Bufs& b = *b_;
+    const size_t tris = 2 * std::max<size_t>(n, 64);
+    if (!b.host_sets || b.host_sets_cap < k + 1)
+    {
+        if (b.host_sets) (void)hipHostFree(b.host_sets);
+        b.host_sets = nullptr;
+        void* h = nullptr;
+        // staging for up to kHostSetArrays arrays of k + 1 entries
+        QDT_HIP(hipHostMalloc(&h, kHostSetArrays * (k + 1) * sizeof(uint32_t), hipHostMallocDefault));
+        b.host_sets = (uint32_t*)h;
+        b.host_sets_cap = k + 1;
+    }
+    if (ws_sets_ && k <= b.sets_cap && tris <= b.tris_cap) return Status::ok;
+    if (ws_sets_) (void)hipFree(ws_sets_);
+    ws_sets_ = nullptr;
+    ws_sets_bytes_ = 0;
+    const size_t K = std::max(k, b.sets_cap) + 1;
+    const size_t T = std::max(tris, b.tris_cap);
+    const Item items[] = {
+        { (void**)&b.soff, 4 * K },   { (void**)&b.srank, 4 * K },  { (void**)&b.rpos, 4 * K },
+        { (void**)&b.vstart, 4 * K }, { (void**)&b.setbad, 4 * K }, { (void**)&b.in_off, 4 * K },
+        { (void**)&b.mstart, 4 * K }, { (void**)&b.sstart, 4 * K }, { (void**)&b.toff, 4 * K },
+        { (void**)&b.trank, 4 * T },  { (void**)&b.btris, 12 * T }, { (void**)&b.tmain, 64 },
+    };
+    const Status st = carve(items, ws_sets_, ws_sets_bytes_);
+    if (st != Status::ok) return st;
+    b.sets_cap = K - 1;
+    b.tris_cap = T;
     return Status::ok;
 }
 
@@ -1252,18 +1577,19 @@ static Status sort_pairs(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*&
 Status Engine::run(const double* d_pts, size_t n)
 {
     res_ = DeviceResult();
+    batch_ = BatchResult();
     st_ = RunStats();
     if (n == 0) return Status::ok;
     Status s = ensure(n);
     if (s != Status::ok) return s;
     bool redo = false;
-    s = run_impl(d_pts, n, seams_, redo);
+    s = run_impl(d_pts, n, nullptr, 1, seams_, redo);
     if (s == Status::ok && redo)
     {
-        // two seam pieces touched the same vertex: repeat with unsplit merges
+        // a seam piece failed (budget or record exhaustion): repeat unsplit
         const RunStats first = st_;
         st_ = RunStats();
-        s = run_impl(d_pts, n, false, redo);
+        s = run_impl(d_pts, n, nullptr, 1, false, redo);
         st_.seam_retries = 1;
         st_.device_ms += first.device_ms;
         for (int i = 0; i < 8; ++i) st_.seam[i] = first.seam[i];
@@ -1273,19 +1599,53 @@ Status Engine::run(const double* d_pts, size_t n)
     return s;
 }
 
-Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& redo)
+Status Engine::run_batch(const double* d_pts, const size_t* set_off, int k)
+{
+    res_ = DeviceResult();
+    batch_ = BatchResult();
+    st_ = RunStats();
+    if (k <= 0) return Status::ok;
+    const size_t n = set_off[k];
+    batch_.k = k;
+    batch_.status.assign(k, 0);
+    batch_.unique.assign(k, 0);
+    batch_.tri_start.assign(k, 0);
+    batch_.tri_count.assign(k, 0);
+    if (n == 0) return Status::ok;
+    Status s = ensure(n);
+    if (s != Status::ok) return s;
+    s = ensure_sets((size_t)k, n);
+    if (s != Status::ok) return s;
+    bool redo = false;
+    s = run_impl(d_pts, n, set_off, k, seams_, redo);
+    if (s == Status::ok && redo)
+    {
+        const RunStats first = st_;
+        st_ = RunStats();
+        s = run_impl(d_pts, n, set_off, k, false, redo);
+        st_.seam_retries = 1;
+        st_.device_ms += first.device_ms;
+        for (int i = 0; i < 8; ++i) st_.seam[i] = first.seam[i];
+        st_.err_bits = first.err_bits;
+    }
+    if (s != Status::ok) batch_ = BatchResult();
+    return s;
+}
+
+Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_off, int k, bool seams,
+                        bool& redo)
 {
     redo = false;
     Bufs& b = *b_;
     hipStream_t s = stream_;
     const uint32_t n = (uint32_t)n_sz;
     const double2* pts = (const double2*)d_pts_raw;
+    const bool multi = set_off != nullptr;
     uint32_t* hs = b.host_scalars;
-    uint32_t* hghist = nullptr;
     std::vector<uint32_t> ghist_host(8 * 256);
-    hghist = ghist_host.data();
+    uint32_t* hghist = ghist_host.data();
 
-    // profiling: event 0..8 = stage boundaries, 9.. = merge launches (start, end)
+    // profiling: event 0..8 = stage boundaries, 9.. = merge levels (start, end)
     const bool prof = profiling_;
     auto mark = [&](size_t i) {
         if (prof)
@@ -1299,6 +1659,82 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     QDT_HIP(hipMemsetAsync(b.counters, 0, 8 * 16, s));
     QDT_HIP(hipMemsetAsync(b.err, 0, 64, s));
 
+    // 0. several sets: reject the sets holding a non-finite point, then give
+    // every set its rank in the kd forest (root level ascending; roots only
+    // on even levels so that every level has one cut frame)
+    std::vector<int> set_rank, rank_set, roots;
+    std::vector<uint32_t> rank_n;
+    int kg = 1; // sets in the forest
+    if (multi)
+    {
+        uint32_t* H = b.host_sets;
+        const size_t K1 = (size_t)k + 1;
+        for (int i = 0; i <= k; ++i) H[i] = (uint32_t)set_off[i];
+        QDT_HIP(hipMemcpyAsync(b.soff, H, 4 * K1, hipMemcpyHostToDevice, s));
+        QDT_HIP(hipMemsetAsync(b.setbad, 0, 4 * K1, s));
+        k_check_sets<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, n, b.soff, k, b.setbad);
+        QDT_HIP(hipMemcpyAsync(H + K1, b.setbad, 4 * (size_t)k, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipStreamSynchronize(s));
+        std::vector<int> depth(k, 0);
+        int D0 = 0;
+        for (int i = 0; i < k; ++i)
+        {
+            const uint32_t ni = (uint32_t)(set_off[i + 1] - set_off[i]);
+            batch_.status[i] = H[K1 + i] ? (int)Status::not_finite : 0;
+            if (ni && !H[K1 + i])
+            {
+                depth[i] = kd_depth(ni);
+                D0 = std::max(D0, depth[i]);
+            }
+        }
+        std::vector<std::pair<int, int>> order; // (root, set)
+        for (int i = 0; i < k; ++i)
+        {
+            const uint32_t ni = (uint32_t)(set_off[i + 1] - set_off[i]);
+            if (!ni || H[K1 + i]) continue;
+            int r = D0 - depth[i];
+            if (r & 1) --r;
+            order.push_back({ r, i });
+        }
+        std::sort(order.begin(), order.end());
+        kg = (int)order.size();
+        set_rank.assign(k, 0);
+        for (int r = 0; r < kg; ++r)
+        {
+            set_rank[order[r].second] = r;
+            rank_set.push_back(order[r].second);
+            roots.push_back(order[r].first);
+        }
+        for (int i = 0, r = kg; i < k; ++i) // rejected and empty sets last, never rooted
+        {
+            const uint32_t ni = (uint32_t)(set_off[i + 1] - set_off[i]);
+            if (!ni || H[K1 + i])
+            {
+                set_rank[i] = r++;
+                rank_set.push_back(i);
+            }
+        }
+        if (kg == 0) return Status::ok;
+        uint32_t* Hr = H + K1;       // srank
+        uint32_t* Hp = H + 2 * K1;   // rpos
+        uint32_t* Hi = H + 3 * K1;   // in_off
+        uint32_t pos = 0;
+        for (int r = 0; r < k; ++r)
+        {
+            const int i = rank_set[r];
+            Hr[i] = (uint32_t)r;
+            Hp[r] = pos;
+            Hi[r] = (uint32_t)set_off[i];
+            pos += (uint32_t)(set_off[i + 1] - set_off[i]);
+        }
+        Hp[k] = pos;
+        Hi[k] = n;
+        QDT_HIP(hipMemcpyAsync(b.srank, Hr, 4 * K1, hipMemcpyHostToDevice, s));
+        QDT_HIP(hipMemcpyAsync(b.rpos, Hp, 4 * K1, hipMemcpyHostToDevice, s));
+        QDT_HIP(hipMemcpyAsync(b.in_off, Hi, 4 * K1, hipMemcpyHostToDevice, s));
+        k_input_rank<<<cdiv(n, kBlock), kBlock, 0, s>>>(n, b.soff, k, b.srank, b.rin);
+    }
+
     // 1. keys; the first sort orders by y and carries the input index
     k_keys<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, n, b.kx, b.kA, b.vA, b.err);
     uint64_t *kA = b.kA, *kB = b.kB;
@@ -1307,31 +1743,49 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     QDT_HIP(hipMemcpyAsync(b.ky, b.kA, 8 * (size_t)n, hipMemcpyDeviceToDevice, s));
     Status st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
     if (st != Status::ok) return st;
-    // non-finite input is reported before any further work
-    QDT_HIP(hipMemcpyAsync(hs, b.err, 4, hipMemcpyDeviceToHost, s));
-    QDT_HIP(hipStreamSynchronize(s));
-    if (hs[0] & 1) return Status::not_finite;
+    if (!multi)
+    {
+        // non-finite input is reported before any further work
+        QDT_HIP(hipMemcpyAsync(hs, b.err, 4, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipStreamSynchronize(s));
+        if (hs[0] & 1) return Status::not_finite;
+    }
+    QDT_HIP(hipMemsetAsync(b.err, 0, 64, s));
     mark(1);
-    // 2. stable sort by x -> (x, y, index)
+    // 2. stable sort by x -> (x, y, index); several sets: then by rank
     k_gather_u64<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.kx, n, kA);
     st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
     if (st != Status::ok) return st;
+    if (multi)
+    {
+        k_gather_rank<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.rin, n, kA);
+        st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
+        if (st != Status::ok) return st;
+    }
 
     mark(2);
     // 3. dedupe
-    DedupeFlag df{ kA, vA, b.ky };
+    DedupeFlag df{ kA, vA, b.ky, multi ? b.kx : nullptr };
     scan_u32(df, n, StoreU32{ b.dist }, b.partials, s);
     const uint32_t nbs = cdiv(n, kScanTile);
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbs, 4, hipMemcpyDeviceToHost, s));
-    k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, vA, df, b.dist, n, b.P0, b.ID0);
+    k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, vA, df, b.dist, n, b.P0, b.ID0, b.vrank);
     k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, df, b.dist, n, b.ID0, b.rep);
+    std::vector<uint32_t> vstart_h;
+    if (multi)
+    {
+        k_vstart<<<cdiv((size_t)k + 1, kBlock), kBlock, 0, s>>>(b.dist, b.rpos, k, n, b.partials + nbs, b.vstart);
+        QDT_HIP(hipMemcpyAsync(b.host_sets, b.vstart, 4 * ((size_t)k + 1), hipMemcpyDeviceToHost, s));
+    }
     QDT_HIP(hipStreamSynchronize(s));
     const uint32_t m = hs[0];
+    if (multi) vstart_h.assign(b.host_sets, b.host_sets + k + 1);
     res_.n = n;
     res_.rep = b.rep;
     res_.unique = m;
     st_.unique = m;
-    if (m < 3)
+    const uint32_t m_forest = multi ? vstart_h[kg] : m; // distinct points in rooted sets
+    if (m_forest < 3 && !multi)
     {
         QDT_HIP(hipEventRecord(ev1_, s));
         QDT_HIP(hipEventSynchronize(ev1_));
@@ -1339,6 +1793,41 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
         (void)hipEventElapsedTime(&ms, ev0_, ev1_);
         st_.device_ms = ms;
         return Status::ok;
+    }
+
+    // the kd forest
+    Plan P;
+    {
+        std::vector<uint32_t> sizes;
+        int D;
+        if (multi)
+        {
+            for (int r = 0; r < kg; ++r) sizes.push_back(vstart_h[r + 1] - vstart_h[r]);
+            D = 0;
+            for (int r = 0; r < kg; ++r) D = std::max(D, roots[r] + kd_depth(std::max<uint32_t>(sizes[r], 1)));
+        }
+        else
+        {
+            sizes.push_back(m);
+            roots.assign(1, 0);
+            D = kd_depth(m);
+        }
+        if (D + 2 > kMaxLevels) return Status::internal;
+        make_plan(P, sizes, roots, D, seams);
+    }
+    st = ensure_nodes(P);
+    if (st != Status::ok) return st;
+    {
+        int* T = b.host_table;
+        for (int d = 0; d < kMaxLevels; ++d)
+        {
+            const bool in = d <= P.D;
+            T[4 * d] = in ? P.count[d] : 0;
+            T[4 * d + 1] = in ? P.offb[d] : 0;
+            T[4 * d + 2] = in ? P.nodeb[d] : 0;
+            T[4 * d + 3] = 0;
+        }
+        QDT_HIP(hipMemcpyAsync(b.off, T, 4 * kLvlHdr, hipMemcpyHostToDevice, s));
     }
 
     mark(3);
@@ -1349,6 +1838,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     // seam anchors (-1 = kept) until the merges end
     QDT_HIP(hipMemsetAsync(b.face, 0xff, 4 * b.rec_cap, s));
     QDT_HIP(hipMemsetAsync(b.spare, 0, 64, s));
+    if (m_forest < m) // records of rejected sets' points stay free
+        QDT_HIP(hipMemsetAsync(b.OR + 6 * (size_t)m_forest, 0xff, 4 * 6 * (size_t)(m - m_forest), s));
     const int lim_e = (int)(4 * b.rec_cap);
     const int spare_base = (int)(3 * (size_t)m);
     const int spare_cap = (int)(b.rec_cap - 3 * (size_t)m);
@@ -1360,41 +1851,62 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     vB = b.vB;
     st = sort_pairs(kA, vA, kB, vB, m, b.hist, b.partials, b.ghist, hghist, s);
     if (st != Status::ok) return st;
+    if (multi)
+    {
+        k_gather_rank<<<cdiv(m, kBlock), kBlock, 0, s>>>(vA, b.vrank, m, kA);
+        st = sort_pairs(kA, vA, kB, vB, m, b.hist, b.partials, b.ghist, hghist, s);
+        if (st != Status::ok) return st;
+    }
     uint32_t* B1 = vA;
-    uint32_t* B1spare = vB;
     uint32_t* B0 = b.B0;
     k_iota<<<cdiv(m, kBlock), kBlock, 0, s>>>(B0, m);
 
     mark(4);
     // 5. kd levels
-    const int D = kd_depth(m);
-    if (D > b.dcap) return Status::internal;
+    const int D = P.D;
+    const uint32_t* vstart = multi ? b.vstart : nullptr;
+    if (!multi)
+    {
+        hs[2] = 0;
+        hs[3] = m;
+        QDT_HIP(hipMemcpyAsync(b.one_set, hs + 2, 8, hipMemcpyHostToDevice, s));
+        vstart = b.one_set; // {0, m}: the one set's distinct points
+    }
     st_.levels = (uint32_t)D;
-    k_root<<<1, 1, 0, s>>>(b.off, m);
-    QDT_HIP(hipMemsetAsync(b.res, 0xff, 16 * (((size_t)2 << D) - 1), s));
-    QDT_HIP(hipMemsetAsync(b.nodeA, 0, 4 * (size_t)m, s));
+    QDT_HIP(hipMemsetAsync(b.res, 0xff, 16 * P.nodes, s));
     uint32_t *nodeA = b.nodeA, *nodeB = b.nodeB;
     uint32_t* Bt = b.Bt;
+    auto roots_at = [&](int d, uint32_t* node) {
+        const int ra = P.ra[d], rb = P.ra[d + 1];
+        if (ra == rb) return;
+        k_roots<<<cdiv((size_t)(rb - ra) + 1, kBlock), kBlock, 0, s>>>(b.off, d, ra, rb, vstart);
+        const uint32_t span = multi ? vstart_h[rb] - vstart_h[ra] : m;
+        k_root_nodes<<<cdiv(std::max<uint32_t>(span, 1), kBlock), kBlock, 0, s>>>(
+            b.off, d, ra, rb, vstart, d ? 2 * (uint32_t)P.count[d - 1] : 0u, node);
+    };
+    roots_at(0, nodeA);
     for (int d = 0; d < D; ++d)
     {
-        const uint32_t cnt = 1u << d;
-        k_children<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(b.off, d);
+        const uint32_t cnt = (uint32_t)P.count[d];
         uint32_t*& Bf = (d & 1) ? B1 : B0;
         uint32_t*& Bo = (d & 1) ? B0 : B1;
-        const int Kd = seams ? seam_anchors((double)(m >> d)) : 0;
-        k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, b.off, d, Kd, b.splitv, b.qv);
-        k_split<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bf, nodeA, b.off, d, m, nodeB, b.bits);
-        scan_u32(ZeroFlag{ Bo, b.bits }, m, StoreU32{ b.Z }, b.partials, s);
-        k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, nodeA, b.off, d, b.bits, b.Z, m, Bt);
-        std::swap(Bo, Bt);
+        if (cnt)
+        {
+            k_children<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(b.off, d);
+            k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, b.off, d, P.K[d], b.splitv, b.qv);
+            k_split<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bf, nodeA, b.off, d, m, nodeB, b.bits);
+            scan_u32(ZeroFlag{ Bo, b.bits }, m, StoreU32{ b.Z }, b.partials, s);
+            k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, nodeA, b.off, d, b.bits, b.Z, m, Bt);
+            std::swap(Bo, Bt);
+        }
+        roots_at(d + 1, nodeB);
         std::swap(nodeA, nodeB);
     }
-    (void)B1spare;
 
     mark(5);
     // 6. leaves, 7. merges
     {
-        const uint32_t cnt = 1u << D;
+        const uint32_t cnt = (uint32_t)P.count[D];
         k_leaves<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, D, m,
                                                                  lim_e, b.res, b.err, b.counters);
     }
@@ -1403,14 +1915,13 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     uint64_t kmerge_levels = 0;
     for (int d = D - 1; d >= 0; --d)
     {
-        const uint32_t cnt = 1u << d;
+        const uint32_t cnt = (uint32_t)P.count[d];
         mark(9 + 2 * (size_t)d);
-        // piece ids of a level must fit the claim id bits
-        const bool fits = (size_t)cnt * kSegSlots < ((size_t)1 << kClaimBits);
-        const int K = seams && fits ? seam_anchors((double)(m >> d)) : 0;
-        if (!K)
-            kmerge_levels |= 1ull << d;
-        if (!K)
+        const int K = P.K[d];
+        if (!K) kmerge_levels |= 1ull << d;
+        if (!cnt)
+            ;
+        else if (!K)
             k_merge<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res,
                                                                     b.err, b.counters);
         else
@@ -1460,7 +1971,10 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
                                                           (uint32_t)spare_cap, b.face, b.queue, b.qcap, b.counters,
                                                           b.err);
     k_extract_exact<<<256, kBlock, 0, s>>>(b.NX, b.OR, b.P0, m, lim_e, b.queue, b.counters, b.qcap, b.face);
-    scan_u32(FaceCount{ b.face }, nrec, FaceWrite{ b.face, b.NX, b.OR, b.ID0, b.tris, m, lim_e }, b.partials, s);
+    scan_u32(FaceCount{ b.face }, nrec,
+             FaceWrite{ b.face, b.NX, b.OR, b.ID0, b.tris, m, lim_e, multi ? 3 * m : 0xffffffffu,
+                        multi ? b.tmain : b.partials },
+             b.partials, s);
     mark(8);
     QDT_HIP(hipEventRecord(ev1_, s));
     const uint32_t nbx = cdiv(nrec, kScanTile);
@@ -1517,6 +2031,80 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, bool seams, bool& 
     }
     res_.tris = b.tris;
     res_.triangles = hs[0];
+    if (multi) return batch_output(hs[0], k, kg, m);
+    return Status::ok;
+}
+
+// Groups the triangles by set with set-local indices (batch_.tris) and makes
+// rep set-local.
+Status Engine::batch_output(uint32_t T, int k, int kg, uint32_t m)
+{
+    Bufs& b = *b_;
+    hipStream_t s = stream_;
+    uint32_t* hs = b.host_scalars;
+    uint32_t* H = b.host_sets;
+    const size_t K1 = (size_t)k + 1;
+    std::vector<uint32_t> ghist_host(8 * 256);
+    QDT_HIP(hipMemcpyAsync(hs, b.tmain, 4, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    const uint32_t tmain = std::min(hs[0], T);
+    const uint32_t c = T - tmain; // triangles of spare records
+    if (T > b.tris_cap || c > cap_n_) return Status::internal;
+    k_tri_rank<<<cdiv(std::max<uint32_t>(T, 1), kBlock), kBlock, 0, s>>>(b.tris, T, b.rin, b.trank);
+    k_rank_bounds<uint32_t><<<cdiv(K1, kBlock), kBlock, 0, s>>>(b.trank, tmain, kg, b.mstart);
+    uint64_t *kA = b.kA, *kB = b.kB;
+    uint32_t *vA = b.vA, *vB = b.vB;
+    if (c)
+    {
+        k_spare_keys<<<cdiv(c, kBlock), kBlock, 0, s>>>(b.trank + tmain, c, kA, vA);
+        const Status st = sort_pairs(kA, vA, kB, vB, c, b.hist, b.partials, b.ghist, ghist_host.data(), s);
+        if (st != Status::ok) return st;
+    }
+    k_rank_bounds<uint64_t><<<cdiv(K1, kBlock), kBlock, 0, s>>>(kA, c, kg, b.sstart);
+    QDT_HIP(hipMemcpyAsync(H, b.mstart, 4 * ((size_t)kg + 1), hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipMemcpyAsync(H + K1, b.sstart, 4 * ((size_t)kg + 1), hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipMemcpyAsync(H + 2 * K1, b.vstart, 4 * K1, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    uint32_t* Ht = H + 3 * K1;
+    uint32_t at = 0;
+    std::vector<uint32_t> cnt(kg);
+    for (int r = 0; r < kg; ++r)
+    {
+        Ht[r] = at;
+        cnt[r] = (H[r + 1] - H[r]) + (H[K1 + r + 1] - H[K1 + r]);
+        at += cnt[r];
+    }
+    Ht[kg] = at;
+    if (at != T) return Status::internal;
+    QDT_HIP(hipMemcpyAsync(b.toff, Ht, 4 * ((size_t)kg + 1), hipMemcpyHostToDevice, s));
+    if (T)
+        k_batch_tris<<<cdiv(T, kBlock), kBlock, 0, s>>>(b.tris, T, tmain, b.trank, kA, vA, b.mstart, b.sstart,
+                                                        b.toff, b.in_off, b.btris);
+    k_rep_local<<<cdiv(res_.n, kBlock), kBlock, 0, s>>>(b.rep, (uint32_t)res_.n, b.soff, k);
+    QDT_HIP(hipStreamSynchronize(s));
+    // per set, in the caller's order
+    const uint32_t* vs = H + 2 * K1;
+    std::vector<int> rank_of(k, -1);
+    {
+        std::vector<uint32_t> srank(k);
+        QDT_HIP(hipMemcpy(srank.data(), b.srank, 4 * (size_t)k, hipMemcpyDeviceToHost));
+        for (int i = 0; i < k; ++i) rank_of[i] = (int)srank[i];
+    }
+    for (int i = 0; i < k; ++i)
+    {
+        const int r = rank_of[i];
+        if (r < kg)
+        {
+            batch_.unique[i] = vs[r + 1] - vs[r];
+            batch_.tri_start[i] = Ht[r];
+            batch_.tri_count[i] = cnt[r];
+        }
+    }
+    (void)m;
+    batch_.tris = b.btris;
+    batch_.rep = b.rep;
+    batch_.n = res_.n;
+    batch_.triangles = T;
     return Status::ok;
 }
 

@@ -48,6 +48,20 @@ enum class Status : int
     internal = -11,
 };
 
+// Results of a run over several point sets (Engine::run_batch).
+struct BatchResult
+{
+    int k = 0;
+    std::vector<int> status;          // per set: 0, or -4 for a set with a non-finite point
+    std::vector<uint32_t> unique;     // per set: distinct points
+    std::vector<uint64_t> tri_start;  // per set: first triangle in `tris`
+    std::vector<uint64_t> tri_count;  // per set: triangles
+    const uint32_t* tris = nullptr;   // device: 3 * triangles set-local input indices, grouped by set
+    const uint32_t* rep = nullptr;    // device: n entries, set-local representative of every input point
+    size_t n = 0;
+    uint64_t triangles = 0;
+};
+
 struct DeviceResult
 {
     const uint32_t* tris = nullptr; // 3 * triangles original indices
@@ -68,6 +82,10 @@ public:
     // Triangulates n points at d_pts (device memory, (x, y) doubles).
     Status run(const double* d_pts, size_t n);
     const DeviceResult& result() const { return res_; }
+    // Triangulates k independent point sets in one pass: set i is points
+    // [set_off[i], set_off[i+1]) of d_pts (set_off on the host).
+    Status run_batch(const double* d_pts, const size_t* set_off, int k);
+    const BatchResult& batch() const { return batch_; }
     const RunStats& stats() const { return st_; }
     hipStream_t stream() const { return stream_; }
     // Frees the workspace (the next run allocates again).
@@ -79,16 +97,23 @@ public:
     // Cuts large merges into seam pieces (default on).
     void set_seams(bool on) { seams_ = on; }
     // Device bytes currently held.
-    size_t workspace_bytes() const { return ws_bytes_; }
+    size_t workspace_bytes() const;
+
+    struct Plan;
 
 private:
     Status ensure(size_t n);
-    Status run_impl(const double* d_pts, size_t n, bool seams, bool& redo);
+    Status ensure_nodes(const Plan& P);
+    Status ensure_sets(size_t k, size_t n);
+    Status run_impl(const double* d_pts, size_t n, const size_t* set_off, int k, bool seams, bool& redo);
+    Status batch_output(uint32_t T, int k, int kg, uint32_t m);
 
     hipStream_t stream_;
     bool own_stream_;
-    char* ws_ = nullptr;
-    size_t ws_bytes_ = 0;
+    char* ws_ = nullptr;       // point-proportional buffers
+    char* ws_nodes_ = nullptr; // kd forest nodes
+    char* ws_sets_ = nullptr;  // per-set buffers of batches
+    size_t ws_bytes_ = 0, ws_nodes_bytes_ = 0, ws_sets_bytes_ = 0;
     size_t cap_n_ = 0;
     int block_points_ = 0;
     bool profiling_ = false;
@@ -96,6 +121,7 @@ private:
     std::vector<hipEvent_t> evs_; // profiling events, created on demand
     hipEvent_t prof_event(size_t i);
     DeviceResult res_;
+    BatchResult batch_;
     RunStats st_{};
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     // carved buffers (see ensure())

@@ -208,25 +208,29 @@ qdt_status ensure_device_input(qdt_t* Q, size_t n)
     return QDT_OK;
 }
 
+void copy_stats(const qdt::RunStats& s, qdt_gpu_stats& o)
+{
+    o.orient_calls = s.orient;
+    o.orient_exact = s.orient_exact;
+    o.incircle_calls = s.incircle;
+    o.incircle_exact = s.incircle_exact;
+    o.extract_exact = s.extract_queue;
+    o.unique_points = s.unique;
+    o.triangles = s.triangles;
+    o.kd_levels = s.levels;
+    o.device_ms = s.device_ms;
+    for (int i = 0; i < 8; ++i) o.stage_ms[i] = s.stage_ms[i];
+    o.merge_launch_ms_sum = s.merge_launch_ms_sum;
+    o.merge_launches = s.merge_launches;
+    o.merge_edges = s.merge_edges;
+    o.seam_level_ms_sum = s.seam_level_ms_sum;
+    o.seam_retries = s.seam_retries;
+    for (int i = 0; i < 8; ++i) o.seam_counters[i] = s.seam[i];
+}
+
 void record_stats(qdt_t* Q)
 {
-    const qdt::RunStats& s = Q->eng->stats();
-    Q->stats.orient_calls = s.orient;
-    Q->stats.orient_exact = s.orient_exact;
-    Q->stats.incircle_calls = s.incircle;
-    Q->stats.incircle_exact = s.incircle_exact;
-    Q->stats.extract_exact = s.extract_queue;
-    Q->stats.unique_points = s.unique;
-    Q->stats.triangles = s.triangles;
-    Q->stats.kd_levels = s.levels;
-    Q->stats.device_ms = s.device_ms;
-    for (int i = 0; i < 8; ++i) Q->stats.stage_ms[i] = s.stage_ms[i];
-    Q->stats.merge_launch_ms_sum = s.merge_launch_ms_sum;
-    Q->stats.merge_launches = s.merge_launches;
-    Q->stats.merge_edges = s.merge_edges;
-    Q->stats.seam_level_ms_sum = s.seam_level_ms_sum;
-    Q->stats.seam_retries = s.seam_retries;
-    for (int i = 0; i < 8; ++i) Q->stats.seam_counters[i] = s.seam[i];
+    copy_stats(Q->eng->stats(), Q->stats);
 }
 
 // Brings triangles and rep of the last device run into host memory.
@@ -668,6 +672,253 @@ void qdt_gpu_release_workspace(qdt_t* ptr)
     if (materialize(Q) != QDT_OK) reset_context(Q);
     if (materialize_points(Q) != QDT_OK) reset_context(Q);
     Q->eng->release();
+}
+
+// ---- batched form ----------------------------------------------------------
+
+} // extern "C"
+
+struct qdt_gpu_batch
+{
+    qdt::Engine* eng = nullptr;
+    double* d_in = nullptr;
+    size_t d_in_cap = 0;
+    size_t k = 0, n = 0;
+    Pinned<uint32_t> tris; // host copy of the grouped triangles (lazy)
+    bool tris_host = false;
+    qdt_gpu_stats stats{};
+    bool profiling = false;
+    bool seams = true;
+    ~qdt_gpu_batch()
+    {
+        delete eng;
+        if (d_in) (void)hipFree(d_in);
+    }
+};
+
+namespace
+{
+
+void reset_batch(qdt_gpu_batch* B)
+{
+    B->k = B->n = 0;
+    B->tris_host = false;
+    B->stats = qdt_gpu_stats{};
+}
+
+qdt_status batch_engine(qdt_gpu_batch* B)
+{
+    if (B->eng) return QDT_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return QDT_ERR_INTERNAL;
+    B->eng = new (std::nothrow) qdt::Engine();
+    if (!B->eng) return QDT_ERR_OUT_OF_MEMORY;
+    B->eng->set_profiling(B->profiling);
+    B->eng->set_seams(B->seams);
+    if (!B->eng->stream())
+    {
+        delete B->eng;
+        B->eng = nullptr;
+        return QDT_ERR_INTERNAL;
+    }
+    return QDT_OK;
+}
+
+qdt_status batch_offsets(const size_t* offsets, size_t k)
+{
+    if (k == 0) return QDT_OK;
+    if (!offsets) return QDT_ERR_NULL_ARGUMENT;
+    if (offsets[0] != 0) return QDT_ERR_INVALID_INDEX;
+    for (size_t i = 0; i < k; ++i)
+        if (offsets[i + 1] < offsets[i]) return QDT_ERR_INVALID_INDEX;
+    if (offsets[k] > kMaxPoints) return QDT_ERR_TOO_MANY_POINTS;
+    if (k > (size_t)INT32_MAX) return QDT_ERR_TOO_MANY_POINTS;
+    return QDT_OK;
+}
+
+// pts: host (device == false) or device memory
+qdt_status batch_run(qdt_gpu_batch* B, const qdt_vec2d* pts, const size_t* offsets, size_t k, bool device)
+{
+    if (!B) return QDT_ERR_NULL_CONTEXT;
+    try
+    {
+        reset_batch(B);
+        qdt_status st = batch_offsets(offsets, k);
+        if (st != QDT_OK || k == 0) return st;
+        const size_t n = offsets[k];
+        if (n && !pts) return QDT_ERR_NULL_ARGUMENT;
+        st = batch_engine(B);
+        if (st != QDT_OK) return st;
+        const double* src = (const double*)pts;
+        if (!device && n)
+        {
+            const size_t bytes = n * sizeof(qdt_vec2d);
+            if (bytes > B->d_in_cap)
+            {
+                if (B->d_in) (void)hipFree(B->d_in);
+                B->d_in = nullptr;
+                B->d_in_cap = 0;
+                void* p = nullptr;
+                const hipError_t e = hipMalloc(&p, bytes);
+                if (e != hipSuccess) return e == hipErrorOutOfMemory ? QDT_ERR_OUT_OF_MEMORY : QDT_ERR_INTERNAL;
+                B->d_in = (double*)p;
+                B->d_in_cap = bytes;
+            }
+            if (hipMemcpy(B->d_in, pts, bytes, hipMemcpyHostToDevice) != hipSuccess) return QDT_ERR_INTERNAL;
+            src = B->d_in;
+        }
+        const qdt::Status s = B->eng->run_batch(src, offsets, (int)k);
+        if (s != qdt::Status::ok)
+        {
+            reset_batch(B);
+            return map_status(s);
+        }
+        copy_stats(B->eng->stats(), B->stats);
+        B->k = k;
+        B->n = n;
+        return QDT_OK;
+    }
+    catch (...)
+    {
+        reset_batch(B);
+        return QDT_ERR_OUT_OF_MEMORY;
+    }
+}
+
+bool batch_materialize(const qdt_gpu_batch* Bc)
+{
+    qdt_gpu_batch* B = const_cast<qdt_gpu_batch*>(Bc);
+    if (B->tris_host) return true;
+    const qdt::BatchResult& r = B->eng->batch();
+    if (!B->tris.reserve(3 * (size_t)r.triangles + 3)) return false;
+    if (r.triangles && hipMemcpy(B->tris.p, r.tris, 12 * (size_t)r.triangles, hipMemcpyDeviceToHost) != hipSuccess)
+        return false;
+    B->tris_host = true;
+    return true;
+}
+
+const qdt::BatchResult* batch_result(const qdt_gpu_batch* B, size_t set)
+{
+    if (!B || !B->eng || set >= B->k) return nullptr;
+    const qdt::BatchResult& r = B->eng->batch();
+    if ((size_t)r.k != B->k) return nullptr;
+    return &r;
+}
+
+} // namespace
+
+extern "C" {
+
+qdt_gpu_batch* qdt_gpu_batch_create(void) { return new (std::nothrow) qdt_gpu_batch(); }
+
+void qdt_gpu_batch_destroy(qdt_gpu_batch* b) { delete b; }
+
+qdt_status qdt_gpu_batch_execute(qdt_gpu_batch* b, const qdt_vec2d* pts, const size_t* offsets, size_t k)
+{
+    return batch_run(b, pts, offsets, k, false);
+}
+
+qdt_status qdt_gpu_batch_execute_device(qdt_gpu_batch* b, const qdt_vec2d* d_pts, const size_t* offsets, size_t k)
+{
+    return batch_run(b, d_pts, offsets, k, true);
+}
+
+size_t qdt_gpu_batch_sets(const qdt_gpu_batch* b) { return b ? b->k : 0; }
+
+qdt_status qdt_gpu_batch_set_status(const qdt_gpu_batch* b, size_t set)
+{
+    if (!b) return QDT_ERR_NULL_CONTEXT;
+    const qdt::BatchResult* r = batch_result(b, set);
+    if (!r) return QDT_ERR_INVALID_INDEX;
+    return (qdt_status)r->status[set];
+}
+
+size_t qdt_gpu_batch_triangles_size(const qdt_gpu_batch* b, size_t set)
+{
+    const qdt::BatchResult* r = batch_result(b, set);
+    return r ? (size_t)r->tri_count[set] : 0;
+}
+
+size_t qdt_gpu_batch_unique_points(const qdt_gpu_batch* b, size_t set)
+{
+    const qdt::BatchResult* r = batch_result(b, set);
+    return r ? (size_t)r->unique[set] : 0;
+}
+
+size_t qdt_gpu_batch_get_triangle(const qdt_gpu_batch* b, size_t set, size_t T[3], size_t id)
+{
+    const qdt::BatchResult* r = batch_result(b, set);
+    if (!r || !T || id >= r->tri_count[set]) return 0;
+    if (!batch_materialize(b)) return 0;
+    const uint32_t* t = b->tris.p + 3 * (r->tri_start[set] + id);
+    T[0] = t[0];
+    T[1] = t[1];
+    T[2] = t[2];
+    return 1;
+}
+
+int qdt_gpu_batch_set_range(const qdt_gpu_batch* b, size_t set, size_t* first, size_t* count)
+{
+    const qdt::BatchResult* r = batch_result(b, set);
+    if (!r) return 0;
+    if (first) *first = (size_t)r->tri_start[set];
+    if (count) *count = (size_t)r->tri_count[set];
+    return 1;
+}
+
+size_t qdt_gpu_batch_copy_triangles(const qdt_gpu_batch* b, uint32_t* out)
+{
+    if (!b || !b->eng || !b->k || !out) return 0;
+    const qdt::BatchResult& r = b->eng->batch();
+    if (b->tris_host)
+    {
+        std::memcpy(out, b->tris.p, 12 * (size_t)r.triangles);
+        return (size_t)r.triangles;
+    }
+    if (r.triangles && hipMemcpy(out, r.tris, 12 * (size_t)r.triangles, hipMemcpyDeviceToHost) != hipSuccess)
+        return 0;
+    return (size_t)r.triangles;
+}
+
+size_t qdt_gpu_batch_device_triangles(const qdt_gpu_batch* b, const uint32_t** d_tris)
+{
+    if (!b || !b->eng || !b->k)
+    {
+        if (d_tris) *d_tris = nullptr;
+        return 0;
+    }
+    const qdt::BatchResult& r = b->eng->batch();
+    if (d_tris) *d_tris = r.tris;
+    return (size_t)r.triangles;
+}
+
+size_t qdt_gpu_batch_copy_rep(const qdt_gpu_batch* b, uint32_t* out)
+{
+    if (!b || !b->eng || !b->k || !out) return 0;
+    const qdt::BatchResult& r = b->eng->batch();
+    if (r.n && hipMemcpy(out, r.rep, 4 * r.n, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    return r.n;
+}
+
+int qdt_gpu_batch_get_stats(const qdt_gpu_batch* b, qdt_gpu_stats* out)
+{
+    if (!b || !out) return 0;
+    *out = b->stats;
+    return 1;
+}
+
+void qdt_gpu_batch_set_profiling(qdt_gpu_batch* b, int enable)
+{
+    if (!b) return;
+    b->profiling = enable != 0;
+    if (b->eng) b->eng->set_profiling(b->profiling);
+}
+
+void qdt_gpu_batch_set_seams(qdt_gpu_batch* b, int enable)
+{
+    if (!b) return;
+    b->seams = enable != 0;
+    if (b->eng) b->eng->set_seams(b->seams);
 }
 
 } // extern "C"

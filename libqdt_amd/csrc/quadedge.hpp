@@ -281,6 +281,11 @@ struct QuadEdges
         }
         pool.head = rec0;
         pool.tail = rec0 + nrec - 1;
+        if (cnt == 1)
+        {
+            first = second = -1; // a point set of one distinct point
+            return;
+        }
         if (cnt == 2)
         {
             const int a = make_edge(pool);
