@@ -123,6 +123,43 @@ QDT_API int qdt_gpu_batch_get_stats(const qdt_gpu_batch* b, qdt_gpu_stats* out);
 QDT_API void qdt_gpu_batch_set_profiling(qdt_gpu_batch* b, int enable);
 QDT_API void qdt_gpu_batch_set_seams(qdt_gpu_batch* b, int enable);
 
+/* ---- Serving form: a request loop with continuous batching ----
+ *
+ * No counterpart in the reference.  Requests (host point sets) can be
+ * submitted at any time from any thread.  `workers` threads, each with its
+ * own HIP stream and resident device workspace, take every queued request
+ * that fits beside the running batches within hbm_budget_bytes (about
+ * 320 bytes per point are counted) and triangulate them together in one
+ * batch, so requests that arrive while others run are admitted as memory
+ * frees up and their batches overlap on the device.  A request's result is
+ * the same as qdt_execute gives for it alone. */
+typedef struct qdt_gpu_server qdt_gpu_server;
+
+/* NULL without a GPU or memory. */
+QDT_API qdt_gpu_server* qdt_gpu_server_create(size_t hbm_budget_bytes, int workers);
+/* Finishes the queued requests, then stops the workers. */
+QDT_API void qdt_gpu_server_destroy(qdt_gpu_server* s);
+/* Copies the n points and queues them; returns the request id (0 on a NULL
+ * argument or no memory). */
+QDT_API uint64_t qdt_gpu_server_submit(qdt_gpu_server* s, const qdt_vec2d* pts, size_t n);
+/* 0 queued, 1 running, 2 done, -1 unknown id. */
+QDT_API int qdt_gpu_server_poll(qdt_gpu_server* s, uint64_t id);
+/* Blocks until the request is done; returns its status (QDT_OK,
+ * QDT_ERR_NOT_FINITE, QDT_ERR_TOO_MANY_POINTS, QDT_ERR_OUT_OF_MEMORY when it
+ * cannot fit the budget alone, ...) and the submit-to-done latency. */
+QDT_API qdt_status qdt_gpu_server_wait(qdt_gpu_server* s, uint64_t id, double* latency_ms);
+/* Submit, start and done times (ms, steady clock) of a done request. */
+QDT_API int qdt_gpu_server_times(qdt_gpu_server* s, uint64_t id, double* submit_ms, double* start_ms,
+                                 double* done_ms);
+QDT_API size_t qdt_gpu_server_triangles_size(qdt_gpu_server* s, uint64_t id);
+QDT_API size_t qdt_gpu_server_unique_points(qdt_gpu_server* s, uint64_t id);
+/* 3 * T uint32 request-local indices, counter-clockwise; returns T. */
+QDT_API size_t qdt_gpu_server_copy_triangles(qdt_gpu_server* s, uint64_t id, uint32_t* out);
+/* Forgets a done request. */
+QDT_API void qdt_gpu_server_release(qdt_gpu_server* s, uint64_t id);
+/* Batches run so far and the requests they held. */
+QDT_API int qdt_gpu_server_counters(qdt_gpu_server* s, uint64_t* batches, uint64_t* requests);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,10 @@ GPU_EXPORTS = [
     "qdt_gpu_batch_unique_points", "qdt_gpu_batch_get_triangle", "qdt_gpu_batch_set_range",
     "qdt_gpu_batch_copy_triangles", "qdt_gpu_batch_device_triangles", "qdt_gpu_batch_copy_rep",
     "qdt_gpu_batch_get_stats", "qdt_gpu_batch_set_profiling", "qdt_gpu_batch_set_seams",
+    "qdt_gpu_server_create", "qdt_gpu_server_destroy", "qdt_gpu_server_submit", "qdt_gpu_server_poll",
+    "qdt_gpu_server_wait", "qdt_gpu_server_times", "qdt_gpu_server_triangles_size",
+    "qdt_gpu_server_unique_points", "qdt_gpu_server_copy_triangles", "qdt_gpu_server_release",
+    "qdt_gpu_server_counters",
 ]
 
 
@@ -158,6 +162,29 @@ def lib():
         for f in ("qdt_gpu_batch_set_profiling", "qdt_gpu_batch_set_seams"):
             getattr(L, f).restype = None
             getattr(L, f).argtypes = [vp, ctypes.c_int]
+        # serving form
+        u64, dp = ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)
+        L.qdt_gpu_server_create.restype = vp
+        L.qdt_gpu_server_create.argtypes = [sz, ctypes.c_int]
+        L.qdt_gpu_server_destroy.restype = None
+        L.qdt_gpu_server_destroy.argtypes = [vp]
+        L.qdt_gpu_server_submit.restype = u64
+        L.qdt_gpu_server_submit.argtypes = [vp, vp, sz]
+        L.qdt_gpu_server_poll.restype = ctypes.c_int
+        L.qdt_gpu_server_poll.argtypes = [vp, u64]
+        L.qdt_gpu_server_wait.restype = st
+        L.qdt_gpu_server_wait.argtypes = [vp, u64, dp]
+        L.qdt_gpu_server_times.restype = ctypes.c_int
+        L.qdt_gpu_server_times.argtypes = [vp, u64, dp, dp, dp]
+        for f in ("qdt_gpu_server_triangles_size", "qdt_gpu_server_unique_points"):
+            getattr(L, f).restype = sz
+            getattr(L, f).argtypes = [vp, u64]
+        L.qdt_gpu_server_copy_triangles.restype = sz
+        L.qdt_gpu_server_copy_triangles.argtypes = [vp, u64, vp]
+        L.qdt_gpu_server_release.restype = None
+        L.qdt_gpu_server_release.argtypes = [vp, u64]
+        L.qdt_gpu_server_counters.restype = ctypes.c_int
+        L.qdt_gpu_server_counters.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         _lib = L
     return _lib
 
@@ -383,6 +410,72 @@ class Batch:
 
     def set_seams(self, on=True):
         lib().qdt_gpu_batch_set_seams(self._h, 1 if on else 0)
+
+
+class Server:
+    """Request loop with continuous batching (include/qdt_gpu.h,
+    qdt_gpu_server_*).  submit() may be called from any thread."""
+
+    def __init__(self, hbm_budget_bytes=200 << 30, workers=2):
+        self._h = lib().qdt_gpu_server_create(int(hbm_budget_bytes), int(workers))
+        if not self._h:
+            raise RuntimeError("qdt_gpu_server_create failed (no GPU?)")
+
+    def close(self):
+        if self._h:
+            lib().qdt_gpu_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, pts):
+        a = _as_points(pts)
+        rid = int(lib().qdt_gpu_server_submit(self._h, a.ctypes.data if len(a) else None, len(a)))
+        if not rid:
+            raise MemoryError("qdt_gpu_server_submit failed")
+        return rid
+
+    def poll(self, rid):
+        return int(lib().qdt_gpu_server_poll(self._h, rid))
+
+    def wait(self, rid):
+        """(status, latency in ms)"""
+        lat = ctypes.c_double()
+        st = int(lib().qdt_gpu_server_wait(self._h, rid, ctypes.byref(lat)))
+        return st, lat.value
+
+    def times(self, rid):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        ok = lib().qdt_gpu_server_times(self._h, rid, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return (a.value, b.value, c.value) if ok else None
+
+    def triangles(self, rid):
+        t = int(lib().qdt_gpu_server_triangles_size(self._h, rid))
+        out = np.zeros((t, 3), dtype=np.uint32)
+        if t:
+            lib().qdt_gpu_server_copy_triangles(self._h, rid, out.ctypes.data)
+        return out
+
+    def unique_points(self, rid):
+        return int(lib().qdt_gpu_server_unique_points(self._h, rid))
+
+    def release(self, rid):
+        lib().qdt_gpu_server_release(self._h, rid)
+
+    def counters(self):
+        b, r = ctypes.c_uint64(), ctypes.c_uint64()
+        lib().qdt_gpu_server_counters(self._h, ctypes.byref(b), ctypes.byref(r))
+        return b.value, r.value
 
 
 def triangulate(pts):

@@ -1,0 +1,404 @@
+// qdt_serve.cpp -- serving form: a request loop with continuous batching.
+//
+// Callers submit point sets at any time.  Worker threads, each owning an
+// Engine (its own HIP stream and device workspace, kept resident between
+// batches), take every queued request that fits the HBM budget the moment
+// they are free and triangulate them together as one batch (Engine::
+// run_batch).  Requests that arrive while a batch runs are picked up by the
+// next free worker, so batches overlap on the device.  Each request's result
+// is exactly what qdt_execute gives for it alone (the batch path keeps every
+// set's triangulation independent).  No counterpart in the reference.
+#include "../../include/qdt.h"
+#include "../../include/qdt_gpu.h"
+#include "engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace
+{
+
+constexpr size_t kMaxPoints = 100000000;  // per request, libqdt/wrapper.hxx:15
+constexpr size_t kBytesPerPoint = 320;    // device workspace + input per point (engine buffers, rounded up)
+constexpr size_t kMaxBatchPoints = 64u << 20;
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request
+{
+    uint64_t id = 0;
+    std::vector<qdt_vec2d> pts;
+    std::vector<uint32_t> tris; // 3 per triangle, request-local indices
+    size_t unique = 0;
+    qdt_status status = QDT_OK;
+    int state = 0; // 0 queued, 1 running, 2 done
+    double t_submit = 0, t_start = 0, t_done = 0;
+};
+
+struct Worker
+{
+    qdt::Engine* eng = nullptr;
+    void* d_in = nullptr;
+    size_t d_in_cap = 0;
+    void* h_in = nullptr; // pinned staging of a batch's points
+    size_t h_in_cap = 0;
+    void* h_out = nullptr; // pinned staging of a batch's triangles
+    size_t h_out_cap = 0;
+    std::thread th;
+};
+
+bool grow_pinned(void*& p, size_t& cap, size_t bytes)
+{
+    if (bytes <= cap) return true;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t c = std::max(bytes, cap * 2);
+    if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess) return false;
+    cap = c;
+    return true;
+}
+
+bool grow_device(void*& p, size_t& cap, size_t bytes)
+{
+    if (bytes <= cap) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t c = std::max(bytes, cap * 2);
+    if (hipMalloc(&p, c) != hipSuccess) return false;
+    cap = c;
+    return true;
+}
+
+} // namespace
+
+struct qdt_gpu_server
+{
+    size_t budget = 0;      // device bytes the in-flight batches may use
+    size_t in_flight = 0;   // device bytes of the batches running now
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::unordered_map<uint64_t, Request*> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::vector<Worker*> workers;
+    uint64_t batches = 0, batched_requests = 0;
+
+    void work(Worker* w);
+    ~qdt_gpu_server();
+};
+
+void qdt_gpu_server::work(Worker* w)
+{
+    std::vector<Request*> take;
+    std::vector<size_t> off;
+    for (;;)
+    {
+        size_t need = 0;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            if (stop && queue.empty()) return;
+            // admission: the queue's head requests while they fit the budget
+            take.clear();
+            size_t pts = 0;
+            while (!queue.empty())
+            {
+                Request* r = queue.front();
+                const size_t b = kBytesPerPoint * std::max<size_t>(r->pts.size(), 64);
+                if (!take.empty() && (pts + r->pts.size() > kMaxBatchPoints || in_flight + need + b > budget)) break;
+                if (take.empty() && in_flight + b > budget)
+                {
+                    if (in_flight == 0)
+                    {
+                        // alone it does not fit: reject it
+                        queue.pop_front();
+                        r->status = QDT_ERR_OUT_OF_MEMORY;
+                        r->state = 2;
+                        r->t_start = r->t_done = now_ms();
+                        cv_done.notify_all();
+                        continue;
+                    }
+                    break; // wait for memory to free up
+                }
+                queue.pop_front();
+                r->state = 1;
+                take.push_back(r);
+                need += b;
+                pts += r->pts.size();
+            }
+            if (take.empty())
+            {
+                // the head does not fit next to the running batches: wait
+                // until enough of them finish
+                cv_done.wait(lk, [&] {
+                    if (stop || queue.empty()) return true;
+                    const size_t hb = kBytesPerPoint * std::max<size_t>(queue.front()->pts.size(), 64);
+                    return in_flight + hb <= budget || in_flight == 0;
+                });
+                continue;
+            }
+            in_flight += need;
+            ++batches;
+            batched_requests += take.size();
+        }
+        const double t0 = now_ms();
+        for (Request* r : take) r->t_start = t0;
+        // one batch: stage the points, run, split the triangles per request
+        off.assign(take.size() + 1, 0);
+        for (size_t i = 0; i < take.size(); ++i) off[i + 1] = off[i] + take[i]->pts.size();
+        const size_t n = off.back();
+        qdt_status st = QDT_OK;
+        if (!grow_pinned(w->h_in, w->h_in_cap, 16 * std::max<size_t>(n, 1)) ||
+            !grow_device(w->d_in, w->d_in_cap, 16 * std::max<size_t>(n, 1)))
+            st = QDT_ERR_OUT_OF_MEMORY;
+        if (st == QDT_OK)
+        {
+            for (size_t i = 0; i < take.size(); ++i)
+                std::memcpy((qdt_vec2d*)w->h_in + off[i], take[i]->pts.data(), 16 * take[i]->pts.size());
+            if (hipMemcpyAsync(w->d_in, w->h_in, 16 * n, hipMemcpyHostToDevice, w->eng->stream()) != hipSuccess)
+                st = QDT_ERR_INTERNAL;
+        }
+        if (st == QDT_OK)
+        {
+            const qdt::Status s = w->eng->run_batch((const double*)w->d_in, off.data(), (int)take.size());
+            if (s != qdt::Status::ok) st = s == qdt::Status::out_of_memory ? QDT_ERR_OUT_OF_MEMORY : QDT_ERR_INTERNAL;
+        }
+        const qdt::BatchResult& R = w->eng->batch();
+        if (st == QDT_OK && R.triangles)
+        {
+            if (!grow_pinned(w->h_out, w->h_out_cap, 12 * (size_t)R.triangles))
+                st = QDT_ERR_OUT_OF_MEMORY;
+            else if (hipMemcpyAsync(w->h_out, R.tris, 12 * (size_t)R.triangles, hipMemcpyDeviceToHost,
+                                    w->eng->stream()) != hipSuccess ||
+                     hipStreamSynchronize(w->eng->stream()) != hipSuccess)
+                st = QDT_ERR_INTERNAL;
+        }
+        for (size_t i = 0; i < take.size(); ++i)
+        {
+            Request* r = take[i];
+            if (st != QDT_OK)
+            {
+                r->status = st;
+                continue;
+            }
+            r->status = (qdt_status)R.status[i];
+            r->unique = R.unique[i];
+            const uint32_t* t = (const uint32_t*)w->h_out + 3 * R.tri_start[i];
+            try
+            {
+                r->tris.assign(t, t + 3 * R.tri_count[i]);
+            }
+            catch (...)
+            {
+                r->status = QDT_ERR_OUT_OF_MEMORY;
+            }
+            r->pts.clear();
+            r->pts.shrink_to_fit();
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            const double t1 = now_ms();
+            for (Request* r : take)
+            {
+                r->t_done = t1;
+                r->state = 2;
+            }
+            in_flight -= need;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+}
+
+qdt_gpu_server::~qdt_gpu_server()
+{
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+    }
+    cv_work.notify_all();
+    cv_done.notify_all();
+    for (Worker* w : workers)
+    {
+        if (w->th.joinable()) w->th.join();
+        delete w->eng;
+        if (w->d_in) (void)hipFree(w->d_in);
+        if (w->h_in) (void)hipHostFree(w->h_in);
+        if (w->h_out) (void)hipHostFree(w->h_out);
+        delete w;
+    }
+    for (auto& kv : reqs) delete kv.second;
+}
+
+namespace
+{
+
+Request* find(qdt_gpu_server* S, uint64_t id)
+{
+    auto it = S->reqs.find(id);
+    return it == S->reqs.end() ? nullptr : it->second;
+}
+
+} // namespace
+
+extern "C" {
+
+qdt_gpu_server* qdt_gpu_server_create(size_t hbm_budget_bytes, int workers)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return nullptr;
+    qdt_gpu_server* S = new (std::nothrow) qdt_gpu_server();
+    if (!S) return nullptr;
+    S->budget = hbm_budget_bytes;
+    const int nw = std::min(std::max(workers, 1), 8);
+    try
+    {
+        for (int i = 0; i < nw; ++i)
+        {
+            Worker* w = new Worker();
+            w->eng = new qdt::Engine();
+            S->workers.push_back(w);
+            if (!w->eng->stream())
+            {
+                delete S;
+                return nullptr;
+            }
+        }
+        for (Worker* w : S->workers) w->th = std::thread([S, w] { S->work(w); });
+    }
+    catch (...)
+    {
+        delete S;
+        return nullptr;
+    }
+    return S;
+}
+
+void qdt_gpu_server_destroy(qdt_gpu_server* s) { delete s; }
+
+uint64_t qdt_gpu_server_submit(qdt_gpu_server* S, const qdt_vec2d* pts, size_t n)
+{
+    if (!S || (n && !pts)) return 0;
+    Request* r = new (std::nothrow) Request();
+    if (!r) return 0;
+    r->t_submit = now_ms();
+    if (n > kMaxPoints)
+    {
+        r->status = QDT_ERR_TOO_MANY_POINTS;
+        r->state = 2;
+        r->t_start = r->t_done = r->t_submit;
+    }
+    else
+    {
+        try
+        {
+            r->pts.assign(pts, pts + n);
+        }
+        catch (...)
+        {
+            delete r;
+            return 0;
+        }
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    r->id = S->next_id++;
+    S->reqs[r->id] = r;
+    if (r->state == 0) S->queue.push_back(r);
+    S->cv_work.notify_one();
+    return r->id;
+}
+
+int qdt_gpu_server_poll(qdt_gpu_server* S, uint64_t id)
+{
+    if (!S) return -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    Request* r = find(S, id);
+    return r ? r->state : -1;
+}
+
+qdt_status qdt_gpu_server_wait(qdt_gpu_server* S, uint64_t id, double* latency_ms)
+{
+    if (!S) return QDT_ERR_NULL_CONTEXT;
+    std::unique_lock<std::mutex> lk(S->mu);
+    Request* r = find(S, id);
+    if (!r) return QDT_ERR_INVALID_INDEX;
+    S->cv_done.wait(lk, [&] { return r->state == 2; });
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    return r->status;
+}
+
+int qdt_gpu_server_times(qdt_gpu_server* S, uint64_t id, double* submit_ms, double* start_ms, double* done_ms)
+{
+    if (!S) return 0;
+    std::lock_guard<std::mutex> lk(S->mu);
+    Request* r = find(S, id);
+    if (!r || r->state != 2) return 0;
+    if (submit_ms) *submit_ms = r->t_submit;
+    if (start_ms) *start_ms = r->t_start;
+    if (done_ms) *done_ms = r->t_done;
+    return 1;
+}
+
+size_t qdt_gpu_server_triangles_size(qdt_gpu_server* S, uint64_t id)
+{
+    if (!S) return 0;
+    std::lock_guard<std::mutex> lk(S->mu);
+    Request* r = find(S, id);
+    return (r && r->state == 2) ? r->tris.size() / 3 : 0;
+}
+
+size_t qdt_gpu_server_unique_points(qdt_gpu_server* S, uint64_t id)
+{
+    if (!S) return 0;
+    std::lock_guard<std::mutex> lk(S->mu);
+    Request* r = find(S, id);
+    return (r && r->state == 2) ? r->unique : 0;
+}
+
+size_t qdt_gpu_server_copy_triangles(qdt_gpu_server* S, uint64_t id, uint32_t* out)
+{
+    if (!S || !out) return 0;
+    std::lock_guard<std::mutex> lk(S->mu);
+    Request* r = find(S, id);
+    if (!r || r->state != 2) return 0;
+    std::memcpy(out, r->tris.data(), 4 * r->tris.size());
+    return r->tris.size() / 3;
+}
+
+void qdt_gpu_server_release(qdt_gpu_server* S, uint64_t id)
+{
+    if (!S) return;
+    std::lock_guard<std::mutex> lk(S->mu);
+    Request* r = find(S, id);
+    if (!r || r->state != 2) return;
+    S->reqs.erase(id);
+    delete r;
+}
+
+int qdt_gpu_server_counters(qdt_gpu_server* S, uint64_t* batches, uint64_t* requests)
+{
+    if (!S) return 0;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (batches) *batches = S->batches;
+    if (requests) *requests = S->batched_requests;
+    return 1;
+}
+
+} // extern "C"
