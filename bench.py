@@ -19,6 +19,20 @@ Extra fields:
   dropin        qdt_execute() from host memory (H2D, pipeline, D2H, host
                 result store): the PCIe-inclusive rate of the drop-in call.
 
+  batched       the batched form: 4,096 sets, sizes log-uniform in [64, 65536]
+                (seed 3), each uniform / Gaussian (sigma 0.1) / 32-cluster
+                mixture; one qdt_gpu_batch_execute_device call per step
+                versus the same sets one qdt_gpu_execute_device call at a
+                time, and the reference (qdt_execute per set, one context
+                per host thread) as its CPU baseline.
+  serve         the serving form: a trace of requests (sizes log-uniform in
+                [1000, 2,000,000], the same mix) with Poisson arrivals at
+                each QPS, through qdt_gpu_server (continuous batching, 200 GB
+                budget): p50/p99 submit-to-done latency and points/s, beside
+                the plain per-request path on the same trace (one request at
+                a time in arrival order; each request's own qdt_execute time,
+                queueing included in its latency).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
 """
 import argparse
@@ -42,10 +56,147 @@ def traffic_per_launch():
     try:
         for line in open(p):
             if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+                return int(line.split(":")[1].split()[0])
     except OSError:
         pass
     return None
+
+
+def concat(sets):
+    import numpy as np
+
+    offs = np.zeros(len(sets) + 1, dtype=np.uint64)
+    for i, p in enumerate(sets):
+        offs[i + 1] = offs[i] + len(p)
+    return np.ascontiguousarray(np.concatenate(sets)), offs
+
+
+def bench_batched(args, Q, W, np, torch):
+    """The batched form against one call per set (see the module docstring)."""
+    sets = W.batch_sets(count=args.batch_sets, seed=3)
+    pts, offs = concat(sets)
+    n = len(pts)
+    dev = torch.from_numpy(pts).to("cuda:0")
+    torch.cuda.synchronize()
+    b = Q.Batch()
+    for _ in range(1):
+        assert b.execute_device(dev.data_ptr(), offs) == Q.QDT_OK
+    ts = []
+    for _ in range(max(args.steps // 2, 2)):
+        t0 = time.perf_counter()
+        assert b.execute_device(dev.data_ptr(), offs) == Q.QDT_OK
+        ts.append(time.perf_counter() - t0)
+    t_batch = statistics.median(ts)
+    tri_total = int(b.all_triangles().shape[0])
+    b.close()
+    # one call per set, device-resident input, same sets
+    t = Q.Triangulation()
+    for i in range(8):
+        t.execute_device(dev.data_ptr() + 16 * int(offs[i]), len(sets[i]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(len(sets)):
+        assert t.execute_device(dev.data_ptr() + 16 * int(offs[i]), len(sets[i])) == Q.QDT_OK
+    t_single = time.perf_counter() - t0
+    t.close()
+    out = {
+        "workload": "%d sets, sizes log-uniform [64, 65536] (seed 3), uniform / Gaussian / 32-cluster mix, "
+                    "%d points in all, device-resident" % (len(sets), n),
+        "sets_per_s": round(len(sets) / t_batch, 1),
+        "mpts_per_s": round(n / t_batch / 1e6, 3),
+        "ms_per_batch": round(t_batch * 1e3, 3),
+        "triangles": tri_total,
+        "one_call_per_set": {"sets_per_s": round(len(sets) / t_single, 1),
+                             "mpts_per_s": round(n / t_single / 1e6, 3), "ms": round(t_single * 1e3, 1)},
+        "speedup_vs_one_call_per_set": round(t_single / t_batch, 2),
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        try:
+            import pyoracle
+
+            if pyoracle.ref_available():
+                thr = min(len(os.sched_getaffinity(0)), 64)
+                dt, tri_ref = pyoracle.ref_batch_time(pts, offs, thr)
+                out["cpu_baseline"] = {"sets_per_s": round(len(sets) / dt, 1), "mpts_per_s": round(n / dt / 1e6, 3),
+                                       "cores": thr, "kind": "reference",
+                                       "sample": "all %d sets, reference qdt_execute per set, one context per "
+                                                 "host thread, %.2f s; triangles %d (GPU %d)"
+                                                 % (len(sets), dt, tri_ref, tri_total)}
+        except Exception as e:  # a report, never the measurement
+            out["cpu_baseline"] = {"error": str(e)}
+    del dev
+    return out
+
+
+def bench_serve(args, Q, W, np):
+    """The serving form against the plain per-request path (module docstring)."""
+    nreq = args.serve_requests
+    sizes = W.log_uniform_sizes(nreq, 1000, 2_000_000, 5)
+    reqs = [W.mixed_set(int(sz), 5_000_011 + i)[0] for i, sz in enumerate(sizes)]
+    npts = int(sum(len(r) for r in reqs))
+    # plain path: each request alone through qdt_execute (host input, host results)
+    t = Q.Triangulation()
+    t.execute(reqs[0])
+    dur = []
+    for r in reqs:
+        t0 = time.perf_counter()
+        assert t.execute(r) == Q.QDT_OK
+        dur.append(time.perf_counter() - t0)
+    t.close()
+    rng = np.random.default_rng(17)
+    gaps = rng.exponential(1.0, size=nreq)
+    rows = []
+    with Q.Server(hbm_budget_bytes=200 << 30, workers=args.serve_workers) as srv:
+        # warm the workers' workspaces
+        for rid in [srv.submit(reqs[i]) for i in range(min(4, nreq))]:
+            srv.wait(rid)
+            srv.release(rid)
+        for qps in args.serve_qps:
+            arr = np.cumsum(gaps) / qps
+            arr -= arr[0]
+            ids = []
+            b0 = srv.counters()
+            t0 = time.perf_counter()
+            for i in range(nreq):
+                delay = t0 + arr[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit(reqs[i]))
+            lat, first, last = [], None, None
+            for rid in ids:
+                st, l = srv.wait(rid)
+                assert st == Q.QDT_OK, Q.status_string(st)
+                sub, _, done = srv.times(rid)
+                lat.append(l)
+                first = sub if first is None else min(first, sub)
+                last = done if last is None else max(last, done)
+                srv.release(rid)
+            b1 = srv.counters()
+            # the plain path on the same arrivals: one request at a time, FIFO
+            end, slat = 0.0, []
+            for i in range(nreq):
+                start = max(arr[i], end)
+                end = start + dur[i]
+                slat.append((end - arr[i]) * 1e3)
+            rows.append({
+                "qps": qps,
+                "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "mpts_per_s": round(npts / ((last - first) * 1e-3) / 1e6, 2),
+                "batches": b1[0] - b0[0],
+                "plain_p50_ms": round(float(np.percentile(slat, 50)), 2),
+                "plain_p99_ms": round(float(np.percentile(slat, 99)), 2),
+                "plain_mpts_per_s": round(float(npts / (end - arr[0]) / 1e6), 2),
+            })
+            print("serve qps %s: %s" % (qps, rows[-1]), file=sys.stderr, flush=True)
+    return {
+        "workload": "%d requests, sizes log-uniform [1000, 2000000] (seed 5), uniform / Gaussian / 32-cluster "
+                    "mix, %d points in all (mean %d), Poisson arrivals, host input and results, 200 GB budget, "
+                    "%d workers" % (nreq, npts, npts // nreq, args.serve_workers),
+        "plain_path_ms_per_request_mean": round(1e3 * statistics.mean(dur), 2),
+        "by_qps": rows,
+    }
 
 
 def main():
@@ -58,6 +209,13 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--batch-sets", type=int, default=4096)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-qps", type=lambda v: [float(x) for x in v.split(",")],
+                    default=[100, 250, 500, 1000, 2500, 5000])
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -202,6 +360,13 @@ def main():
                 line["cpu_baseline"] = None
         except Exception as e:  # the baseline is a report, never the measurement
             line["cpu_baseline"] = {"error": str(e)}
+
+    if not args.no_batched:
+        print("batched form ...", file=sys.stderr, flush=True)
+        line["batched"] = bench_batched(args, Q, W, np, torch)
+    if not args.no_serve:
+        print("serving form ...", file=sys.stderr, flush=True)
+        line["serve"] = bench_serve(args, Q, W, np)
 
     print(json.dumps(line), flush=True)
 

@@ -38,11 +38,21 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Pinned host memory holding one batch's triangles; the batch's requests
+// point into it and the last one released returns it to the server's pool.
+struct ResultBuf
+{
+    void* p = nullptr;
+    size_t cap = 0;
+    int refs = 0;
+};
+
 struct Request
 {
     uint64_t id = 0;
     std::vector<qdt_vec2d> pts;
-    std::vector<uint32_t> tris; // 3 per triangle, request-local indices
+    ResultBuf* buf = nullptr;    // triangles: 3 * ntris uint32 at buf->p + tri_off
+    size_t tri_off = 0, ntris = 0; // (request-local indices)
     size_t unique = 0;
     qdt_status status = QDT_OK;
     int state = 0; // 0 queued, 1 running, 2 done
@@ -56,8 +66,6 @@ struct Worker
     size_t d_in_cap = 0;
     void* h_in = nullptr; // pinned staging of a batch's points
     size_t h_in_cap = 0;
-    void* h_out = nullptr; // pinned staging of a batch's triangles
-    size_t h_out_cap = 0;
     std::thread th;
 };
 
@@ -98,11 +106,57 @@ struct qdt_gpu_server
     uint64_t next_id = 1;
     bool stop = false;
     std::vector<Worker*> workers;
+    std::vector<ResultBuf*> pool; // free result buffers
     uint64_t batches = 0, batched_requests = 0;
 
     void work(Worker* w);
+    ResultBuf* get_buf(size_t bytes);   // with mu held
+    void put_ref(ResultBuf* b);         // with mu held
     ~qdt_gpu_server();
 };
+
+ResultBuf* qdt_gpu_server::get_buf(size_t bytes)
+{
+    ResultBuf* best = nullptr;
+    size_t at = 0;
+    for (size_t i = 0; i < pool.size(); ++i)
+        if (pool[i]->cap >= bytes && (!best || pool[i]->cap < best->cap))
+        {
+            best = pool[i];
+            at = i;
+        }
+    if (best)
+    {
+        pool.erase(pool.begin() + (long)at);
+        return best;
+    }
+    ResultBuf* b = new (std::nothrow) ResultBuf();
+    if (!b) return nullptr;
+    const size_t cap = std::max<size_t>(bytes, 1 << 20);
+    if (hipHostMalloc(&b->p, cap, hipHostMallocDefault) != hipSuccess)
+    {
+        delete b;
+        return nullptr;
+    }
+    b->cap = cap;
+    return b;
+}
+
+void qdt_gpu_server::put_ref(ResultBuf* b)
+{
+    if (!b || --b->refs > 0) return;
+    pool.push_back(b);
+    // keep the pool small: free the smallest buffers beyond eight
+    while (pool.size() > 8)
+    {
+        size_t lo = 0;
+        for (size_t i = 1; i < pool.size(); ++i)
+            if (pool[i]->cap < pool[lo]->cap) lo = i;
+        (void)hipHostFree(pool[lo]->p);
+        delete pool[lo];
+        pool.erase(pool.begin() + (long)lo);
+    }
+}
 
 void qdt_gpu_server::work(Worker* w)
 {
@@ -181,45 +235,47 @@ void qdt_gpu_server::work(Worker* w)
             if (s != qdt::Status::ok) st = s == qdt::Status::out_of_memory ? QDT_ERR_OUT_OF_MEMORY : QDT_ERR_INTERNAL;
         }
         const qdt::BatchResult& R = w->eng->batch();
+        ResultBuf* buf = nullptr;
         if (st == QDT_OK && R.triangles)
         {
-            if (!grow_pinned(w->h_out, w->h_out_cap, 12 * (size_t)R.triangles))
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                buf = get_buf(12 * (size_t)R.triangles);
+            }
+            if (!buf)
                 st = QDT_ERR_OUT_OF_MEMORY;
-            else if (hipMemcpyAsync(w->h_out, R.tris, 12 * (size_t)R.triangles, hipMemcpyDeviceToHost,
+            else if (hipMemcpyAsync(buf->p, R.tris, 12 * (size_t)R.triangles, hipMemcpyDeviceToHost,
                                     w->eng->stream()) != hipSuccess ||
                      hipStreamSynchronize(w->eng->stream()) != hipSuccess)
                 st = QDT_ERR_INTERNAL;
         }
-        for (size_t i = 0; i < take.size(); ++i)
+        for (Request* r : take)
         {
-            Request* r = take[i];
-            if (st != QDT_OK)
-            {
-                r->status = st;
-                continue;
-            }
-            r->status = (qdt_status)R.status[i];
-            r->unique = R.unique[i];
-            const uint32_t* t = (const uint32_t*)w->h_out + 3 * R.tri_start[i];
-            try
-            {
-                r->tris.assign(t, t + 3 * R.tri_count[i]);
-            }
-            catch (...)
-            {
-                r->status = QDT_ERR_OUT_OF_MEMORY;
-            }
             r->pts.clear();
             r->pts.shrink_to_fit();
         }
         {
             std::lock_guard<std::mutex> lk(mu);
             const double t1 = now_ms();
-            for (Request* r : take)
+            for (size_t i = 0; i < take.size(); ++i)
             {
+                Request* r = take[i];
+                r->status = st != QDT_OK ? st : (qdt_status)R.status[i];
+                if (st == QDT_OK)
+                {
+                    r->unique = R.unique[i];
+                    r->ntris = R.tri_count[i];
+                    r->tri_off = R.tri_start[i];
+                    if (buf && r->ntris)
+                    {
+                        r->buf = buf;
+                        ++buf->refs;
+                    }
+                }
                 r->t_done = t1;
                 r->state = 2;
             }
+            if (buf && buf->refs == 0) pool.push_back(buf);
             in_flight -= need;
         }
         cv_done.notify_all();
@@ -241,10 +297,18 @@ qdt_gpu_server::~qdt_gpu_server()
         delete w->eng;
         if (w->d_in) (void)hipFree(w->d_in);
         if (w->h_in) (void)hipHostFree(w->h_in);
-        if (w->h_out) (void)hipHostFree(w->h_out);
         delete w;
     }
-    for (auto& kv : reqs) delete kv.second;
+    for (auto& kv : reqs)
+    {
+        put_ref(kv.second->buf);
+        delete kv.second;
+    }
+    for (ResultBuf* b : pool)
+    {
+        (void)hipHostFree(b->p);
+        delete b;
+    }
 }
 
 namespace
@@ -361,7 +425,7 @@ size_t qdt_gpu_server_triangles_size(qdt_gpu_server* S, uint64_t id)
     if (!S) return 0;
     std::lock_guard<std::mutex> lk(S->mu);
     Request* r = find(S, id);
-    return (r && r->state == 2) ? r->tris.size() / 3 : 0;
+    return (r && r->state == 2) ? r->ntris : 0;
 }
 
 size_t qdt_gpu_server_unique_points(qdt_gpu_server* S, uint64_t id)
@@ -378,8 +442,8 @@ size_t qdt_gpu_server_copy_triangles(qdt_gpu_server* S, uint64_t id, uint32_t* o
     std::lock_guard<std::mutex> lk(S->mu);
     Request* r = find(S, id);
     if (!r || r->state != 2) return 0;
-    std::memcpy(out, r->tris.data(), 4 * r->tris.size());
-    return r->tris.size() / 3;
+    if (r->ntris) std::memcpy(out, (const uint32_t*)r->buf->p + 3 * r->tri_off, 12 * r->ntris);
+    return r->ntris;
 }
 
 void qdt_gpu_server_release(qdt_gpu_server* S, uint64_t id)
@@ -389,6 +453,7 @@ void qdt_gpu_server_release(qdt_gpu_server* S, uint64_t id)
     Request* r = find(S, id);
     if (!r || r->state != 2) return;
     S->reqs.erase(id);
+    S->put_ref(r->buf);
     delete r;
 }
 

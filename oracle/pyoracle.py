@@ -207,6 +207,23 @@ def ref_triangulate(pts, parallel=False, timed=False):
         R.qdt_destroy(q)
 
 
+def ref_batch_time(pts, offsets, threads):
+    """Wall time (s) of the reference triangulating every set [offsets[i],
+    offsets[i+1]) with qdt_execute on `threads` host threads (one context per
+    thread), and the total triangle count."""
+    R = ref()
+    a = _xy(pts)
+    o = np.ascontiguousarray(offsets, dtype=np.uint64)
+    tot = ctypes.c_uint64()
+    R.ref_batch_seconds.restype = ctypes.c_double
+    R.ref_batch_seconds.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_uint64)]
+    dt = R.ref_batch_seconds(a.ctypes.data, o.ctypes.data, len(o) - 1, int(threads), ctypes.byref(tot))
+    if dt < 0:
+        raise RuntimeError("reference failed on a set")
+    return dt, tot.value
+
+
 def ref_time(pts, parallel=True):
     """Wall time (s) of one reference batch call, without reading results."""
     import time

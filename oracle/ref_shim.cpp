@@ -32,3 +32,38 @@ extern "C" size_t ref_copy_triangles(const qdt_t* q, uint32_t* out)
             for (int k = 0; k < 3; ++k) out[3 * i + k] = static_cast<uint32_t>(T->triangles[i][k]);
     return n;
 }
+
+// Many independent sets on a pool of host threads, each thread with its own
+// reference context (qdt_execute, serial path, per set): the CPU baseline of
+// the batched form.  Returns the wall seconds of the whole pool, or -1.
+#include <atomic>
+#include <chrono>
+#include <thread>
+#include <vector>
+
+extern "C" double ref_batch_seconds(const double* xy, const uint64_t* offs, size_t k, int threads,
+                                    uint64_t* total_tris)
+{
+    std::atomic<size_t> next{ 0 };
+    std::atomic<uint64_t> tris{ 0 };
+    std::atomic<int> bad{ 0 };
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            qdt_t* q = qdt_create();
+            for (;;)
+            {
+                const size_t i = next.fetch_add(1);
+                if (i >= k) break;
+                const qdt_vec2d* p = reinterpret_cast<const qdt_vec2d*>(xy + 2 * offs[i]);
+                if (qdt_execute(q, p, (size_t)(offs[i + 1] - offs[i])) != QDT_OK) bad.fetch_add(1);
+                tris.fetch_add(qdt_triangles_size(q));
+            }
+            qdt_destroy(q);
+        });
+    for (std::thread& th : pool) th.join();
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (total_tris) *total_tris = tris.load();
+    return bad.load() ? -1.0 : dt;
+}
