@@ -95,3 +95,32 @@ def test_no_gpu_is_a_loud_error():
         st = t.execute(np.array([[0, 0], [1, 0], [0, 1]], np.float64))
         assert st == Q.QDT_ERR_INTERNAL
         assert t.points_size() == 0 and t.triangles_size() == 0
+
+
+def test_batch_validation_needs_no_gpu():
+    """qdt_gpu_batch_*: offsets and pointer checks come before any device
+    work; with no GPU the call itself fails loudly (no CPU fallback)."""
+    import numpy as np
+
+    b = Q.Batch()
+    pts = np.zeros((4, 2))
+    assert b.execute(pts, [1, 4]) == Q.QDT_ERR_INVALID_INDEX
+    assert b.execute(pts, [0, 3, 2]) == Q.QDT_ERR_INVALID_INDEX
+    assert b.execute(pts, [0]) == Q.QDT_OK and b.sets() == 0
+    L = Q.lib()
+    assert L.qdt_gpu_batch_execute(None, None, None, 0) == Q.QDT_ERR_NULL_CONTEXT
+    assert L.qdt_gpu_batch_execute(b._h, None, None, 2) == Q.QDT_ERR_NULL_ARGUMENT
+    assert b.status(0) == Q.QDT_ERR_INVALID_INDEX
+    assert b.triangles_size(0) == 0 and b.get_triangle(0, 0) is None
+    if not Q.device_count():
+        assert b.execute(pts, [0, 4]) == Q.QDT_ERR_INTERNAL
+        assert b.sets() == 0
+    b.close()
+
+
+def test_server_needs_a_gpu():
+    if Q.device_count():
+        return
+    assert Q.lib().qdt_gpu_server_create(1 << 30, 2) is None
+    assert Q.lib().qdt_gpu_server_poll(None, 1) == -1
+    assert Q.lib().qdt_gpu_server_wait(None, 1, None) == Q.QDT_ERR_NULL_CONTEXT
