@@ -41,6 +41,7 @@ typedef struct qdt_gpu_stats
     uint64_t seam_counters[8];    /* anchors certified, anchors inserted, pieces run, pieces
                                      resumed, pieces stopped by a claimed vertex, anchors with a
                                      clean dry run, anchors selected, vertex rings rewritten */
+    uint64_t err_bits;            /* device error flags of the last run (0 when it succeeded) */
 } qdt_gpu_stats;
 
 /* Like qdt_execute(), but d_pts points to n points in device memory.  The
@@ -67,6 +68,10 @@ QDT_API void qdt_gpu_set_profiling(qdt_t* ptr, int enable);
 /* Cuts large merges into seam pieces that run in parallel (default 1); 0
  * merges every kd node with one thread.  Results are identical either way. */
 QDT_API void qdt_gpu_set_seams(qdt_t* ptr, int enable);
+
+/* Builds the bottom kd levels in LDS cells, one wavefront per cell (default
+ * 1); 0 uses one launch per level there too.  Results are identical. */
+QDT_API void qdt_gpu_set_cells(qdt_t* ptr, int enable);
 
 /* Number of HIP devices visible to the library (0 without a GPU). */
 QDT_API int qdt_gpu_device_count(void);

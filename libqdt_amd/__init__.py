@@ -37,7 +37,8 @@ EXPORTS = [
 ]
 GPU_EXPORTS = [
     "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
-    "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_device_count", "qdt_gpu_release_workspace",
+    "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_set_cells", "qdt_gpu_device_count",
+    "qdt_gpu_release_workspace",
     "qdt_gpu_batch_create", "qdt_gpu_batch_destroy", "qdt_gpu_batch_execute", "qdt_gpu_batch_execute_device",
     "qdt_gpu_batch_sets", "qdt_gpu_batch_set_status", "qdt_gpu_batch_triangles_size",
     "qdt_gpu_batch_unique_points", "qdt_gpu_batch_get_triangle", "qdt_gpu_batch_set_range",
@@ -67,7 +68,7 @@ class GpuStats(ctypes.Structure):
         ("stage_ms", ctypes.c_double * 8), ("merge_launch_ms_sum", ctypes.c_double),
         ("merge_launches", ctypes.c_uint64), ("merge_edges", ctypes.c_uint64),
         ("seam_level_ms_sum", ctypes.c_double), ("seam_retries", ctypes.c_uint64),
-        ("seam_counters", ctypes.c_uint64 * 8),
+        ("seam_counters", ctypes.c_uint64 * 8), ("err_bits", ctypes.c_uint64),
     ]
     STAGES = ("keys_sort_y", "sort_x", "dedupe", "frame1_sort", "kd_levels", "leaves", "merges", "extract")
 
@@ -77,6 +78,7 @@ class GpuStats(ctypes.Structure):
         d["seam"] = dict(zip(("certified", "inserted", "pieces", "resumed", "stopped", "dry_run_ok", "selected",
                                   "rings_rebuilt"),
                              list(self.seam_counters)))
+        d["err_bits"] = int(self.err_bits)
         return d
 
 
@@ -129,6 +131,8 @@ def lib():
         L.qdt_gpu_set_profiling.argtypes = [vp, ctypes.c_int]
         L.qdt_gpu_set_seams.restype = None
         L.qdt_gpu_set_seams.argtypes = [vp, ctypes.c_int]
+        L.qdt_gpu_set_cells.restype = None
+        L.qdt_gpu_set_cells.argtypes = [vp, ctypes.c_int]
         L.qdt_gpu_device_count.restype = ctypes.c_int
         L.qdt_gpu_device_count.argtypes = []
         L.qdt_gpu_release_workspace.restype = None
@@ -302,6 +306,9 @@ class Triangulation:
 
     def set_seams(self, on=True):
         lib().qdt_gpu_set_seams(self._h, 1 if on else 0)
+
+    def set_cells(self, on=True):
+        lib().qdt_gpu_set_cells(self._h, 1 if on else 0)
 
     def release_workspace(self):
         lib().qdt_gpu_release_workspace(self._h)

@@ -327,12 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_roots(int* off, int d, int ra, int r
     if (i > nr) return;
     int* od = off + off_base(off, d);
     const uint32_t first = d > 0 ? 2 * lcount(off, d - 1) : 0;
-    if (i == 0)
-    {
-        if (first == 0) od[0] = (int)vstart[ra];
-        return;
-    }
-    od[first + i] = (int)vstart[ra + i];
+    od[first + i] = (int)vstart[ra + i]; // i = 0: where the level's children end
 }
 
 __global__ __launch_bounds__(kBlock) void k_children(int* off, int d)
@@ -490,10 +485,301 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
 
 
 // ---------------------------------------------------------------------------
+// 6-7 in LDS: cells.  One wavefront builds a whole kd subtree of at most
+// kCellMax points: its kd partition, leaves and merges run on a copy of the
+// cell's points and quad-edge records in LDS, and the records are written to
+// HBM once.  The steps are the same as k_leaves / k_merge (same cuts, same
+// leaf and merge code, same record ranges), so the store that leaves the
+// cell is the one the level kernels would have built.
+// ---------------------------------------------------------------------------
+
+constexpr int kCellMax = 128;             // points per cell at most
+constexpr int kCellNodes = 2 * kCellMax;  // subtree node slots (levels of 1, 2, 4, ... nodes)
+constexpr int kCellLevels = 12;
+
+using LQE = QuadEdges<int*, const double2*>;
+
+// cell c -> (level, node index): the nodes of level dc, then the roots of
+// the sets rooted below dc
+__device__ __forceinline__ bool cell_node(const int* off, int dc, int D, uint32_t c, int& dl, uint32_t& j)
+{
+    const uint32_t n0 = lcount(off, dc);
+    if (c < n0)
+    {
+        dl = dc;
+        j = c;
+        return true;
+    }
+    if (dc >= D) return false;
+    c -= n0;
+    for (int d = dc + 1; d <= D; ++d)
+    {
+        const uint32_t first = 2 * lcount(off, d - 1);
+        const uint32_t roots = lcount(off, d) - first;
+        if (c < roots)
+        {
+            dl = d;
+            j = first + c;
+            return true;
+        }
+        c -= roots;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
+                                             const uint32_t* B1, const int* off, int dc, int D, uint32_t m,
+                                             int lim_e, int4* res, int* err, unsigned long long* counters)
+{
+    __shared__ double2 P[kCellMax];
+    __shared__ int gid[kCellMax];
+    __shared__ int A0[kCellMax], A1[kCellMax], T[kCellMax], Z[kCellMax + 1];
+    __shared__ uint8_t right[kCellMax];
+    __shared__ int LNX[4 * 3 * kCellMax], LOR[2 * 3 * kCellMax], LVE[kCellMax];
+    __shared__ int loff[kCellNodes + kCellLevels]; // sub-level node offsets
+    __shared__ int lbase[kCellLevels + 1];        // first offset of each sub-level in loff
+    __shared__ int4 lres[kCellNodes];
+    __shared__ int nlev, bad;
+
+    const int lane = threadIdx.x;
+    int dl;
+    uint32_t j;
+    if (!cell_node(off, dc, D, blockIdx.x, dl, j)) return;
+    const int* od = off + off_base(off, dl);
+    const int o = od[j];
+    const int s = od[j + 1] - o;
+    if (s <= 0) return;
+    if (s > kCellMax)
+    {
+        if (lane == 0) atomicOr(err, 2 << 1);
+        return;
+    }
+    // points in frame-0 order; vertex ids within a node ascend in that order
+    for (int i = lane; i < s; i += 64)
+    {
+        const int v = (int)B0[o + i];
+        gid[i] = v;
+        P[i] = PT[v];
+        A0[i] = i;
+        LVE[i] = -1;
+    }
+    if (lane == 0)
+    {
+        bad = 0;
+        lbase[0] = 0;
+        loff[0] = 0;
+        loff[1] = s;
+    }
+    __syncthreads();
+    for (int i = lane; i < s; i += 64)
+    {
+        const int v = (int)B1[o + i];
+        int lo = 0, hi = s - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi) >> 1;
+            if (gid[mid] < v)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (gid[lo] != v) bad = 1;
+        A1[i] = lo;
+    }
+    __syncthreads();
+    // kd partition of the cell, sub-level by sub-level (absolute level dl + l)
+    int L = 0;
+    for (;;)
+    {
+        const int cnt = 1 << L;
+        const int* ol = loff + lbase[L];
+        bool mine = false;
+        for (int q = lane; q < cnt; q += 64) mine |= ol[q + 1] - ol[q] >= 4;
+        if (__ballot(mine) == 0) break;
+        if (L + 1 >= kCellLevels || (4 << L) - 1 > kCellNodes || lbase[L] + 3 * cnt + 2 > kCellNodes + kCellLevels)
+        {
+            bad = 1; // deeper than the slots allow: cannot happen for kCellMax points
+            break;
+        }
+        const int axis = (dl + L) & 1;
+        int* Af = axis ? A1 : A0;
+        int* Ao = axis ? A0 : A1;
+        // children offsets
+        const int nb = lbase[L] + cnt + 1;
+        int* oc = loff + nb;
+        for (int qn = lane; qn < cnt; qn += 64)
+        {
+            const int a = ol[qn], sz = ol[qn + 1] - a;
+            oc[2 * qn] = a;
+            oc[2 * qn + 1] = a + left_size(sz);
+        }
+        if (lane == 0)
+        {
+            lbase[L + 1] = nb;
+            oc[2 * cnt] = ol[cnt];
+        }
+        __syncthreads();
+        // membership from the positional split of the frame array
+        for (int i = lane; i < s; i += 64)
+        {
+            int q = 0; // node of position i: last q with ol[q] <= i (nodes are few)
+            int lo = 0, hi = cnt - 1;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (ol[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            q = lo;
+            const int a = ol[q], sz = ol[q + 1] - a;
+            right[Af[i]] = (uint8_t)((i - a) >= left_size(sz) ? 1 : 0);
+        }
+        __syncthreads();
+        // stable partition of the other array: exclusive count of lefts
+        // (two positions per lane, one wave-wide scan)
+        {
+            const int i0 = 2 * lane, i1 = 2 * lane + 1;
+            const int f0 = i0 < s ? (right[Ao[i0]] ? 0 : 1) : 0;
+            const int f1 = i1 < s ? (right[Ao[i1]] ? 0 : 1) : 0;
+            int x = f0 + f1;
+#pragma unroll
+            for (int dd = 1; dd < 64; dd <<= 1)
+            {
+                const int y = __shfl_up(x, dd, 64);
+                if (lane >= dd) x += y;
+            }
+            const int ex = x - f0 - f1;
+            if (i0 < s) Z[i0] = ex;
+            if (i1 < s) Z[i1] = ex + f0;
+            if (lane == 63) Z[s] = x;
+        }
+        __syncthreads();
+        for (int i = lane; i < s; i += 64)
+        {
+            int lo = 0, hi = cnt - 1;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (ol[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int a = ol[lo], sz = ol[lo + 1] - a;
+            const int v = Ao[i];
+            const int z = Z[i] - Z[a];
+            T[right[v] ? a + left_size(sz) + (i - a - z) : a + z] = v;
+        }
+        __syncthreads();
+        for (int i = lane; i < s; i += 64) Ao[i] = T[i];
+        __syncthreads();
+        ++L;
+    }
+    // A cell of at most three points is a leaf whose result belongs to its
+    // highest ancestor still below four points (the node its parent merges,
+    // as in k_leaves); roots have no parent.
+    int tdl = dl;
+    uint32_t tj = j;
+    if (s < 4)
+        while (tdl > 0 && tj < 2 * lcount(off, tdl - 1))
+        {
+            const int* op = off + off_base(off, tdl - 1);
+            const uint32_t pj = tj >> 1;
+            if (op[pj + 1] - op[pj] >= 4) break;
+            --tdl;
+            tj = pj;
+        }
+    LQE q;
+    q.init(LNX, LOR, P, LVE, 4 * 3 * s, s);
+    // leaves at sub-level L (absolute dl + L)
+    {
+        const int cnt = 1 << L;
+        const int* ol = loff + lbase[L];
+        for (int jl = lane; jl < cnt; jl += 64)
+        {
+            const int a = ol[jl], sz = ol[jl + 1] - a;
+            if (sz < 1) continue;
+            int db = L; // highest ancestor (sub-level) still below four points
+            int jb = jl;
+            while (db > 0)
+            {
+                const int* op = loff + lbase[db - 1];
+                if (op[(jb >> 1) + 1] - op[jb >> 1] >= 4) break;
+                --db;
+                jb >>= 1;
+            }
+            const int abs_db = db > 0 ? dl + db : tdl;
+            const int out_axis = abs_db > 0 ? ((abs_db - 1) & 1) : 0;
+            const int* B = out_axis ? A1 : A0;
+            int v[3];
+            for (int i = 0; i < sz; ++i) v[i] = B[a + i];
+            int first, second;
+            Pool pool;
+            q.leaf(v, sz, 3 * a, first, second, pool);
+            // node slot of (db, jb): sub-levels are numbered 2^l - 1 + index
+            lres[(1 << db) - 1 + jb] = make_int4(first, second, pool.head, pool.tail);
+        }
+    }
+    __syncthreads();
+    // merges, deepest sub-level first
+    for (int l = L - 1; l >= 0; --l)
+    {
+        const int cnt = 1 << l;
+        const int* ol = loff + lbase[l];
+        for (int jl = lane; jl < cnt; jl += 64)
+        {
+            const int sz = ol[jl + 1] - ol[jl];
+            if (sz < 4) continue;
+            const int4 Lr = lres[(2 << l) - 1 + 2 * jl];
+            const int4 Rr = lres[(2 << l) - 1 + 2 * jl + 1];
+            Pool pool = q.cat(Pool{ Lr.z, Lr.w }, Pool{ Rr.z, Rr.w });
+            const int abs_d = dl + l;
+            const int axis = abs_d & 1;
+            const int out_axis = abs_d > 0 ? ((abs_d - 1) & 1) : 0;
+            int first, second;
+            q.merge_node(Lr.x, Lr.y, Rr.x, Rr.y, axis, out_axis, sz, first, second, pool);
+            lres[(1 << l) - 1 + jl] = make_int4(first, second, pool.head, pool.tail);
+        }
+        __syncthreads();
+    }
+    // write the records, vertex edges and the cell's result
+    const int E0 = 12 * o; // local edge e is global E0 + e
+    const int R0 = 3 * o;  // local record k is global R0 + k
+    for (int k = lane; k < 3 * s; k += 64)
+    {
+        const int g = R0 + k;
+        const int o0 = LOR[2 * k], o1 = LOR[2 * k + 1];
+        if (o0 < 0)
+        {
+            const int nx = LNX[4 * k];
+            NX[4 * g] = nx >= 0 ? R0 + nx : -1;
+            OR[2 * g] = -1;
+            OR[2 * g + 1] = -1;
+            continue;
+        }
+        for (int r = 0; r < 4; ++r) NX[4 * g + r] = E0 + LNX[4 * k + r];
+        OR[2 * g] = gid[o0];
+        OR[2 * g + 1] = gid[o1];
+    }
+    for (int i = lane; i < s; i += 64) VE[gid[i]] = LVE[i] >= 0 ? E0 + LVE[i] : -1;
+    if (lane == 0)
+    {
+        const int4 r = lres[0];
+        res[node_index(off, tdl, tj)] = make_int4(r.x >= 0 ? E0 + r.x : -1, r.y >= 0 ? E0 + r.y : -1,
+                                                r.z >= 0 ? R0 + r.z : -1, r.w >= 0 ? R0 + r.w : -1);
+        if (bad) q.err = q.err ? q.err : 1;
+    }
+    flush_counts(counters, q.pc);
+    if (q.err) atomicOr(err, 2 << q.err);
+}
+
+// ---------------------------------------------------------------------------
 // 7b: large merges cut into seam pieces (see seam.hpp)
 // ---------------------------------------------------------------------------
 
-constexpr int kSegMaxK = 64;           // anchors per node at most
+constexpr int kSegMaxK = 256;          // anchors per node at most (a power of two)
 constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
 constexpr int kSegMin = 8192;          // smaller nodes merge in one piece
 constexpr int kSegBlock = 64;          // pieces per block
@@ -505,7 +791,7 @@ constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit mer
 __host__ __device__ inline int seam_anchors(double s)
 {
     if (s < kSegMin) return 0;
-    int k = (int)(std::sqrt(s) / 20.0);
+    int k = (int)(std::sqrt(s) / 8.0);
     return k < 2 ? 2 : (k > kSegMaxK ? kSegMaxK : k);
 }
 
@@ -664,82 +950,129 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_tangent(int* NX, int* OR, c
 }
 
 // Anchors in chain order, each strictly above the previous cut, and no two
-// cuts with a common end vertex (coordinates are cached in the records: no
-// pointer chasing here).  The tangent's rings changed after the anchors' dry
-// runs, so its two vertices are excluded as anchor ends too.
-__global__ __launch_bounds__(kMergeBlock) void k_seg_select(const int* OR, const double2* PT, const int* off, int d,
-                                                            int K, uint32_t m, int lim_e, const int* segstart,
-                                                            const int* arec,
-                                                            const double4* acoord, const double* akey, int* astate,
-                                                            int* selk, int* nsel, int* alist_n,
-                                                            unsigned long long* counters)
+// cuts with a common end vertex.  One workgroup per node: the candidates and
+// their cached coordinates go to LDS and are sorted by crossing height
+// (bitonic); one thread then takes them in order (exact orient2d against the
+// last cut; end vertices kept in an LDS hash set).  The tangent's rings
+// changed after the anchors' dry runs, so its two vertices are excluded as
+// anchor ends too.
+constexpr int kSelHash = 2048; // > 2 * (kSegMaxK + 1), a power of two
+
+__device__ __forceinline__ bool hset_add(int* h, int v) // false if present
 {
-    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
-    if (j == 0) *alist_n = 0;
+    unsigned x = ((unsigned)v * 2654435761u) & (kSelHash - 1);
+    for (int probe = 0; probe < kSelHash; ++probe)
+    {
+        if (h[x] == v) return false;
+        if (h[x] < 0)
+        {
+            h[x] = v;
+            return true;
+        }
+        x = (x + 1) & (kSelHash - 1);
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool hset_has(const int* h, int v)
+{
+    unsigned x = ((unsigned)v * 2654435761u) & (kSelHash - 1);
+    for (int probe = 0; probe < kSelHash; ++probe)
+    {
+        if (h[x] == v) return true;
+        if (h[x] < 0) return false;
+        x = (x + 1) & (kSelHash - 1);
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const double2* PT, const int* off, int d,
+                                                         int K, uint32_t m, int lim_e, const int* segstart,
+                                                         const int* arec, const double4* acoord,
+                                                         const double* akey, int* astate, int* selk, int* nsel,
+                                                         int* alist_n, unsigned long long* counters)
+{
+    __shared__ double skey[kSegMaxK];
+    __shared__ int sslot[kSegMaxK];
+    __shared__ int su[kSegMaxK], sv[kSegMaxK];
+    __shared__ double4 sc[kSegMaxK];
+    __shared__ int hset[kSelHash];
+    const uint32_t j = blockIdx.x;
+    const int k = threadIdx.x;
+    if (j == 0 && k == 0) *alist_n = 0;
     if (j >= lcount(off, d)) return;
-    nsel[j] = 0;
     const int* od = off + off_base(off, d);
     const int s = od[j + 1] - od[j];
-    int cnt = 0;
-    if (s >= 4)
+    if (k == 0) nsel[j] = 0;
+    if (s < 4) return;
+    const int b0 = segstart[(size_t)j * kSegSlots];
+    if (b0 < 0 || b0 >= lim_e) return; // the tangent failed: the run's error flag is set
+    const int pu0 = OR[(b0 ^ 2) >> 1], pv0 = OR[b0 >> 1];
+    if ((unsigned)pu0 >= m || (unsigned)pv0 >= m) return;
+    const size_t t = (size_t)j * K + k;
+    const bool valid = k < K && astate[t] == 1;
+    skey[k] = valid ? akey[t] : INFINITY;
+    sslot[k] = valid ? k : -1;
+    if (valid)
     {
-        const int b0 = segstart[(size_t)j * kSegSlots];
-        if (b0 < 0 || b0 >= lim_e) return; // the tangent failed: the run's error flag is set
-        int pu = OR[(b0 ^ 2) >> 1], pv = OR[b0 >> 1];
-        if ((unsigned)pu >= m || (unsigned)pv >= m) return;
-        int used[2 * kSegSlots];
-        int nused = 0;
-        used[nused++] = pu;
-        used[nused++] = pv;
-        double2 PU = PT[pu], PV = PT[pv];
-        unsigned dummy = 0;
-        // candidates by crossing height (insertion sort, K is small); the
-        // exact tests below drop any that the rounded keys misorder
-        int order[kSegMaxK];
-        double keys[kSegMaxK];
-        int nc = 0;
-        for (int k = 0; k < K; ++k)
+        su[k] = arec[t * kRec];
+        sv[k] = arec[t * kRec + 1];
+        sc[k] = acoord[t];
+    }
+    for (int x = k; x < kSelHash; x += kSegMaxK) hset[x] = -1;
+    __syncthreads();
+    // bitonic sort of (key, slot), ascending; invalid slots (infinite keys) last
+    for (int size = 2; size <= kSegMaxK; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
         {
-            const size_t t = (size_t)j * K + k;
-            if (astate[t] != 1) continue;
-            const double key = akey[t];
-            int i = nc++;
-            while (i > 0 && keys[i - 1] > key)
+            const int partner = k ^ stride;
+            if (partner > k)
             {
-                keys[i] = keys[i - 1];
-                order[i] = order[i - 1];
-                --i;
+                const bool up = (k & size) == 0;
+                const double a = skey[k], b = skey[partner];
+                const int sa = sslot[k], sb = sslot[partner];
+                // ties (and invalid pairs) by slot, so the order is a total one
+                const bool gt = a > b || (a == b && sa > sb);
+                if (gt == up)
+                {
+                    skey[k] = b;
+                    skey[partner] = a;
+                    sslot[k] = sb;
+                    sslot[partner] = sa;
+                }
             }
-            keys[i] = key;
-            order[i] = (int)t;
+            __syncthreads();
         }
-        for (int i = 0; i < nc; ++i)
+    if (k != 0) return;
+    hset_add(hset, pu0);
+    hset_add(hset, pv0);
+    double2 PU = PT[pu0], PV = PT[pv0];
+    unsigned dummy = 0;
+    int cnt = 0;
+    for (int i = 0; i < K; ++i)
+    {
+        const int sl = sslot[i];
+        if (sl < 0) continue;
+        const size_t tt = (size_t)j * K + sl;
+        const int u = su[sl], v = sv[sl];
+        const double4 c = sc[sl];
+        bool ok = u != v && !hset_has(hset, u) && !hset_has(hset, v);
+        ok = ok && orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
+        ok = ok && orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
+        if (!ok)
         {
-            const size_t t = (size_t)order[i];
-            const int* r = arec + t * kRec;
-            const int u = r[0], v = r[1];
-            const double4 c = acoord[t];
-            bool ok = true;
-            for (int q = 0; q < nused && ok; ++q) ok = used[q] != u && used[q] != v;
-            ok = ok && orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
-            ok = ok && orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
-            if (!ok)
-            {
-                astate[t] = 0;
-                continue;
-            }
-            astate[t] = 2;
-            used[nused++] = u;
-            used[nused++] = v;
-            selk[(size_t)j * kSegSlots + cnt++] = (int)t;
-            atomicAdd(&counters[11], 1ull);
-            pu = u;
-            pv = v;
-            PU = make_double2(c.x, c.y);
-            PV = make_double2(c.z, c.w);
+            astate[tt] = 0;
+            continue;
         }
+        astate[tt] = 2;
+        hset_add(hset, u);
+        hset_add(hset, v);
+        selk[(size_t)j * kSegSlots + cnt++] = (int)tt;
+        PU = make_double2(c.x, c.y);
+        PV = make_double2(c.z, c.w);
     }
     nsel[j] = cnt;
+    if (cnt) atomicAdd(&counters[11], (unsigned long long)cnt);
 }
 
 // Parallel anchor insertion.  Several anchors may cross the same edge (long
@@ -1271,7 +1604,7 @@ struct Engine::Bufs
     double* akey;
     int2 *apool, *segpool;
     int4* nstate;
-    size_t nodes_cap, offs_cap, seg_nodes, anchor_cap;
+    size_t nodes_cap, offs_cap, seg_nodes, anchor_cap, qv_nodes;
     // per set (ensure_sets)
     uint32_t *soff, *srank, *rpos, *vstart, *setbad, *in_off, *mstart, *sstart, *toff, *trank, *btris, *tmain;
     size_t sets_cap, tris_cap;
@@ -1293,6 +1626,7 @@ struct Engine::Plan
     std::vector<double> smax; // largest node of the level (bound)
     std::vector<int> K;       // anchors per node at the level (0: not cut into pieces)
     size_t nodes = 0, offs = 0, seg_nodes = 1, anchor_cap = 1;
+    size_t qv_nodes = 1; // node index bound of the levels cut into pieces (qv is indexed by node)
 };
 
 namespace
@@ -1333,12 +1667,14 @@ void make_plan(Engine::Plan& P, const std::vector<uint32_t>& sizes, const std::v
     P.offs = offs;
     P.seg_nodes = 1;
     P.anchor_cap = 1;
+    P.qv_nodes = 1;
     for (int d = 0; d < D; ++d)
     {
         const bool fits = (size_t)P.count[d] * kSegSlots < ((size_t)1 << kClaimBits);
         P.K[d] = seams && fits ? seam_anchors(P.smax[d]) : 0;
         if (!P.K[d]) continue;
         P.seg_nodes = std::max(P.seg_nodes, (size_t)P.count[d]);
+        P.qv_nodes = std::max(P.qv_nodes, (size_t)P.nodeb[d] + (size_t)P.count[d]);
         P.anchor_cap = std::max(P.anchor_cap, (size_t)P.count[d] * P.K[d]);
     }
 }
@@ -1418,7 +1754,7 @@ void Engine::release()
     }
     ws_bytes_ = ws_nodes_bytes_ = ws_sets_bytes_ = 0;
     cap_n_ = 0;
-    b_->nodes_cap = b_->offs_cap = b_->seg_nodes = b_->anchor_cap = 0;
+    b_->nodes_cap = b_->offs_cap = b_->seg_nodes = b_->anchor_cap = b_->qv_nodes = 0;
     b_->sets_cap = b_->tris_cap = 0;
     res_ = DeviceResult();
     batch_ = BatchResult();
@@ -1482,6 +1818,7 @@ Status Engine::ensure_nodes(const Plan& P)
 {
     Bufs& b = *b_;
     if (ws_nodes_ && P.nodes <= b.nodes_cap && P.offs <= b.offs_cap && P.seg_nodes <= b.seg_nodes &&
+        P.qv_nodes <= b.qv_nodes &&
         P.anchor_cap <= b.anchor_cap)
         return Status::ok;
     if (ws_nodes_) (void)hipFree(ws_nodes_);
@@ -1491,10 +1828,11 @@ Status Engine::ensure_nodes(const Plan& P)
     const size_t nodes = std::max(P.nodes, b.nodes_cap) + 64;
     const size_t offs = std::max(P.offs, b.offs_cap) + 64;
     const size_t seg_nodes = std::max(P.seg_nodes, b.seg_nodes);
+    const size_t qv_nodes = std::max(P.qv_nodes, b.qv_nodes);
     const size_t anchor_cap = std::max(P.anchor_cap, b.anchor_cap);
     const Item items[] = {
         { (void**)&b.off, 4 * (kLvlHdr + offs) }, { (void**)&b.res, 16 * nodes }, { (void**)&b.splitv, 4 * nodes },
-        { (void**)&b.qv, 4 * (size_t)kSegMaxK * seg_nodes },
+        { (void**)&b.qv, 4 * (size_t)kSegMaxK * qv_nodes },
         { (void**)&b.segstart, 4 * (size_t)kSegSlots * seg_nodes },
         { (void**)&b.nseg, 4 * seg_nodes }, { (void**)&b.nstate, 16 * seg_nodes },
         { (void**)&b.segpool, 8 * (size_t)kSegSlots * seg_nodes },
@@ -1511,6 +1849,7 @@ Status Engine::ensure_nodes(const Plan& P)
     b.nodes_cap = nodes;
     b.offs_cap = offs;
     b.seg_nodes = seg_nodes;
+    b.qv_nodes = qv_nodes;
     b.anchor_cap = anchor_cap;
     return Status::ok;
 }
@@ -1884,8 +2223,20 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         k_root_nodes<<<cdiv(std::max<uint32_t>(span, 1), kBlock), kBlock, 0, s>>>(
             b.off, d, ra, rb, vstart, d ? 2 * (uint32_t)P.count[d - 1] : 0u, node);
     };
+    // cells: the first level whose nodes all fit in LDS builds everything
+    // below it in one kernel (and so do the roots of sets rooted deeper)
+    int dc = D;
+    if (cells_)
+        for (int d = 0; d <= D; ++d)
+            if (P.smax[d] <= kCellMax)
+            {
+                dc = d;
+                break;
+            }
+    const bool use_cells = cells_ && P.smax[dc] <= kCellMax;
+    const int Dk = use_cells ? dc : D; // levels partitioned by the level kernels
     roots_at(0, nodeA);
-    for (int d = 0; d < D; ++d)
+    for (int d = 0; d < Dk; ++d)
     {
         const uint32_t cnt = (uint32_t)P.count[d];
         uint32_t*& Bf = (d & 1) ? B1 : B0;
@@ -1902,9 +2253,27 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         roots_at(d + 1, nodeB);
         std::swap(nodeA, nodeB);
     }
+    if (use_cells)
+        for (int d = Dk + 1; d <= D; ++d) // offsets of the roots below the cells' level
+        {
+            const int ra = P.ra[d], rb = P.ra[d + 1];
+            if (ra < rb)
+                k_roots<<<cdiv((size_t)(rb - ra) + 1, kBlock), kBlock, 0, s>>>(b.off, d, ra, rb, vstart);
+        }
 
     mark(5);
     // 6. leaves, 7. merges
+    if (use_cells)
+    {
+        uint32_t ncell = (uint32_t)P.count[dc];
+        for (int d = dc + 1; d <= D; ++d) ncell += (uint32_t)(P.count[d] - 2 * P.count[d - 1]);
+        st_.leaf_levels = (uint32_t)(D - dc);
+        st_.block_level = (uint32_t)dc;
+        if (ncell)
+            k_cell<<<ncell, 64, 0, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, dc, D, m, lim_e, b.res, b.err,
+                                        b.counters);
+    }
+    else
     {
         const uint32_t cnt = (uint32_t)P.count[D];
         k_leaves<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, D, m,
@@ -1913,7 +2282,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     mark(6);
     int epoch = 0; // claim epochs: one per piece launch
     uint64_t kmerge_levels = 0;
-    for (int d = D - 1; d >= 0; --d)
+    for (int d = Dk - 1; d >= 0; --d)
     {
         const uint32_t cnt = (uint32_t)P.count[d];
         mark(9 + 2 * (size_t)d);
@@ -1928,12 +2297,12 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         {
             const int total = (int)(cnt * (uint32_t)K);
             k_seg_anchor<<<cdiv((size_t)total, kMergeBlock), kMergeBlock, 0, s>>>(
-                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, D, d, K, m, lim_e, b.arec, b.acoord, b.akey,
+                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, Dk, d, K, m, lim_e, b.arec, b.acoord, b.akey,
                 b.astate, b.counters);
             k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
                                                                           b.spare, spare_base, spare_cap, b.res,
                                                                           b.segstart, b.nstate, b.err, b.counters);
-            k_seg_select<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.OR, b.P0, b.off, d, K, m, lim_e, b.segstart,
+            k_seg_select<<<cnt, kSegMaxK, 0, s>>>(b.OR, b.P0, b.off, d, K, m, lim_e, b.segstart,
                                                                          b.arec, b.acoord, b.akey, b.astate, b.selk,
                                                                          b.nsel, b.alist_n, b.counters);
             const int acap = (int)(b.anchor_cap * (2 + 4 * kCross));
@@ -1990,7 +2359,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         redo = true;
         return Status::ok;
     }
-    if (hs[1] & ~1) return Status::internal;
+    if (hs[1] & ~1)
+    {
+        st_.err_bits = hs[1];
+        return Status::internal;
+    }
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev0_, ev1_);
     st_.device_ms = ms;
@@ -2003,7 +2376,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     st_.merge_edges = cnts[13];
     st_.err_bits = hs[1];
     st_.triangles = hs[0];
-    if (prof && evs_.size() >= 9 + 2 * (size_t)D)
+    if (prof && evs_.size() >= 9 + 2 * (size_t)Dk)
     {
         for (int i = 0; i < 8; ++i)
         {
@@ -2013,7 +2386,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         }
         double sum = 0, seam_sum = 0;
         uint32_t launches = 0;
-        for (int d = 0; d < D; ++d)
+        for (int d = 0; d < Dk; ++d)
         {
             float t = 0;
             (void)hipEventElapsedTime(&t, evs_[9 + 2 * d], evs_[10 + 2 * d]);
@@ -2049,7 +2422,11 @@ Status Engine::batch_output(uint32_t T, int k, int kg, uint32_t m)
     QDT_HIP(hipStreamSynchronize(s));
     const uint32_t tmain = std::min(hs[0], T);
     const uint32_t c = T - tmain; // triangles of spare records
-    if (T > b.tris_cap || c > cap_n_) return Status::internal;
+    if (T > b.tris_cap || c > cap_n_)
+    {
+        st_.err_bits = 1u << 20;
+        return Status::internal;
+    }
     k_tri_rank<<<cdiv(std::max<uint32_t>(T, 1), kBlock), kBlock, 0, s>>>(b.tris, T, b.rin, b.trank);
     k_rank_bounds<uint32_t><<<cdiv(K1, kBlock), kBlock, 0, s>>>(b.trank, tmain, kg, b.mstart);
     uint64_t *kA = b.kA, *kB = b.kB;
@@ -2075,7 +2452,11 @@ Status Engine::batch_output(uint32_t T, int k, int kg, uint32_t m)
         at += cnt[r];
     }
     Ht[kg] = at;
-    if (at != T) return Status::internal;
+    if (at != T)
+    {
+        st_.err_bits = 1u << 21;
+        return Status::internal;
+    }
     QDT_HIP(hipMemcpyAsync(b.toff, Ht, 4 * ((size_t)kg + 1), hipMemcpyHostToDevice, s));
     if (T)
         k_batch_tris<<<cdiv(T, kBlock), kBlock, 0, s>>>(b.tris, T, tmain, b.trank, kA, vA, b.mstart, b.sstart,

@@ -96,6 +96,8 @@ public:
     void set_profiling(bool on) { profiling_ = on; }
     // Cuts large merges into seam pieces (default on).
     void set_seams(bool on) { seams_ = on; }
+    // Builds the bottom kd levels in LDS cells (default on).
+    void set_cells(bool on) { cells_ = on; }
     // Device bytes currently held.
     size_t workspace_bytes() const;
 
@@ -118,6 +120,7 @@ private:
     int block_points_ = 0;
     bool profiling_ = false;
     bool seams_ = true;
+    bool cells_ = true;
     std::vector<hipEvent_t> evs_; // profiling events, created on demand
     hipEvent_t prof_event(size_t i);
     DeviceResult res_;

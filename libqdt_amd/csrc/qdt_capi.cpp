@@ -87,6 +87,7 @@ struct qdt_t
     qdt_gpu_stats stats{};
     bool profiling = false;
     bool seams = true;
+    bool cells = true;
     // constraints accepted so far (pairs of representative indices)
     std::set<std::pair<size_t, size_t>> cons;
 
@@ -184,6 +185,7 @@ qdt_status ensure_engine(qdt_t* Q)
     if (!Q->eng) return QDT_ERR_OUT_OF_MEMORY;
     Q->eng->set_profiling(Q->profiling);
     Q->eng->set_seams(Q->seams);
+    Q->eng->set_cells(Q->cells);
     if (!Q->eng->stream())
     {
         delete Q->eng;
@@ -226,6 +228,7 @@ void copy_stats(const qdt::RunStats& s, qdt_gpu_stats& o)
     o.seam_level_ms_sum = s.seam_level_ms_sum;
     o.seam_retries = s.seam_retries;
     for (int i = 0; i < 8; ++i) o.seam_counters[i] = s.seam[i];
+    o.err_bits = s.err_bits;
 }
 
 void record_stats(qdt_t* Q)
@@ -658,6 +661,13 @@ void qdt_gpu_set_seams(qdt_t* ptr, int enable)
     if (ptr->eng) ptr->eng->set_seams(ptr->seams);
 }
 
+void qdt_gpu_set_cells(qdt_t* ptr, int enable)
+{
+    if (!ptr) return;
+    ptr->cells = enable != 0;
+    if (ptr->eng) ptr->eng->set_cells(ptr->cells);
+}
+
 int qdt_gpu_device_count(void)
 {
     int n = 0;
@@ -771,6 +781,7 @@ qdt_status batch_run(qdt_gpu_batch* B, const qdt_vec2d* pts, const size_t* offse
         if (s != qdt::Status::ok)
         {
             reset_batch(B);
+            copy_stats(B->eng->stats(), B->stats); // the failed run's counters and error flags
             return map_status(s);
         }
         copy_stats(B->eng->stats(), B->stats);

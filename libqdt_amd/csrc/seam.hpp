@@ -16,22 +16,22 @@
 //               or of v (in R) forbids the centres on one side of the
 //               circumcentre of (u, v, w).  (u, v) is in every Delaunay
 //               triangulation of L u R iff every neighbour left of u->v and
-//               every neighbour right of it are pairwise strictly separated:
+//               every neighbour right of it are strictly separated:
 //               incircle(u, v, wl, wr) < 0 for all such pairs (u, v, wl
 //               counter-clockwise), and no neighbour is collinear with u, v.
 //               (Vor_L(u) is cut out by u's Delaunay neighbours in L, Vor_R(v)
-//               by v's in R, so these constraints are all there are.)
-//  3. insert    one thread per node: the lower tangent, then every anchor in
-//               order along the chain: the edges of L and of R crossing the
-//               anchor segment are found by a straight walk (dry run first,
-//               any degeneracy drops the anchor), deleted, and the anchor is
-//               spliced into the rings of u and v.
-//  4. segments  one thread per piece, in three phases (pieces k, k+3, ...
-//               together): the rising bubble from anchor k stops when anchor
-//               k+1 is the next cross edge (QuadEdges::bubble).  Pieces of one
-//               phase must not share vertices; each logs the vertices whose
-//               rings it reads or splices, and a check kernel flags any vertex
-//               logged twice.  A flagged run is repeated with unsplit merges.
+//               by v's in R, so these constraints are all there are.)  Only
+//               the extreme neighbour of each side needs testing.
+//  3. insert    the edges of L and of R crossing each anchor are found by a
+//               straight walk (dry run; any degeneracy drops the anchor).
+//               Selected anchors (ordered, vertex-disjoint) are inserted in
+//               parallel by rewriting the ring of every vertex they touch.
+//  4. segments  one thread per piece, in two phases (even pieces, then odd):
+//               the rising bubble from anchor k stops when anchor k+1 is the
+//               next cross edge (QuadEdges::bubble).  A piece claims each
+//               vertex before touching its ring and stops at a vertex another
+//               piece holds; stopped pieces are finished afterwards, one
+//               thread per node.
 #pragma once
 
 #include "quadedge.hpp"
@@ -96,13 +96,19 @@ __device__ int nn_walk(Q& q, int s, double2 p, int budget, int& e_out)
 }
 
 // Exact certificate that (u, v) is an edge of every Delaunay triangulation of
-// L u R (u in L, v in R, eu / ev edges out of u / v).
+// L u R (u in L, v in R, eu / ev edges out of u / v).  The circles through u
+// and v form a one-parameter family; every neighbour w has the member c(w)
+// through it.  The condition "incircle(u, v, wl, wr) < 0 for every left
+// neighbour wl and right neighbour wr" is max c(right) < min c(left), so it
+// suffices to find the extreme neighbour of each side by a streaming scan
+// (the side's point inside the circles of all the others; comparisons are
+// the same exact incircle tests) and test that one pair.  A neighbour
+// collinear with u and v rejects the anchor.
 template <class Q>
 __device__ bool certify(Q& q, int u, int eu, int v, int ev)
 {
-    constexpr int kMax = 48;
-    int left[kMax], right[kMax];
-    int nl = 0, nr = 0;
+    constexpr int kMax = 64; // ring length bound
+    int wl = -1, wr = -1;
     for (int side = 0; side < 2; ++side)
     {
         const int e0 = side ? ev : eu;
@@ -116,22 +122,20 @@ __device__ bool certify(Q& q, int u, int eu, int v, int ev)
                 if (o == 0) return false;
                 if (o > 0)
                 {
-                    if (nl >= kMax) return false;
-                    left[nl++] = w;
+                    // left: keep the one inside the others' circles (u, v, w ccw)
+                    if (wl < 0 || q.incircle_v(u, v, wl, w) > 0) wl = w;
                 }
                 else
                 {
-                    if (nr >= kMax) return false;
-                    right[nr++] = w;
+                    // right: (v, u, w) is ccw
+                    if (wr < 0 || q.incircle_v(v, u, wr, w) > 0) wr = w;
                 }
             }
             e = q.onext(e);
         } while (e != e0 && ++guard < kMax && !q.err);
         if (e != e0) return false;
     }
-    for (int i = 0; i < nl; ++i)
-        for (int j = 0; j < nr; ++j)
-            if (!(q.incircle_v(u, v, left[i], right[j]) < 0)) return false;
+    if (wl >= 0 && wr >= 0 && !(q.incircle_v(u, v, wl, wr) < 0)) return false;
     return !q.err;
 }
 
