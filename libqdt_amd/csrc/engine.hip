@@ -781,7 +781,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
 
 constexpr int kSegMaxK = 256;          // anchors per node at most (a power of two)
 constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
-constexpr int kSegMin = 8192;          // smaller nodes merge in one piece
+constexpr int kSegMin = 4096;          // smaller nodes merge in one piece
 constexpr int kSegBlock = 64;          // pieces per block
 constexpr int kPhases = 2;             // pieces k, k+2, ... run together
 constexpr int kClaimBits = 20;         // piece id bits of a claim (the rest: launch epoch)
@@ -791,7 +791,7 @@ constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit mer
 __host__ __device__ inline int seam_anchors(double s)
 {
     if (s < kSegMin) return 0;
-    int k = (int)(std::sqrt(s) / 8.0);
+    int k = (int)(std::sqrt(s) / 5.0);
     return k < 2 ? 2 : (k > kSegMaxK ? kSegMaxK : k);
 }
 
@@ -1507,35 +1507,39 @@ struct FaceCount
     __device__ uint32_t operator()(uint32_t k) const { return __popc(face[k] & 3u); }
 };
 
-struct FaceWrite
+// the scan's output: exclusive triangle count of every record, packed above
+// its two face bits
+struct FacePrefix
 {
-    const uint32_t* face;
-    const int* NX;
-    const int* OR;
-    const uint32_t* ID0;
-    uint32_t* tris;
-    uint32_t m;
-    int lim_e;
-    uint32_t split;  // first spare record: where the set-grouped triangles end
-    uint32_t* tmain; // receives the count of triangles before `split`
-    __device__ void operator()(uint32_t k, uint32_t pre) const
-    {
-        if (k == split) *tmain = pre;
-        const uint32_t fl = face[k] & 3u;
-        if (!fl) return;
-        for (int r = 0; r < 4; r += 2)
-        {
-            if (!(fl & (1u << (r >> 1)))) continue;
-            const int e = 4 * (int)k + r;
-            int f, g, a, b, c;
-            if (!face_of(NX, OR, e, lim_e, (int)m, f, g, a, b, c)) a = b = c = 0;
-            tris[3 * pre] = ID0[a];
-            tris[3 * pre + 1] = ID0[b];
-            tris[3 * pre + 2] = ID0[c];
-            ++pre;
-        }
-    }
+    uint32_t* face;
+    __device__ void operator()(uint32_t k, uint32_t pre) const { face[k] = (pre << 2) | (face[k] & 3u); }
 };
+
+// one thread per record: writes its faces at their prefix (as original,
+// lowest-duplicate indices)
+__global__ __launch_bounds__(kBlock) void k_face_write(const uint32_t* face, const int* NX, const int* OR,
+                                                       const uint32_t* ID0, uint32_t nrec, uint32_t m, int lim_e,
+                                                       uint32_t split, uint32_t* tmain, uint32_t* tris)
+{
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nrec) return;
+    const uint32_t f = face[k];
+    uint32_t pre = f >> 2;
+    if (k == split) *tmain = pre; // triangles of the records below the spare region
+    const uint32_t fl = f & 3u;
+    if (!fl) return;
+    for (int r = 0; r < 4; r += 2)
+    {
+        if (!(fl & (1u << (r >> 1)))) continue;
+        const int e = 4 * (int)k + r;
+        int g, h, a, b, c;
+        if (!face_of(NX, OR, e, lim_e, (int)m, g, h, a, b, c)) a = b = c = 0;
+        tris[3 * pre] = ID0[a];
+        tris[3 * pre + 1] = ID0[b];
+        tris[3 * pre + 2] = ID0[c];
+        ++pre;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // host helpers
@@ -2340,10 +2344,10 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                                                           (uint32_t)spare_cap, b.face, b.queue, b.qcap, b.counters,
                                                           b.err);
     k_extract_exact<<<256, kBlock, 0, s>>>(b.NX, b.OR, b.P0, m, lim_e, b.queue, b.counters, b.qcap, b.face);
-    scan_u32(FaceCount{ b.face }, nrec,
-             FaceWrite{ b.face, b.NX, b.OR, b.ID0, b.tris, m, lim_e, multi ? 3 * m : 0xffffffffu,
-                        multi ? b.tmain : b.partials },
-             b.partials, s);
+    scan_u32(FaceCount{ b.face }, nrec, FacePrefix{ b.face }, b.partials, s);
+    k_face_write<<<cdiv(nrec, kBlock), kBlock, 0, s>>>(b.face, b.NX, b.OR, b.ID0, nrec, m, lim_e,
+                                                        multi ? 3 * m : 0xffffffffu, multi ? b.tmain : b.partials,
+                                                        b.tris);
     mark(8);
     QDT_HIP(hipEventRecord(ev1_, s));
     const uint32_t nbx = cdiv(nrec, kScanTile);

@@ -110,19 +110,33 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* partials, uint
     if (threadIdx.x == 0) partials[nb] = carry;
 }
 
+// LDS slot of tile item p: one pad word per 32 keeps the blocked reads
+// (thread t reads 16 consecutive items) free of bank conflicts
+__device__ __forceinline__ uint32_t scan_slot(uint32_t p) { return p + (p >> 5); }
+
 template <class In, class Out>
 __global__ __launch_bounds__(kBlock) void k_scan_down(In in, uint32_t n, const uint32_t* partials, Out out)
 {
-    // Each thread owns kScanItems consecutive items of the tile (blocked
-    // layout) so that one thread-level prefix suffices.
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    // Items are read and written striped (consecutive lanes, consecutive
+    // items: coalesced, and the functors' own gathers spread over the
+    // wave), and scanned blocked through LDS (each thread owns kScanItems
+    // consecutive items, so one thread-level prefix suffices).
+    __shared__ uint32_t tile[kScanTile + kScanTile / 32];
+    const uint32_t base = blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        const uint32_t p = k * kBlock + threadIdx.x;
+        const uint32_t i = base + p;
+        tile[scan_slot(p)] = i < n ? in(i) : 0;
+    }
+    __syncthreads();
     uint32_t v[kScanItems];
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
     {
-        const uint32_t i = base + k;
-        v[k] = i < n ? in(i) : 0;
+        v[k] = tile[scan_slot(threadIdx.x * kScanItems + k)];
         s += v[k];
     }
     uint32_t total;
@@ -130,9 +144,16 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(In in, uint32_t n, const u
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
     {
-        const uint32_t i = base + k;
-        if (i < n) out(i, pre);
+        tile[scan_slot(threadIdx.x * kScanItems + k)] = pre;
         pre += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        const uint32_t p = k * kBlock + threadIdx.x;
+        const uint32_t i = base + p;
+        if (i < n) out(i, tile[scan_slot(p)]);
     }
 }
 
