@@ -6,13 +6,16 @@ merges + triangle extraction, results left in HBM.  `value` is points per
 second over the K timed steps.
 
 Extra fields:
-  roofline      the dominant kernel, k_merge (the Guibas-Stolfi merge of every
-                node of one kd level, one thread per node; one launch per level
-                whose nodes are merged in one piece): algorithmic bytes per
-                launch = 48 B (one 24 B quad-edge record written once and read
-                once) per edge the launch creates, counted on the device, over
-                the launch's average duration, measured with HIP events on the
-                engine's stream around every k_merge launch of the timed steps.
+  roofline      the dominant kernel, k_cell (one wavefront per LDS cell builds
+                the cell's whole kd subtree: partition, leaves, merges; one
+                launch per step): algorithmic bytes per launch = 100 B per
+                distinct point (reads: 16 B coordinates, 4 + 4 B of the two
+                frame orders; writes: three 24 B quad-edge records and a 4 B
+                vertex-edge entry) + 16 B per cell (its result), over the
+                launch's duration measured with HIP events on the engine's
+                stream around the k_cell launch of every timed step.  The
+                k_merge launches (levels above the cells merged with one
+                thread per node) are reported beside it.
   cpu_baseline  the reference's threaded qdt_execute_parallel, compiled from
                 the reference sources into oracle/_ref/, on the same 10M input,
                 on this host's cores (median of --cpu-reps runs).
@@ -49,10 +52,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
 def traffic_per_launch():
-    """HBM bytes per k_merge launch from the PMC runs summarised in
-    profiles/k_merge_10M_uniform.md (FETCH_SIZE and WRITE_SIZE, each scaled
+    """HBM bytes per k_cell launch from the PMC runs summarised in
+    profiles/k_cell_10M_uniform.md (FETCH_SIZE and WRITE_SIZE, each scaled
     by a calibration copy); None when that summary is absent."""
-    p = os.path.join(ROOT, "profiles", "k_merge_10M_uniform.md")
+    p = os.path.join(ROOT, "profiles", "k_cell_10M_uniform.md")
     try:
         for line in open(p):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -251,7 +254,7 @@ def main():
         assert st == Q.QDT_OK, Q.status_string(st)
     torch.cuda.synchronize()
 
-    step_ms, merge_ms, merge_launches, merge_edges, stats = [], [], [], [], None
+    step_ms, merge_ms, merge_launches, merge_edges, cell_ms, stats = [], [], [], [], [], None
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
@@ -262,6 +265,7 @@ def main():
         merge_ms.append(stats["merge_launch_ms_sum"])
         merge_launches.append(stats["merge_launches"])
         merge_edges.append(stats["merge_edges"])
+        cell_ms.append(stats["stage_ms"]["leaves"])
     torch.cuda.synchronize()
     total_s = time.perf_counter() - t0
     ms_per_step = total_s * 1e3 / args.steps
@@ -270,12 +274,17 @@ def main():
     T = int(stats["triangles"])
     m = int(stats["unique_points"])
     E = 3 * m - 3 - (2 * m - 2 - T)  # Euler: E = 3m - 3 - b with b = 2m - 2 - T
-    # k_merge roofline: each edge a launch creates is a 24 B record (four
-    # onext + two origins) written once and read once -> 48 B per edge.
+    # k_cell roofline (module docstring): 100 B per distinct point + 16 B per cell
+    cells = int(stats["cells"])
+    cell_bytes = 100.0 * m + 16.0 * cells
+    cell_avg_ms = statistics.mean(cell_ms)
+    achieved = cell_bytes / (cell_avg_ms * 1e-3) / 1e9
+    # k_merge: each edge a launch creates is a 24 B record (four onext + two
+    # origins) written once and read once -> 48 B per edge
     launches = merge_launches[-1] or 1
     avg_launch_ms = statistics.mean(merge_ms) / launches
     bytes_per_launch = 48.0 * statistics.mean(merge_edges) / launches
-    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    merge_achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
 
     line = {
         "metric": "Mpts/s triangulated (10M uniform f64); batched sets/s; serve p50/p99 ms",
@@ -305,17 +314,20 @@ def main():
             },
         },
         "roofline": {
-            "kernel": "k_merge (one launch per kd level merged in one piece per node)",
+            "kernel": "k_cell (LDS cells, one launch per step)",
             "bound": "hbm",
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_per_launch(),
-            "avg_launch_ms": round(avg_launch_ms, 4),
-            "launches_per_step": launches,
-            "edges_per_step": int(merge_edges[-1]),
-            "bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": round(cell_avg_ms, 4),
+            "bytes_per_launch": cell_bytes,
+            "cells": cells,
+            "cell_level": int(stats["cell_level"]),
+            "k_merge": {"launches_per_step": launches, "avg_launch_ms": round(avg_launch_ms, 4),
+                        "edges_per_step": int(merge_edges[-1]), "bytes_per_launch": bytes_per_launch,
+                        "achieved_gbs": round(merge_achieved, 3)},
             "seam_levels_ms": round(stats["seam_level_ms_sum"], 3),
             "pipeline_device_ms": round(stats["device_ms"], 3),
             "pipeline_alg_bytes": 36 * n + 40 * m + 48 * E + 12 * T,

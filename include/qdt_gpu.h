@@ -42,6 +42,8 @@ typedef struct qdt_gpu_stats
                                      resumed, pieces stopped by a claimed vertex, anchors with a
                                      clean dry run, anchors selected, vertex rings rewritten */
     uint64_t err_bits;            /* device error flags of the last run (0 when it succeeded) */
+    uint64_t cells;               /* LDS cells (one wavefront builds each cell's kd subtree) */
+    uint64_t cell_level;          /* kd level of the cells */
 } qdt_gpu_stats;
 
 /* Like qdt_execute(), but d_pts points to n points in device memory.  The

@@ -496,6 +496,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
 constexpr int kCellMax = 128;             // points per cell at most
 constexpr int kCellNodes = 2 * kCellMax;  // subtree node slots (levels of 1, 2, 4, ... nodes)
 constexpr int kCellLevels = 12;
+constexpr int kCellPer = (kCellMax + 63) / 64; // positions per lane in the cell's scans
 
 using LQE = QuadEdges<int*, const double2*>;
 
@@ -638,21 +639,32 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
         }
         __syncthreads();
         // stable partition of the other array: exclusive count of lefts
-        // (two positions per lane, one wave-wide scan)
+        // (kCellPer consecutive positions per lane, one wave-wide scan)
         {
-            const int i0 = 2 * lane, i1 = 2 * lane + 1;
-            const int f0 = i0 < s ? (right[Ao[i0]] ? 0 : 1) : 0;
-            const int f1 = i1 < s ? (right[Ao[i1]] ? 0 : 1) : 0;
-            int x = f0 + f1;
+            int f[kCellPer];
+            int x = 0;
+#pragma unroll
+            for (int r = 0; r < kCellPer; ++r)
+            {
+                const int i = kCellPer * lane + r;
+                f[r] = i < s ? (right[Ao[i]] ? 0 : 1) : 0;
+                x += f[r];
+            }
+            const int own = x;
 #pragma unroll
             for (int dd = 1; dd < 64; dd <<= 1)
             {
                 const int y = __shfl_up(x, dd, 64);
                 if (lane >= dd) x += y;
             }
-            const int ex = x - f0 - f1;
-            if (i0 < s) Z[i0] = ex;
-            if (i1 < s) Z[i1] = ex + f0;
+            int ex = x - own;
+#pragma unroll
+            for (int r = 0; r < kCellPer; ++r)
+            {
+                const int i = kCellPer * lane + r;
+                if (i < s) Z[i] = ex;
+                ex += f[r];
+            }
             if (lane == 63) Z[s] = x;
         }
         __syncthreads();
@@ -817,9 +829,12 @@ __device__ __forceinline__ bool less_frame_pt(double2 a, double2 b, int frame)
     return a.y < b.y || (a.y == b.y && a.x > b.x);
 }
 
-// a vertex of the kd leaf whose cell contains p, below node (dd, jj)
-__device__ int kd_descend(const int* off, const int* splitv, const double2* PT, const uint32_t* B0, int D, int dd,
-                          uint32_t jj, double2 p)
+// A vertex of the deepest built kd node whose region contains p, below node
+// (dd, jj): the node's point closest to the cut of frame `axis` on the given
+// side (0: the lower child, which faces the cut with its largest points in
+// that frame; 1: the upper child), so that the walks toward p start close.
+__device__ int kd_descend(const int* off, const int* splitv, const double2* PT, const uint32_t* B0,
+                          const uint32_t* B1, int D, int dd, uint32_t jj, double2 p, int axis, int side)
 {
     while (dd < D)
     {
@@ -831,8 +846,10 @@ __device__ int kd_descend(const int* off, const int* splitv, const double2* PT, 
         ++dd;
     }
     const int* od = off + off_base(off, dd);
-    const int o = od[jj];
-    return od[jj + 1] > o ? (int)B0[o] : -1;
+    const int o = od[jj], e = od[jj + 1];
+    if (e <= o) return -1;
+    const uint32_t* B = axis ? B1 : B0;
+    return (int)B[side ? o : e - 1];
 }
 
 constexpr int kCross = 16;                     // crossing edges per side of an anchor at most
@@ -840,7 +857,8 @@ constexpr int kRec = 6 + 2 * kCross;           // ints per anchor record: u v xu
 
 __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, const double2* PT, int* VE,
                                                             const int* off, const int* splitv, const int* qv,
-                                                            const uint32_t* B0, int D, int d, int K, uint32_t m,
+                                                            const uint32_t* B0, const uint32_t* B1, int D, int d,
+                                                            int K, uint32_t m,
                                                             int lim_e, int* arec, double4* acoord, double* akey,
                                                             int* astate, unsigned long long* counters)
 {
@@ -861,8 +879,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
             const double c = frame_x(PT[splitv[ni]], axis);
             const double Y = frame_y(PT[qv[ni * kSegMaxK + k]], axis);
             double2 p = from_frame(c, Y, axis);
-            const int sL = kd_descend(off, splitv, PT, B0, D, d + 1, 2 * j, p);
-            const int sR = kd_descend(off, splitv, PT, B0, D, d + 1, 2 * j + 1, p);
+            const int sL = kd_descend(off, splitv, PT, B0, B1, D, d + 1, 2 * j, p, axis, 0);
+            const int sR = kd_descend(off, splitv, PT, B0, B1, D, d + 1, 2 * j + 1, p, axis, 1);
             int eu = -1, ev = -1;
             int u = sL >= 0 ? nn_walk(q, sL, p, 512, eu) : -1;
             int v = sR >= 0 ? nn_walk(q, sR, p, 512, ev) : -1;
@@ -2273,6 +2291,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         for (int d = dc + 1; d <= D; ++d) ncell += (uint32_t)(P.count[d] - 2 * P.count[d - 1]);
         st_.leaf_levels = (uint32_t)(D - dc);
         st_.block_level = (uint32_t)dc;
+        st_.cells = ncell;
+        st_.cell_points = m_forest;
         if (ncell)
             k_cell<<<ncell, 64, 0, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, dc, D, m, lim_e, b.res, b.err,
                                         b.counters);
@@ -2301,7 +2321,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         {
             const int total = (int)(cnt * (uint32_t)K);
             k_seg_anchor<<<cdiv((size_t)total, kMergeBlock), kMergeBlock, 0, s>>>(
-                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, Dk, d, K, m, lim_e, b.arec, b.acoord, b.akey,
+                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, Dk, d, K, m, lim_e, b.arec, b.acoord, b.akey,
                 b.astate, b.counters);
             k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
                                                                           b.spare, spare_base, spare_cap, b.res,

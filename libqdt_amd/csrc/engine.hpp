@@ -23,8 +23,10 @@ struct RunStats
     uint32_t unique;                                          // distinct points
     uint32_t triangles;
     uint32_t levels;      // kd levels with merges
-    uint32_t leaf_levels; // levels handled inside LDS leaf blocks
-    uint32_t block_level; // first level of the LDS blocks (0 = whole set)
+    uint32_t leaf_levels; // kd levels built inside LDS cells
+    uint32_t block_level; // level of the LDS cells (0 = a whole set per cell)
+    uint64_t cells;       // LDS cells of the run
+    uint64_t cell_points; // distinct points inside them
     double device_ms;     // first kernel start .. last kernel end
     // with profiling on: time per stage (ms) and the merge launches
     double stage_ms[8];   // keys+sort y, sort x, dedupe, frame-1 sort, kd levels, leaves, merges, extract

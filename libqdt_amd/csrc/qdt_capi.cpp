@@ -229,6 +229,8 @@ void copy_stats(const qdt::RunStats& s, qdt_gpu_stats& o)
     o.seam_retries = s.seam_retries;
     for (int i = 0; i < 8; ++i) o.seam_counters[i] = s.seam[i];
     o.err_bits = s.err_bits;
+    o.cells = s.cells;
+    o.cell_level = s.block_level;
 }
 
 void record_stats(qdt_t* Q)
