@@ -498,7 +498,7 @@ constexpr int kCellNodes = 2 * kCellMax;  // subtree node slots (levels of 1, 2,
 constexpr int kCellLevels = 12;
 constexpr int kCellPer = (kCellMax + 63) / 64; // positions per lane in the cell's scans
 
-using LQE = QuadEdges<int*, const double2*>;
+using LQE = QuadEdges<short*, const double2*>; // cell records: local ids fit 16 bits
 
 // cell c -> (level, node index): the nodes of level dc, then the roots of
 // the sets rooted below dc
@@ -536,10 +536,11 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
     __shared__ int gid[kCellMax];
     __shared__ int A0[kCellMax], A1[kCellMax], T[kCellMax], Z[kCellMax + 1];
     __shared__ uint8_t right[kCellMax];
-    __shared__ int LNX[4 * 3 * kCellMax], LOR[2 * 3 * kCellMax], LVE[kCellMax];
+    __shared__ short LNX[4 * 3 * kCellMax], LOR[2 * 3 * kCellMax];
+    __shared__ int LVE[kCellMax];
     __shared__ int loff[kCellNodes + kCellLevels]; // sub-level node offsets
     __shared__ int lbase[kCellLevels + 1];        // first offset of each sub-level in loff
-    __shared__ int4 lres[kCellNodes];
+    __shared__ short4 lres[kCellNodes];
     __shared__ int nlev, bad;
 
     const int lane = threadIdx.x;
@@ -731,7 +732,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
             Pool pool;
             q.leaf(v, sz, 3 * a, first, second, pool);
             // node slot of (db, jb): sub-levels are numbered 2^l - 1 + index
-            lres[(1 << db) - 1 + jb] = make_int4(first, second, pool.head, pool.tail);
+            lres[(1 << db) - 1 + jb] = make_short4((short)first, (short)second, (short)pool.head, (short)pool.tail);
         }
     }
     __syncthreads();
@@ -744,15 +745,15 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
         {
             const int sz = ol[jl + 1] - ol[jl];
             if (sz < 4) continue;
-            const int4 Lr = lres[(2 << l) - 1 + 2 * jl];
-            const int4 Rr = lres[(2 << l) - 1 + 2 * jl + 1];
+            const short4 Lr = lres[(2 << l) - 1 + 2 * jl];
+            const short4 Rr = lres[(2 << l) - 1 + 2 * jl + 1];
             Pool pool = q.cat(Pool{ Lr.z, Lr.w }, Pool{ Rr.z, Rr.w });
             const int abs_d = dl + l;
             const int axis = abs_d & 1;
             const int out_axis = abs_d > 0 ? ((abs_d - 1) & 1) : 0;
             int first, second;
             q.merge_node(Lr.x, Lr.y, Rr.x, Rr.y, axis, out_axis, sz, first, second, pool);
-            lres[(1 << l) - 1 + jl] = make_int4(first, second, pool.head, pool.tail);
+            lres[(1 << l) - 1 + jl] = make_short4((short)first, (short)second, (short)pool.head, (short)pool.tail);
         }
         __syncthreads();
     }
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
     for (int i = lane; i < s; i += 64) VE[gid[i]] = LVE[i] >= 0 ? E0 + LVE[i] : -1;
     if (lane == 0)
     {
-        const int4 r = lres[0];
+        const short4 r = lres[0];
         res[node_index(off, tdl, tj)] = make_int4(r.x >= 0 ? E0 + r.x : -1, r.y >= 0 ? E0 + r.y : -1,
                                                 r.z >= 0 ? R0 + r.z : -1, r.w >= 0 ? R0 + r.w : -1);
         if (bad) q.err = q.err ? q.err : 1;
@@ -2247,15 +2248,12 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     };
     // cells: the first level whose nodes all fit in LDS builds everything
     // below it in one kernel (and so do the roots of sets rooted deeper)
-    int dc = D;
+    // (the level from which every node, the roots below included, fits a cell)
+    int dc = D + 1;
     if (cells_)
-        for (int d = 0; d <= D; ++d)
-            if (P.smax[d] <= kCellMax)
-            {
-                dc = d;
-                break;
-            }
-    const bool use_cells = cells_ && P.smax[dc] <= kCellMax;
+        while (dc > 0 && P.smax[dc - 1] <= kCellMax) --dc;
+    const bool use_cells = cells_ && dc <= D;
+    if (!use_cells) dc = D;
     const int Dk = use_cells ? dc : D; // levels partitioned by the level kernels
     roots_at(0, nodeA);
     for (int d = 0; d < Dk; ++d)
