@@ -794,7 +794,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
 
 constexpr int kSegMaxK = 256;          // anchors per node at most (a power of two)
 constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
-constexpr int kSegMin = 4096;          // smaller nodes merge in one piece
+constexpr int kSegMin = 2048;          // smaller nodes merge in one piece
 constexpr int kSegBlock = 64;          // pieces per block
 constexpr int kPhases = 2;             // pieces k, k+2, ... run together
 constexpr int kClaimBits = 20;         // piece id bits of a claim (the rest: launch epoch)
@@ -1288,26 +1288,32 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_pools(int* NX, int* OR, int
 }
 
 // cut list of every node: the tangent, then the inserted anchors in order
-__global__ __launch_bounds__(kMergeBlock) void k_seg_compact(int d, const int* selk, const int* nsel,
-                                                             const int* astate, const int* aedge, int* segstart,
-                                                             int* nseg, const int* off)
+// (one workgroup per node, a block scan compacts the inserted ones)
+__global__ __launch_bounds__(kSegMaxK) void k_seg_compact(int d, const int* selk, const int* nsel,
+                                                         const int* astate, const int* aedge, int* segstart,
+                                                         int* nseg, const int* off)
 {
-    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    const uint32_t j = blockIdx.x;
+    const int k = threadIdx.x;
     if (j >= lcount(off, d)) return;
     const int* od = off + off_base(off, d);
     const int s = od[j + 1] - od[j];
-    int cnt = 0;
-    if (s >= 4)
+    if (s < 4)
     {
-        cnt = 1;
-        const int ns = nsel[j];
-        for (int i = 0; i < ns; ++i)
-        {
-            const int t = selk[(size_t)j * kSegSlots + i];
-            if (astate[t] == 2 && aedge[t] >= 0) segstart[(size_t)j * kSegSlots + cnt++] = aedge[t];
-        }
+        if (k == 0) nseg[j] = 0;
+        return;
     }
-    nseg[j] = cnt;
+    const int ns = nsel[j];
+    int v = -1;
+    if (k < ns)
+    {
+        const int t = selk[(size_t)j * kSegSlots + k];
+        if (astate[t] == 2 && aedge[t] >= 0) v = aedge[t];
+    }
+    uint32_t total;
+    const uint32_t pos = block_exclusive_scan(v >= 0 ? 1u : 0u, total);
+    if (v >= 0) segstart[(size_t)j * kSegSlots + 1 + pos] = v;
+    if (k == 0) nseg[j] = 1 + (int)total;
 }
 
 // One thread per piece of one phase: the rising bubble from cut i to cut
@@ -1316,25 +1322,28 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_compact(int d, const int* s
 __global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const double2* PT, int* VE, const int* off,
                                                       int d, uint32_t m, int lim_e, unsigned* spare, int spare_base,
                                                       int spare_cap, const int* segstart, const int* nseg,
-                                                      int parity, int epoch, int* claim, int2* segpool,
+                                                      int parity, int hp, int epoch, int* claim, int2* segpool,
                                                       int* segresume, int* err, unsigned long long* counters)
 {
-    const uint32_t t = blockIdx.x * kSegBlock + threadIdx.x;
+    // thread (node j, h) runs piece i = kPhases * h + parity; hp pieces per
+    // node and phase at most; the claim id of the piece is the thread index
+    const uint32_t tid = blockIdx.x * kSegBlock + threadIdx.x;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
     q.spare = spare;
     q.spare_base = spare_base;
     q.spare_cap = spare_cap;
-    const uint32_t j = t / kSegSlots;
-    const int i = (int)(t % kSegSlots);
-    if (j < lcount(off, d) && i % kPhases == parity)
+    const uint32_t j = tid / hp;
+    const int i = kPhases * (int)(tid % hp) + parity;
+    const size_t t = (size_t)j * kSegSlots + i;
+    if (j < lcount(off, d))
     {
         const int ns = nseg[j];
         if (i < ns)
         {
             q.claim = claim;
             q.claim_base = epoch << kClaimBits;
-            q.claim_me = q.claim_base | (int)t;
+            q.claim_me = q.claim_base | (int)tid;
             const int* od = off + off_base(off, d);
             const int s = od[j + 1] - od[j];
             const int* st = segstart + (size_t)j * kSegSlots;
@@ -1361,81 +1370,127 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const d
     if (q.err) atomicOr(err, q.err >= 4 ? kSeamErr : (2 << q.err));
 }
 
-// one thread per node: the stopped pieces, bottom to top, with no other
-// piece of the node running
-__global__ __launch_bounds__(kMergeBlock) void k_seg_resume(int* NX, int* OR, const double2* PT, int* VE,
-                                                            const int* off, int d, uint32_t m, int lim_e,
-                                                            unsigned* spare, int spare_base, int spare_cap,
-                                                            const int* segstart, const int* nseg, int2* segpool,
-                                                            const int* segresume, int* err,
-                                                            unsigned long long* counters)
+// one workgroup per node: its threads find the stopped pieces, one thread
+// finishes them bottom to top with no other piece of the node running
+__global__ __launch_bounds__(kSegMaxK) void k_seg_resume(int* NX, int* OR, const double2* PT, int* VE,
+                                                        const int* off, int d, uint32_t m, int lim_e,
+                                                        unsigned* spare, int spare_base, int spare_cap,
+                                                        const int* segstart, const int* nseg, int2* segpool,
+                                                        const int* segresume, int* err,
+                                                        unsigned long long* counters)
 {
-    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    __shared__ int stopped[kSegSlots];
+    __shared__ int nstop;
+    const uint32_t j = blockIdx.x;
+    const int k = threadIdx.x;
+    if (j >= lcount(off, d)) return;
+    const int ns = nseg[j];
+    if (k == 0) nstop = 0;
+    __syncthreads();
+    for (int i = k; i < ns; i += kSegMaxK)
+        if (segresume[(size_t)j * kSegSlots + i] >= 0) stopped[atomicAdd(&nstop, 1)] = i;
+    __syncthreads();
+    if (k != 0 || nstop == 0) return;
+    // bottom to top
+    for (int a = 1; a < nstop; ++a)
+    {
+        const int x = stopped[a];
+        int b = a;
+        while (b > 0 && stopped[b - 1] > x)
+        {
+            stopped[b] = stopped[b - 1];
+            --b;
+        }
+        stopped[b] = x;
+    }
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
     q.spare = spare;
     q.spare_base = spare_base;
     q.spare_cap = spare_cap;
-    if (j < lcount(off, d))
+    const int* od = off + off_base(off, d);
+    const int s = od[j + 1] - od[j];
+    int budget = 16 * s + 64;
+    for (int a = 0; a < nstop && !q.err; ++a)
     {
-        const int ns = nseg[j];
-        const int* od = off + off_base(off, d);
-        const int s = od[j + 1] - od[j];
-        int budget = 16 * s + 64;
-        for (int i = 0; i < ns && !q.err; ++i)
-        {
-            const size_t t = (size_t)j * kSegSlots + i;
-            int basel = segresume[t];
-            if (basel < 0) continue;
-            const int stop = i + 1 < ns ? segstart[(size_t)j * kSegSlots + i + 1] : -1;
-            const int2 sp = segpool[t];
-            Pool pool = { sp.x, sp.y };
-            const int r = q.bubble(basel, stop, pool, budget);
-            if (stop >= 0 && r != 1 && !q.err) q.err = 4;
-            segpool[t] = make_int2(pool.head, pool.tail);
-            atomicAdd(&counters[8], 1ull);
-        }
+        const int i = stopped[a];
+        const size_t t = (size_t)j * kSegSlots + i;
+        int basel = segresume[t];
+        const int stop = i + 1 < ns ? segstart[(size_t)j * kSegSlots + i + 1] : -1;
+        const int2 sp = segpool[t];
+        Pool pool = { sp.x, sp.y };
+        const int r = q.bubble(basel, stop, pool, budget);
+        if (stop >= 0 && r != 1 && !q.err) q.err = 4;
+        segpool[t] = make_int2(pool.head, pool.tail);
     }
-    flush_counts(counters, q.pc);
+    atomicAdd(&counters[8], (unsigned long long)nstop);
+    atomicAdd(&counters[0], (unsigned long long)q.pc.orient);
+    atomicAdd(&counters[1], (unsigned long long)q.pc.orient_exact);
+    atomicAdd(&counters[2], (unsigned long long)q.pc.incircle);
+    atomicAdd(&counters[3], (unsigned long long)q.pc.incircle_exact);
     if (q.err) atomicOr(err, q.err >= 4 ? kSeamErr : (2 << q.err));
 }
 
-__global__ __launch_bounds__(kMergeBlock) void k_seg_final(int* NX, int* OR, const double2* PT, int* VE,
-                                                           const int* off, int d, int K, uint32_t m, int lim_e,
-                                                           const int* nseg, const int4* nstate, const int2* segpool,
-                                                           const int* astate, const int2* apool, int4* res, int* err,
-                                                           unsigned long long* counters)
+// one workgroup per node: the free lists of the node, its pieces and its
+// anchors are joined in parallel (compacted by a block scan, each list's
+// tail linked to the next list's head); then the hull re-walk
+__global__ __launch_bounds__(kSegMaxK) void k_seg_final(int* NX, int* OR, const double2* PT, int* VE,
+                                                       const int* off, int d, int K, uint32_t m, int lim_e,
+                                                       const int* nseg, const int4* nstate, const int2* segpool,
+                                                       const int* astate, const int2* apool, int4* res, int* err,
+                                                       unsigned long long* counters)
 {
-    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    constexpr int kPer = 3; // lists per thread: 1 + kSegSlots + kSegMaxK <= 3 * kSegMaxK
+    __shared__ int2 lists[kPer * kSegMaxK];
+    const uint32_t j = blockIdx.x;
+    const int k = threadIdx.x;
+    if (j >= lcount(off, d)) return;
+    const int* od = off + off_base(off, d);
+    const int s = od[j + 1] - od[j];
+    if (s < 4) return;
+    const int4 ns = nstate[j];
+    const int cnt = nseg[j];
+    int2 mine[kPer];
+    uint32_t c = 0;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r)
+    {
+        const int x = kPer * k + r;
+        int2 l = make_int2(-1, -1);
+        if (x == 0)
+            l = make_int2(ns.z, ns.w);
+        else if (x <= cnt)
+            l = segpool[(size_t)j * kSegSlots + (x - 1)];
+        else if (x <= cnt + K)
+        {
+            const size_t t = (size_t)j * K + (x - cnt - 1);
+            if (astate[t] == 2) l = apool[t];
+        }
+        mine[r] = l;
+        c += l.x >= 0 ? 1u : 0u;
+    }
+    uint32_t total;
+    uint32_t pos = block_exclusive_scan(c, total);
+#pragma unroll
+    for (int r = 0; r < kPer; ++r)
+        if (mine[r].x >= 0) lists[pos++] = mine[r];
+    __syncthreads();
+    const int lim_r = lim_e >> 2;
+    for (uint32_t x = k; x + 1 < total; x += kSegMaxK)
+    {
+        const int tail = lists[x].y;
+        if ((unsigned)tail < (unsigned)lim_r)
+            NX[4 * tail] = lists[x + 1].x;
+        else
+            atomicOr(err, 2 << 1);
+    }
+    if (k != 0) return;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
-    if (j < lcount(off, d))
-    {
-        const int* od = off + off_base(off, d);
-        const int s = od[j + 1] - od[j];
-        if (s >= 4)
-        {
-            const int4 ns = nstate[j];
-            Pool pool = { ns.z, ns.w };
-            const int cnt = nseg[j];
-            for (int i = 0; i < cnt; ++i)
-            {
-                const int2 sp = segpool[(size_t)j * kSegSlots + i];
-                pool = q.cat(pool, Pool{ sp.x, sp.y });
-            }
-            for (int k = 0; k < K; ++k)
-            {
-                const size_t t = (size_t)j * K + k;
-                if (astate[t] != 2) continue;
-                const int2 ap = apool[t];
-                pool = q.cat(pool, Pool{ ap.x, ap.y });
-            }
-            int first, second;
-            q.rewalk(ns.x, ns.y, d & 1, d > 0 ? ((d - 1) & 1) : 0, s, first, second);
-            res[node_index(off, d, j)] = make_int4(first, second, pool.head, pool.tail);
-        }
-    }
-    flush_counts(counters, q.pc);
+    const int head = total ? lists[0].x : -1, tail = total ? lists[total - 1].y : -1;
+    int first, second;
+    q.rewalk(ns.x, ns.y, d & 1, d > 0 ? ((d - 1) & 1) : 0, s, first, second);
+    res[node_index(off, d, j)] = make_int4(first, second, head, tail);
     if (q.err) atomicOr(err, 2 << q.err);
 }
 
@@ -1693,8 +1748,10 @@ void make_plan(Engine::Plan& P, const std::vector<uint32_t>& sizes, const std::v
     P.qv_nodes = 1;
     for (int d = 0; d < D; ++d)
     {
-        const bool fits = (size_t)P.count[d] * kSegSlots < ((size_t)1 << kClaimBits);
-        P.K[d] = seams && fits ? seam_anchors(P.smax[d]) : 0;
+        // the pieces of one phase must have distinct claim ids
+        const int K = seams ? seam_anchors(P.smax[d]) : 0;
+        const bool fits = (size_t)P.count[d] * (size_t)((K + kPhases) / kPhases) < ((size_t)1 << kClaimBits);
+        P.K[d] = fits ? K : 0;
         if (!P.K[d]) continue;
         P.seg_nodes = std::max(P.seg_nodes, (size_t)P.count[d]);
         P.qv_nodes = std::max(P.qv_nodes, (size_t)P.nodeb[d] + (size_t)P.count[d]);
@@ -2338,17 +2395,18 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                 b.err, b.counters);
             k_seg_pools<<<ga, kMergeBlock, 0, s>>>(b.NX, b.OR, total, b.arec, b.astate, b.aedge, lim_e, (int*)b.face,
                                                    b.apool);
-            k_seg_compact<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge,
+            k_seg_compact<<<cnt, kSegMaxK, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge,
                                                                           b.segstart, b.nseg, b.off);
             for (int parity = 0; parity < kPhases; ++parity)
-                k_seg_run<<<cdiv((size_t)cnt * kSegSlots, kSegBlock), kSegBlock, 0, s>>>(
+                k_seg_run<<<cdiv((size_t)cnt * ((K + kPhases) / kPhases), kSegBlock), kSegBlock, 0, s>>>(
                     b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.spare, spare_base, spare_cap, b.segstart, b.nseg,
-                    parity, epoch++, b.claim, b.segpool, b.segresume, b.err, b.counters);
-            k_seg_resume<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
+                    parity, (K + kPhases) / kPhases, epoch++, b.claim, b.segpool, b.segresume, b.err,
+                    b.counters);
+            k_seg_resume<<<cnt, kSegMaxK, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
                                                                          b.spare, spare_base, spare_cap, b.segstart,
                                                                          b.nseg, b.segpool, b.segresume, b.err,
                                                                          b.counters);
-            k_seg_final<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m,
+            k_seg_final<<<cnt, kSegMaxK, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m,
                                                                         lim_e, b.nseg, b.nstate, b.segpool,
                                                                         b.astate, b.apool, b.res, b.err, b.counters);
         }
