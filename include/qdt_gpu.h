@@ -44,6 +44,9 @@ typedef struct qdt_gpu_stats
     uint64_t err_bits;            /* device error flags of the last run (0 when it succeeded) */
     uint64_t cells;               /* LDS cells (one wavefront builds each cell's kd subtree) */
     uint64_t cell_level;          /* kd level of the cells */
+    uint64_t spare_records;       /* quad-edge records taken from the spare region above 3 * unique
+                                     points (demand; a run that needs more than the region holds
+                                     is repeated without seam pieces) */
 } qdt_gpu_stats;
 
 /* Like qdt_execute(), but d_pts points to n points in device memory.  The

@@ -498,7 +498,7 @@ constexpr int kCellNodes = 2 * kCellMax;  // subtree node slots (levels of 1, 2,
 constexpr int kCellLevels = 12;
 constexpr int kCellPer = (kCellMax + 63) / 64; // positions per lane in the cell's scans
 
-using LQE = QuadEdges<short*, const double2*>; // cell records: local ids fit 16 bits
+using LQE = QuadEdges<short*, const double2*, false>; // cell records in LDS: local ids fit 16 bits
 
 // cell c -> (level, node index): the nodes of level dc, then the roots of
 // the sets rooted below dc
@@ -794,23 +794,35 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
 
 constexpr int kSegMaxK = 256;          // anchors per node at most (a power of two)
 constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
-constexpr int kSegMin = 2048;          // smaller nodes merge in one piece
+constexpr int kSegMin = 1024;          // smaller nodes merge in one piece
 constexpr int kSegBlock = 64;          // pieces per block
 constexpr int kPhases = 2;             // pieces k, k+2, ... run together
 constexpr int kClaimBits = 20;         // piece id bits of a claim (the rest: launch epoch)
 constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit merges
 
-// number of anchors for a node of s points (0: one piece)
-__host__ __device__ inline int seam_anchors(double s)
+// Anchors for a node of s points whose extent across the cut is w and along
+// it h: the seam has about sqrt(s h / w) cross edges (the points within one
+// spacing of the cut line), and a piece gets ~5 of them.  0: one piece.
+__host__ __device__ inline int seam_anchors(double s, double w, double h)
 {
     if (s < kSegMin) return 0;
-    int k = (int)(std::sqrt(s) / 5.0);
-    return k < 2 ? 2 : (k > kSegMaxK ? kSegMaxK : k);
+    const double seam = w > 0 ? std::sqrt(s * h / w) : 1e9;
+    const double k = seam / 5.0;
+    if (k < 3) return 0;
+    return k > kSegMaxK ? kSegMaxK : (int)k;
+}
+
+// upper bound of seam_anchors for nodes of up to s points (elongation up to
+// 16:1 along the cut)
+__host__ __device__ inline int seam_anchors_max(double s)
+{
+    return s < kSegMin ? 0 : seam_anchors(s, 1.0, 16.0);
 }
 
 // split vertex of every internal node; anchor heights of segmented levels
 __global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const uint32_t* Bo, const int* off, int d,
-                                                      int K, int* splitv, int* qv)
+                                                      int K, const double2* PT, int* splitv, int* qv, int* kn,
+                                                      int* lvlK)
 {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= lcount(off, d)) return;
@@ -820,7 +832,20 @@ __global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const 
     if (s < 4) return;
     const size_t ni = node_index(off, d, j);
     splitv[ni] = (int)Bf[o + left_size(s)];
-    const int Kn = K ? min(K, seam_anchors((double)s)) : 0;
+    int Kn = 0;
+    if (K)
+    {
+        // extents from the two orders' ends: across the cut (this level's
+        // frame) and along it (the other frame)
+        const int axis = d & 1;
+        const double2 f0 = PT[Bf[o]], f1 = PT[Bf[o + s - 1]];
+        const double2 g0 = PT[Bo[o]], g1 = PT[Bo[o + s - 1]];
+        const double w = axis ? f1.y - f0.y : f1.x - f0.x;
+        const double h = axis ? g1.x - g0.x : g1.y - g0.y;
+        Kn = min(K, seam_anchors((double)s, w, h));
+    }
+    kn[ni] = Kn;
+    if (Kn) atomicMax(&lvlK[d], Kn);
     for (int k = 0; k < Kn; ++k) qv[ni * kSegMaxK + k] = (int)Bo[o + (int)(((long long)(2 * k + 1) * s) / (2 * Kn))];
 }
 
@@ -858,8 +883,8 @@ constexpr int kRec = 6 + 2 * kCross;           // ints per anchor record: u v xu
 
 __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, const double2* PT, int* VE,
                                                             const int* off, const int* splitv, const int* qv,
-                                                            const uint32_t* B0, const uint32_t* B1, int D, int d,
-                                                            int K, uint32_t m,
+                                                            const uint32_t* B0, const uint32_t* B1, const int* kn,
+                                                            int D, int d, int K, uint32_t m,
                                                             int lim_e, int* arec, double4* acoord, double* akey,
                                                             int* astate, unsigned long long* counters)
 {
@@ -873,7 +898,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
         int state = 0;
         const int* od = off + off_base(off, d);
         const int s = od[j + 1] - od[j];
-        if (s >= 4 && k < seam_anchors((double)s))
+        if (s >= 4 && k < kn[node_index(off, d, j)])
         {
             const int axis = d & 1;
             const size_t ni = node_index(off, d, j);
@@ -1676,7 +1701,7 @@ struct Engine::Bufs
     // node-proportional (ensure_nodes)
     int* off; // level table, then every level's node offsets
     int4* res;
-    int *splitv, *qv, *segstart, *nseg, *segresume;
+    int *splitv, *qv, *kn, *lvlK, *segstart, *nseg, *segresume;
     int *arec, *astate, *aedge, *selk, *nsel, *alist, *alist_n;
     double4* acoord;
     double* akey;
@@ -1748,8 +1773,9 @@ void make_plan(Engine::Plan& P, const std::vector<uint32_t>& sizes, const std::v
     P.qv_nodes = 1;
     for (int d = 0; d < D; ++d)
     {
-        // the pieces of one phase must have distinct claim ids
-        const int K = seams ? seam_anchors(P.smax[d]) : 0;
+        // candidate levels (the nodes report their own anchor counts during
+        // the kd phase); the pieces of one phase must have distinct claim ids
+        const int K = seams ? seam_anchors_max(P.smax[d]) : 0;
         const bool fits = (size_t)P.count[d] * (size_t)((K + kPhases) / kPhases) < ((size_t)1 << kClaimBits);
         P.K[d] = fits ? K : 0;
         if (!P.K[d]) continue;
@@ -1827,12 +1853,12 @@ Engine::~Engine()
 
 void Engine::release()
 {
-    for (char** r : { &ws_, &ws_nodes_, &ws_sets_ })
+    for (char** r : { &ws_, &ws_nodes_, &ws_anchors_, &ws_sets_ })
     {
         if (*r) (void)hipFree(*r);
         *r = nullptr;
     }
-    ws_bytes_ = ws_nodes_bytes_ = ws_sets_bytes_ = 0;
+    ws_bytes_ = ws_nodes_bytes_ = ws_anchors_bytes_ = ws_sets_bytes_ = 0;
     cap_n_ = 0;
     b_->nodes_cap = b_->offs_cap = b_->seg_nodes = b_->anchor_cap = b_->qv_nodes = 0;
     b_->sets_cap = b_->tris_cap = 0;
@@ -1840,7 +1866,7 @@ void Engine::release()
     batch_ = BatchResult();
 }
 
-size_t Engine::workspace_bytes() const { return ws_bytes_ + ws_nodes_bytes_ + ws_sets_bytes_; }
+size_t Engine::workspace_bytes() const { return ws_bytes_ + ws_nodes_bytes_ + ws_anchors_bytes_ + ws_sets_bytes_; }
 
 Status Engine::ensure(size_t n)
 {
@@ -1865,7 +1891,11 @@ Status Engine::ensure(size_t n)
     cap_n_ = 0;
     const size_t N = std::max<size_t>(n, 64);
     const uint32_t nb_sort = cdiv(N, kSortTile);
-    const size_t spare_recs = N / 8 + 65536;
+    // Seam pieces start with empty pools: each takes the records for the
+    // edges it adds beyond those it deletes (about the children's hull
+    // vertices near its cut) from here, while as many records stay free in
+    // the node's own region.  Summed over the seam levels this is ~n/8.
+    const size_t spare_recs = N / 2 + 65536;
     const size_t R = 3 * N + spare_recs;
     const uint32_t nb_scan = cdiv(R, kScanTile) + 1;
     const size_t qcap = std::max<size_t>(4096, N / 4);
@@ -1897,22 +1927,38 @@ Status Engine::ensure(size_t n)
 Status Engine::ensure_nodes(const Plan& P)
 {
     Bufs& b = *b_;
-    if (ws_nodes_ && P.nodes <= b.nodes_cap && P.offs <= b.offs_cap && P.seg_nodes <= b.seg_nodes &&
-        P.qv_nodes <= b.qv_nodes &&
-        P.anchor_cap <= b.anchor_cap)
-        return Status::ok;
+    if (ws_nodes_ && P.nodes <= b.nodes_cap && P.offs <= b.offs_cap && P.qv_nodes <= b.qv_nodes) return Status::ok;
     if (ws_nodes_) (void)hipFree(ws_nodes_);
     ws_nodes_ = nullptr;
     ws_nodes_bytes_ = 0;
     // grow with headroom so that similar batches reuse the allocation
     const size_t nodes = std::max(P.nodes, b.nodes_cap) + 64;
     const size_t offs = std::max(P.offs, b.offs_cap) + 64;
-    const size_t seg_nodes = std::max(P.seg_nodes, b.seg_nodes);
     const size_t qv_nodes = std::max(P.qv_nodes, b.qv_nodes);
-    const size_t anchor_cap = std::max(P.anchor_cap, b.anchor_cap);
     const Item items[] = {
         { (void**)&b.off, 4 * (kLvlHdr + offs) }, { (void**)&b.res, 16 * nodes }, { (void**)&b.splitv, 4 * nodes },
+        { (void**)&b.kn, 4 * nodes }, { (void**)&b.lvlK, 4 * kMaxLevels },
         { (void**)&b.qv, 4 * (size_t)kSegMaxK * qv_nodes },
+    };
+    const Status st = carve(items, ws_nodes_, ws_nodes_bytes_);
+    if (st != Status::ok) return st;
+    b.nodes_cap = nodes;
+    b.offs_cap = offs;
+    b.qv_nodes = qv_nodes;
+    return Status::ok;
+}
+
+// buffers of the seam pieces, sized by the levels the kd phase marked
+Status Engine::ensure_anchors(size_t seg_nodes_need, size_t anchor_need)
+{
+    Bufs& b = *b_;
+    if (ws_anchors_ && seg_nodes_need <= b.seg_nodes && anchor_need <= b.anchor_cap) return Status::ok;
+    if (ws_anchors_) (void)hipFree(ws_anchors_);
+    ws_anchors_ = nullptr;
+    ws_anchors_bytes_ = 0;
+    const size_t seg_nodes = std::max(seg_nodes_need, b.seg_nodes) + 1;
+    const size_t anchor_cap = std::max(anchor_need, b.anchor_cap) + 1;
+    const Item items[] = {
         { (void**)&b.segstart, 4 * (size_t)kSegSlots * seg_nodes },
         { (void**)&b.nseg, 4 * seg_nodes }, { (void**)&b.nstate, 16 * seg_nodes },
         { (void**)&b.segpool, 8 * (size_t)kSegSlots * seg_nodes },
@@ -1924,12 +1970,9 @@ Status Engine::ensure_nodes(const Plan& P)
         { (void**)&b.nsel, 4 * seg_nodes },
         { (void**)&b.alist, 4 * (2 + 4 * (size_t)kCross) * anchor_cap }, { (void**)&b.alist_n, 64 },
     };
-    const Status st = carve(items, ws_nodes_, ws_nodes_bytes_);
+    const Status st = carve(items, ws_anchors_, ws_anchors_bytes_);
     if (st != Status::ok) return st;
-    b.nodes_cap = nodes;
-    b.offs_cap = offs;
     b.seg_nodes = seg_nodes;
-    b.qv_nodes = qv_nodes;
     b.anchor_cap = anchor_cap;
     return Status::ok;
 }
@@ -2013,6 +2056,7 @@ Status Engine::run(const double* d_pts, size_t n)
         st_.device_ms += first.device_ms;
         for (int i = 0; i < 8; ++i) st_.seam[i] = first.seam[i];
         st_.err_bits = first.err_bits;
+        st_.spare_records = first.spare_records;
     }
     if (s != Status::ok) res_ = DeviceResult();
     return s;
@@ -2046,6 +2090,7 @@ Status Engine::run_batch(const double* d_pts, const size_t* set_off, int k)
         st_.device_ms += first.device_ms;
         for (int i = 0; i < 8; ++i) st_.seam[i] = first.seam[i];
         st_.err_bits = first.err_bits;
+        st_.spare_records = first.spare_records;
     }
     if (s != Status::ok) batch_ = BatchResult();
     return s;
@@ -2293,6 +2338,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     }
     st_.levels = (uint32_t)D;
     QDT_HIP(hipMemsetAsync(b.res, 0xff, 16 * P.nodes, s));
+    QDT_HIP(hipMemsetAsync(b.lvlK, 0, 4 * kMaxLevels, s));
     uint32_t *nodeA = b.nodeA, *nodeB = b.nodeB;
     uint32_t* Bt = b.Bt;
     auto roots_at = [&](int d, uint32_t* node) {
@@ -2321,7 +2367,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         if (cnt)
         {
             k_children<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(b.off, d);
-            k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, b.off, d, P.K[d], b.splitv, b.qv);
+            k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, b.off, d, P.K[d], b.P0, b.splitv, b.qv,
+                                                              b.kn, b.lvlK);
             k_split<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bf, nodeA, b.off, d, m, nodeB, b.bits);
             scan_u32(ZeroFlag{ Bo, b.bits }, m, StoreU32{ b.Z }, b.partials, s);
             k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, nodeA, b.off, d, b.bits, b.Z, m, Bt);
@@ -2337,6 +2384,23 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             if (ra < rb)
                 k_roots<<<cdiv((size_t)(rb - ra) + 1, kBlock), kBlock, 0, s>>>(b.off, d, ra, rb, vstart);
         }
+
+    // the levels the nodes asked to cut into pieces, and their anchor counts
+    {
+        std::vector<int> lk(kMaxLevels, 0);
+        QDT_HIP(hipMemcpyAsync(lk.data(), b.lvlK, 4 * kMaxLevels, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipStreamSynchronize(s));
+        size_t seg_nodes = 1, anchor_cap = 1;
+        for (int d = 0; d < D; ++d)
+        {
+            P.K[d] = d < Dk ? std::min(P.K[d], lk[d]) : 0;
+            if (!P.K[d]) continue;
+            seg_nodes = std::max(seg_nodes, (size_t)P.count[d]);
+            anchor_cap = std::max(anchor_cap, (size_t)P.count[d] * P.K[d]);
+        }
+        st = ensure_anchors(seg_nodes, anchor_cap);
+        if (st != Status::ok) return st;
+    }
 
     mark(5);
     // 6. leaves, 7. merges
@@ -2376,7 +2440,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         {
             const int total = (int)(cnt * (uint32_t)K);
             k_seg_anchor<<<cdiv((size_t)total, kMergeBlock), kMergeBlock, 0, s>>>(
-                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, Dk, d, K, m, lim_e, b.arec, b.acoord, b.akey,
+                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, b.kn, Dk, d, K, m, lim_e, b.arec, b.acoord, b.akey,
                 b.astate, b.counters);
             k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
                                                                           b.spare, spare_base, spare_cap, b.res,
@@ -2395,8 +2459,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                 b.err, b.counters);
             k_seg_pools<<<ga, kMergeBlock, 0, s>>>(b.NX, b.OR, total, b.arec, b.astate, b.aedge, lim_e, (int*)b.face,
                                                    b.apool);
-            k_seg_compact<<<cnt, kSegMaxK, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge,
-                                                                          b.segstart, b.nseg, b.off);
+            k_seg_compact<<<cnt, kSegMaxK, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge, b.segstart, b.nseg,
+                                                   b.off);
             for (int parity = 0; parity < kPhases; ++parity)
                 k_seg_run<<<cdiv((size_t)cnt * ((K + kPhases) / kPhases), kSegBlock), kSegBlock, 0, s>>>(
                     b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.spare, spare_base, spare_cap, b.segstart, b.nseg,
@@ -2429,9 +2493,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     const uint32_t nbx = cdiv(nrec, kScanTile);
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbx, 4, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipMemcpyAsync(hs + 1, b.err, 4, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipMemcpyAsync(hs + 2, b.spare, 4, hipMemcpyDeviceToHost, s));
     unsigned long long cnts[16];
     QDT_HIP(hipMemcpyAsync(cnts, b.counters, 8 * 16, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
+    st_.spare_records = hs[2];
     if (hs[1] & kSeamErr)
     {
         for (int i = 0; i < 8; ++i) st_.seam[i] = cnts[5 + i];

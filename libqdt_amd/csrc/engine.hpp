@@ -40,6 +40,7 @@ struct RunStats
     // anchors with a clean dry run, anchors selected, rings rewritten
     uint64_t seam[8];
     uint32_t err_bits;
+    uint64_t spare_records; // records drawn from the spare region (demand, may exceed it)
 };
 
 enum class Status : int
@@ -108,6 +109,7 @@ public:
 private:
     Status ensure(size_t n);
     Status ensure_nodes(const Plan& P);
+    Status ensure_anchors(size_t seg_nodes, size_t anchor_cap);
     Status ensure_sets(size_t k, size_t n);
     Status run_impl(const double* d_pts, size_t n, const size_t* set_off, int k, bool seams, bool& redo);
     Status batch_output(uint32_t T, int k, int kg, uint32_t m);
@@ -116,8 +118,9 @@ private:
     bool own_stream_;
     char* ws_ = nullptr;       // point-proportional buffers
     char* ws_nodes_ = nullptr; // kd forest nodes
+    char* ws_anchors_ = nullptr; // seam pieces
     char* ws_sets_ = nullptr;  // per-set buffers of batches
-    size_t ws_bytes_ = 0, ws_nodes_bytes_ = 0, ws_sets_bytes_ = 0;
+    size_t ws_bytes_ = 0, ws_nodes_bytes_ = 0, ws_anchors_bytes_ = 0, ws_sets_bytes_ = 0;
     size_t cap_n_ = 0;
     int block_points_ = 0;
     bool profiling_ = false;

@@ -231,6 +231,7 @@ void copy_stats(const qdt::RunStats& s, qdt_gpu_stats& o)
     o.err_bits = s.err_bits;
     o.cells = s.cells;
     o.cell_level = s.block_level;
+    o.spare_records = s.spare_records;
 }
 
 void record_stats(qdt_t* Q)

@@ -53,7 +53,11 @@ struct PredCount
 // dependent load costs nothing measurable) and every loop has a budget: a
 // logic error unwinds instead of touching memory outside the store or
 // spinning forever.
-template <class IntPtr, class PtPtr>
+// kChecked = false drops the index checks: for stores in LDS, where an index
+// out of range cannot fault (the hardware discards such writes and returns 0
+// for such reads) and the checks cost a quarter of the merge instructions.
+// The loop budgets stay, so a damaged store still cannot spin forever.
+template <class IntPtr, class PtPtr, bool kChecked = true>
 struct QuadEdges
 {
     IntPtr NX;
@@ -91,6 +95,7 @@ struct QuadEdges
 
     __device__ __forceinline__ int chk(int e)
     {
+        if (!kChecked) return e;
         if ((unsigned)e >= (unsigned)lim_e)
         {
             err = err ? err : 1;
@@ -108,6 +113,7 @@ struct QuadEdges
     __device__ __forceinline__ int rprev(int e) { return onext(e ^ 2); }
     __device__ __forceinline__ double2 pt(int v)
     {
+        if (!kChecked) return PT[v];
         if ((unsigned)v >= (unsigned)lim_v)
         {
             err = err ? err : 1;
