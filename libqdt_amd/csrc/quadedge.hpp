@@ -257,6 +257,19 @@ struct QuadEdges
         ++pc.incircle;
         return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact) > 0;
     }
+    // the same tests on coordinates already loaded
+    __device__ __forceinline__ bool ccw_p(double2 A, double2 B, double2 C)
+    {
+        ++pc.orient;
+        return orient2d(A.x, A.y, B.x, B.y, C.x, C.y, pc.orient_exact) > 0;
+    }
+    __device__ __forceinline__ bool in_circle_p(int a, int b, int c, int d, double2 A, double2 B, double2 C,
+                                                double2 D)
+    {
+        if (a == d || b == d || c == d || a == b || a == c || b == c) return false;
+        ++pc.incircle;
+        return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact) > 0;
+    }
     __device__ __forceinline__ double incircle_v(int a, int b, int c, int d)
     {
         const double2 A = pt(a), B = pt(b), C = pt(c), D = pt(d);
@@ -366,6 +379,10 @@ struct QuadEdges
         const int srec = stop >= 0 ? (stop >> 2) : -1;
         const int so = stop >= 0 ? org(stop) : -1;
         const int sd = stop >= 0 ? dest(stop) : -1;
+        // the base edge's ends and their coordinates carry over to the next
+        // step (one end stays, the other is the chosen candidate's)
+        int bo = org(basel), bd = dest(basel);
+        double2 Po = pt(bo), Pd = pt(bd);
         for (;;)
         {
             if (err) return 0;
@@ -374,67 +391,123 @@ struct QuadEdges
                 err = 3;
                 return 0;
             }
-            const int bo = org(basel);
-            const int bd = dest(basel);
             if (!take(bo) || !take(bd)) return 2;
+            // The left scan deletes only edges at bd between left vertices
+            // and the right scan only edges at bo between right vertices, so
+            // neither changes what the other side reads.  In global memory
+            // the first two candidates of both sides are loaded before any
+            // is used (their latencies overlap); in LDS the second ones are
+            // loaded only when needed.
             int lc = onext(basel ^ 2);
+            int rc = oprev(basel);
+            int ln = -1, rp = -1, lnd = -1, rpd = -1;
+            if (kChecked)
+            {
+                ln = onext(lc);
+                rp = oprev(rc);
+            }
+            int lcd = dest(lc), rcd = dest(rc);
+            if (kChecked)
+            {
+                lnd = dest(ln);
+                rpd = dest(rp);
+            }
+            double2 Pl = pt(lcd), Pr = pt(rcd), Pln, Prp;
+            if (kChecked)
+            {
+                Pln = pt(lnd);
+                Prp = pt(rpd);
+            }
             bool lv = false;
             if ((lc >> 2) != srec)
             {
-                lv = ccw(dest(lc), bd, bo);
+                lv = ccw_p(Pl, Pd, Po);
                 if (lv)
                 {
                     bool deleted = false;
                     for (;;)
                     {
                         if (err || --budget < 0) break;
-                        const int nx = onext(lc);
-                        if ((nx >> 2) == srec) break;
-                        if (!in_circle(bd, bo, dest(lc), dest(nx))) break;
-                        if (!take(dest(lc))) return 2;
+                        if (!kChecked) ln = onext(lc);
+                        if ((ln >> 2) == srec) break;
+                        if (!kChecked)
+                        {
+                            lnd = dest(ln);
+                            Pln = pt(lnd);
+                        }
+                        if (!in_circle_p(bd, bo, lcd, lnd, Pd, Po, Pl, Pln)) break;
+                        if (!take(lcd)) return 2;
                         delete_edge(lc, pool);
-                        lc = nx;
+                        lc = ln;
+                        lcd = lnd;
+                        Pl = Pln;
+                        if (kChecked)
+                        {
+                            ln = onext(lc);
+                            lnd = dest(ln);
+                            Pln = pt(lnd);
+                        }
                         deleted = true;
                     }
-                    if (deleted) lv = ccw(dest(lc), bd, bo);
+                    if (deleted) lv = ccw_p(Pl, Pd, Po);
                 }
             }
-            int rc = oprev(basel);
             bool rv = false;
             if ((rc >> 2) != srec)
             {
-                rv = ccw(dest(rc), bd, bo);
+                rv = ccw_p(Pr, Pd, Po);
                 if (rv)
                 {
                     bool deleted = false;
                     for (;;)
                     {
                         if (err || --budget < 0) break;
-                        const int pv = oprev(rc);
-                        if ((pv >> 2) == srec) break;
-                        if (!in_circle(bd, bo, dest(rc), dest(pv))) break;
-                        if (!take(dest(rc))) return 2;
+                        if (!kChecked) rp = oprev(rc);
+                        if ((rp >> 2) == srec) break;
+                        if (!kChecked)
+                        {
+                            rpd = dest(rp);
+                            Prp = pt(rpd);
+                        }
+                        if (!in_circle_p(bd, bo, rcd, rpd, Pd, Po, Pr, Prp)) break;
+                        if (!take(rcd)) return 2;
                         delete_edge(rc, pool);
-                        rc = pv;
+                        rc = rp;
+                        rcd = rpd;
+                        Pr = Prp;
+                        if (kChecked)
+                        {
+                            rp = oprev(rc);
+                            rpd = dest(rp);
+                            Prp = pt(rpd);
+                        }
                         deleted = true;
                     }
-                    if (deleted) rv = ccw(dest(rc), bd, bo);
+                    if (deleted) rv = ccw_p(Pr, Pd, Po);
                 }
             }
             if (!lv && !rv) return 0;
-            const bool right = !lv || (rv && in_circle(dest(lc), bd, bo, dest(rc)));
-            // the new cross edge would run dest(rc) -> bd or bo -> dest(lc)
+            const bool right = !lv || (rv && in_circle_p(lcd, bd, bo, rcd, Pl, Pd, Po, Pr));
+            // the new cross edge would run rcd -> bd or bo -> lcd
             if (stop >= 0)
             {
-                const int no = right ? dest(rc) : bo;
-                const int nd = right ? bd : dest(lc);
+                const int no = right ? rcd : bo;
+                const int nd = right ? bd : lcd;
                 if (no == so && nd == sd) return 1;
             }
-            if (!take(right ? dest(rc) : dest(lc))) return 2;
+            if (!take(right ? rcd : lcd)) return 2;
             if (right)
+            {
                 basel = connect(rc, basel ^ 2, pool);
+                bo = rcd;
+                Po = Pr;
+            }
             else
+            {
                 basel = connect(basel ^ 2, lc ^ 2, pool);
+                bd = lcd;
+                Pd = Pl;
+            }
         }
     }
 

@@ -67,21 +67,30 @@ __device__ int nn_walk(Q& q, int s, double2 p, int budget, int& e_out)
     double bd = dist2(q, s, p);
     for (int step = 0; step < budget && !q.err; ++step)
     {
-        int e = e0, nb = -1, ne = -1;
+        int nb = -1, ne = -1;
         double nbd = bd;
         int guard = 0;
-        do
+        // ring scan, pipelined: the next edge and its far end are requested
+        // before the current end's coordinates are used, so the scan waits
+        // for one load per edge instead of two
+        int e = e0, w = q.dest(e0), en = q.onext(e0);
+        for (;;)
         {
-            const int w = q.dest(e);
-            const double d = dist2(q, w, p);
+            const double2 W = q.pt(w);
+            const int wn = q.dest(en), enn = q.onext(en);
+            const double dx = W.x - p.x, dy = W.y - p.y;
+            const double d = dx * dx + dy * dy;
             if (d < nbd)
             {
                 nbd = d;
                 nb = w;
                 ne = e ^ 2;
             }
-            e = q.onext(e);
-        } while (e != e0 && ++guard < 4096 && !q.err);
+            e = en;
+            if (e == e0 || ++guard >= 4096 || q.err) break;
+            w = wn;
+            en = enn;
+        }
         if (nb < 0)
         {
             e_out = e0;
@@ -113,9 +122,12 @@ __device__ bool certify(Q& q, int u, int eu, int v, int ev)
     {
         const int e0 = side ? ev : eu;
         int e = e0, guard = 0;
+        int w = q.dest(e0), en = q.onext(e0);
         do
         {
-            const int w = q.dest(e);
+            // the next edge and its far end are requested before this one's
+            // tests (pipelined ring scan)
+            const int wn = q.dest(en), enn = q.onext(en);
             if (w != u && w != v)
             {
                 const double o = q.orient(u, v, w);
@@ -131,7 +143,9 @@ __device__ bool certify(Q& q, int u, int eu, int v, int ev)
                     if (wr < 0 || q.incircle_v(v, u, wr, w) > 0) wr = w;
                 }
             }
-            e = q.onext(e);
+            e = en;
+            w = wn;
+            en = enn;
         } while (e != e0 && ++guard < kMax && !q.err);
         if (e != e0) return false;
     }
