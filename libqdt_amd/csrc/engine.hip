@@ -881,6 +881,11 @@ __device__ int kd_descend(const int* off, const int* splitv, const double2* PT, 
 constexpr int kCross = 16;                     // crossing edges per side of an anchor at most
 constexpr int kRec = 6 + 2 * kCross;           // ints per anchor record: u v xu xv nl nr crossings
 
+// Two lanes per anchor (adjacent, so in one wavefront): lane side 0 works in
+// the lower child L, lane side 1 in the upper child R.  Every decision both
+// lanes take depends only on values they exchange, so a pair never diverges
+// and its shuffles always find the partner active; the two walks, ring scans
+// and dry runs of an anchor overlap their memory latencies.
 __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, const double2* PT, int* VE,
                                                             const int* off, const int* splitv, const int* qv,
                                                             const uint32_t* B0, const uint32_t* B1, const int* kn,
@@ -888,7 +893,9 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
                                                             int lim_e, int* arec, double4* acoord, double* akey,
                                                             int* astate, unsigned long long* counters)
 {
-    const uint32_t t = blockIdx.x * kMergeBlock + threadIdx.x;
+    const uint32_t lane2 = blockIdx.x * kMergeBlock + threadIdx.x;
+    const uint32_t t = lane2 >> 1; // anchor
+    const int side = (int)(lane2 & 1);
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
     if (t < lcount(off, d) * (uint32_t)K)
@@ -905,58 +912,75 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
             const double c = frame_x(PT[splitv[ni]], axis);
             const double Y = frame_y(PT[qv[ni * kSegMaxK + k]], axis);
             double2 p = from_frame(c, Y, axis);
-            const int sL = kd_descend(off, splitv, PT, B0, B1, D, d + 1, 2 * j, p, axis, 0);
-            const int sR = kd_descend(off, splitv, PT, B0, B1, D, d + 1, 2 * j + 1, p, axis, 1);
-            int eu = -1, ev = -1;
-            int u = sL >= 0 ? nn_walk(q, sL, p, 512, eu) : -1;
-            int v = sR >= 0 ? nn_walk(q, sR, p, 512, ev) : -1;
-            for (int it = 0; it < 8 && u >= 0 && v >= 0 && !q.err; ++it)
+            // own side: x (nearest vertex) and ex (an edge out of it)
+            const int st = kd_descend(off, splitv, PT, B0, B1, D, d + 1, 2 * j + side, p, axis, side);
+            int ex = -1;
+            int x = st >= 0 ? nn_walk(q, st, p, 512, ex) : -1;
+            int xo = __shfl_xor(x, 1), exo = __shfl_xor(ex, 1);
+            int u = side ? xo : x, v = side ? x : xo;
+            for (int it = 0; it < 8 && u >= 0 && v >= 0; ++it)
             {
+                if (q.err | __shfl_xor(q.err, 1)) break;
                 const double2 U = PT[u], V = PT[v];
                 const double ux = frame_x(U, axis), uy = frame_y(U, axis);
                 const double vx = frame_x(V, axis), vy = frame_y(V, axis);
                 if (!(vx > ux)) break;
                 const double X = ((vx - ux) * (vx + ux) + (Y - vy) * (Y - vy) - (Y - uy) * (Y - uy)) / (2.0 * (vx - ux));
                 p = from_frame(X, Y, axis);
-                int eu2 = -1, ev2 = -1;
-                const int u2 = nn_walk(q, u, p, 512, eu2);
-                const int v2 = nn_walk(q, v, p, 512, ev2);
-                const bool same = u2 == u && v2 == v;
-                u = u2;
-                v = v2;
-                eu = eu2;
-                ev = ev2;
+                int ex2 = -1;
+                const int x2 = nn_walk(q, x, p, 512, ex2);
+                const int xo2 = __shfl_xor(x2, 1), exo2 = __shfl_xor(ex2, 1);
+                const bool same = x2 == x && xo2 == xo;
+                x = x2;
+                xo = xo2;
+                ex = ex2;
+                exo = exo2;
+                u = side ? xo : x;
+                v = side ? x : xo;
                 if (same) break;
             }
-            if (u >= 0 && v >= 0 && eu >= 0 && ev >= 0 && !q.err && certify(q, u, eu, v, ev))
+            const int eu = side ? exo : ex, ev = side ? ex : exo;
+            bool ok = u >= 0 && v >= 0 && eu >= 0 && ev >= 0 && !(q.err | __shfl_xor(q.err, 1));
+            if (ok)
             {
-                atomicAdd(&counters[5], 1ull);
-                // dry run of the insertion on the unmerged halves
+                int wl = -1, wr = -1;
+                const int ring = certify_ring(q, u, v, side ? ev : eu, wl, wr) ? 1 : 0;
+                const int wlo = __shfl_xor(wl, 1), wro = __shfl_xor(wr, 1), ringo = __shfl_xor(ring, 1);
+                ok = ring && ringo && !(q.err | __shfl_xor(q.err, 1));
+                if (ok)
+                    ok = certify_join(q, u, v, side ? wlo : wl, side ? wro : wr, side ? wl : wlo, side ? wr : wro);
+            }
+            if (ok)
+            {
+                if (side == 0) atomicAdd(&counters[5], 1ull);
+                // dry run of the insertion on the unmerged halves: each lane
+                // finds its end's wedge and the crossings on its side
                 int* r = arec + (size_t)t * kRec;
-                const int xu = find_wedge(q, u, eu, v);
-                const int xv = xu >= 0 ? find_wedge(q, v, ev, u) : -1;
-                const int nl = xv >= 0 ? collect_crossings(q, u, xu, v, r + 6, kCross) : -1;
-                const int nr = nl >= 0 ? collect_crossings(q, v, xv, u, r + 6 + kCross, kCross) : -1;
-                if (nr >= 0 && !q.err)
+                const int me = side ? v : u, other = side ? u : v;
+                const int xw = find_wedge(q, me, side ? ev : eu, other);
+                const int n = xw >= 0 ? collect_crossings(q, me, xw, other, r + 6 + side * kCross, kCross) : -1;
+                const int no = __shfl_xor(n, 1);
+                const int bad = q.err | __shfl_xor(q.err, 1);
+                if (n >= 0 && no >= 0 && !bad)
                 {
-                    r[0] = u;
-                    r[1] = v;
-                    r[2] = xu;
-                    r[3] = xv;
-                    r[4] = nl;
-                    r[5] = nr;
-                    const double2 U = PT[u], V = PT[v];
-                    acoord[t] = make_double4(U.x, U.y, V.x, V.y);
-                    // height where the anchor crosses the cut line: the sort key
-                    const double ux = frame_x(U, axis), uy = frame_y(U, axis);
-                    const double vx = frame_x(V, axis), vy = frame_y(V, axis);
-                    akey[t] = vx > ux ? uy + (c - ux) / (vx - ux) * (vy - uy) : 0.5 * (uy + vy);
+                    r[side] = me;
+                    r[2 + side] = xw;
+                    r[4 + side] = n;
+                    if (side == 0)
+                    {
+                        const double2 U = PT[u], V = PT[v];
+                        acoord[t] = make_double4(U.x, U.y, V.x, V.y);
+                        // height where the anchor crosses the cut line: the sort key
+                        const double ux = frame_x(U, axis), uy = frame_y(U, axis);
+                        const double vx = frame_x(V, axis), vy = frame_y(V, axis);
+                        akey[t] = vx > ux ? uy + (c - ux) / (vx - ux) * (vy - uy) : 0.5 * (uy + vy);
+                        atomicAdd(&counters[10], 1ull);
+                    }
                     state = 1;
-                    atomicAdd(&counters[10], 1ull);
                 }
             }
         }
-        astate[t] = state;
+        if (side == 0) astate[t] = state;
     }
     flush_counts(counters, q.pc);
 }
@@ -2439,7 +2463,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         else
         {
             const int total = (int)(cnt * (uint32_t)K);
-            k_seg_anchor<<<cdiv((size_t)total, kMergeBlock), kMergeBlock, 0, s>>>(
+            k_seg_anchor<<<cdiv(2 * (size_t)total, kMergeBlock), kMergeBlock, 0, s>>>(
                 b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, b.kn, Dk, d, K, m, lim_e, b.arec, b.acoord, b.akey,
                 b.astate, b.counters);
             k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,

@@ -105,50 +105,59 @@ __device__ int nn_walk(Q& q, int s, double2 p, int budget, int& e_out)
 }
 
 // Exact certificate that (u, v) is an edge of every Delaunay triangulation of
-// L u R (u in L, v in R, eu / ev edges out of u / v).  The circles through u
-// and v form a one-parameter family; every neighbour w has the member c(w)
-// through it.  The condition "incircle(u, v, wl, wr) < 0 for every left
-// neighbour wl and right neighbour wr" is max c(right) < min c(left), so it
-// suffices to find the extreme neighbour of each side by a streaming scan
-// (the side's point inside the circles of all the others; comparisons are
-// the same exact incircle tests) and test that one pair.  A neighbour
-// collinear with u and v rejects the anchor.
+// L u R (u in L, v in R).  The circles through u and v form a one-parameter
+// family; every neighbour w has the member c(w) through it.  The condition
+// "incircle(u, v, wl, wr) < 0 for every left neighbour wl and right
+// neighbour wr" is max c(right) < min c(left), so it suffices to find the
+// extreme neighbour of each side by a streaming scan (the side's point inside
+// the circles of all the others; comparisons are the same exact incircle
+// tests) and test that one pair.  A neighbour collinear with u and v rejects
+// the anchor.  certify_ring scans the ring of e0's origin (u's ring or v's
+// ring; both are needed), certify_join combines the two scans.
 template <class Q>
-__device__ bool certify(Q& q, int u, int eu, int v, int ev)
+__device__ bool certify_ring(Q& q, int u, int v, int e0, int& wl, int& wr)
 {
     constexpr int kMax = 64; // ring length bound
-    int wl = -1, wr = -1;
-    for (int side = 0; side < 2; ++side)
+    int e = e0, guard = 0;
+    int w = q.dest(e0), en = q.onext(e0);
+    do
     {
-        const int e0 = side ? ev : eu;
-        int e = e0, guard = 0;
-        int w = q.dest(e0), en = q.onext(e0);
-        do
+        // the next edge and its far end are requested before this one's
+        // tests (pipelined ring scan)
+        const int wn = q.dest(en), enn = q.onext(en);
+        if (w != u && w != v)
         {
-            // the next edge and its far end are requested before this one's
-            // tests (pipelined ring scan)
-            const int wn = q.dest(en), enn = q.onext(en);
-            if (w != u && w != v)
+            const double o = q.orient(u, v, w);
+            if (o == 0) return false;
+            if (o > 0)
             {
-                const double o = q.orient(u, v, w);
-                if (o == 0) return false;
-                if (o > 0)
-                {
-                    // left: keep the one inside the others' circles (u, v, w ccw)
-                    if (wl < 0 || q.incircle_v(u, v, wl, w) > 0) wl = w;
-                }
-                else
-                {
-                    // right: (v, u, w) is ccw
-                    if (wr < 0 || q.incircle_v(v, u, wr, w) > 0) wr = w;
-                }
+                // left: keep the one inside the others' circles (u, v, w ccw)
+                if (wl < 0 || q.incircle_v(u, v, wl, w) > 0) wl = w;
             }
-            e = en;
-            w = wn;
-            en = enn;
-        } while (e != e0 && ++guard < kMax && !q.err);
-        if (e != e0) return false;
-    }
+            else
+            {
+                // right: (v, u, w) is ccw
+                if (wr < 0 || q.incircle_v(v, u, wr, w) > 0) wr = w;
+            }
+        }
+        e = en;
+        w = wn;
+        en = enn;
+    } while (e != e0 && ++guard < kMax && !q.err);
+    return e == e0;
+}
+
+// The extreme neighbours of two scans (of u's ring and of v's ring) combined
+// with the same test, then the one pair that decides.  The circles through u
+// and v are ordered by their centres along the bisector, so the extreme of
+// the union is the more extreme of the two partial extremes; where two
+// neighbours tie (cocircular with u and v) either one gives the same answer.
+template <class Q>
+__device__ bool certify_join(Q& q, int u, int v, int wl0, int wr0, int wl1, int wr1)
+{
+    int wl = wl0, wr = wr0;
+    if (wl1 >= 0 && (wl < 0 || q.incircle_v(u, v, wl, wl1) > 0)) wl = wl1;
+    if (wr1 >= 0 && (wr < 0 || q.incircle_v(v, u, wr, wr1) > 0)) wr = wr1;
     if (wl >= 0 && wr >= 0 && !(q.incircle_v(u, v, wl, wr) < 0)) return false;
     return !q.err;
 }
