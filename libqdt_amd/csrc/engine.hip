@@ -802,12 +802,12 @@ constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit mer
 
 // Anchors for a node of s points whose extent across the cut is w and along
 // it h: the seam has about sqrt(s h / w) cross edges (the points within one
-// spacing of the cut line), and a piece gets ~5 of them.  0: one piece.
+// spacing of the cut line), and a piece gets ~3.5 of them.  0: one piece.
 __host__ __device__ inline int seam_anchors(double s, double w, double h)
 {
     if (s < kSegMin) return 0;
     const double seam = w > 0 ? std::sqrt(s * h / w) : 1e9;
-    const double k = seam / 5.0;
+    const double k = seam / 3.5;
     if (k < 3) return 0;
     return k > kSegMaxK ? kSegMaxK : (int)k;
 }
