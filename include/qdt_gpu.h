@@ -47,6 +47,9 @@ typedef struct qdt_gpu_stats
     uint64_t spare_records;       /* quad-edge records taken from the spare region above 3 * unique
                                      points (demand; a run that needs more than the region holds
                                      is repeated without seam pieces) */
+    uint64_t cells_exact;         /* LDS cells rebuilt with the exact predicates: the first pass
+                                     runs the floating-point filters alone and lists every cell
+                                     where one could not decide */
 } qdt_gpu_stats;
 
 /* Like qdt_execute(), but d_pts points to n points in device memory.  The

@@ -70,6 +70,7 @@ class GpuStats(ctypes.Structure):
         ("seam_level_ms_sum", ctypes.c_double), ("seam_retries", ctypes.c_uint64),
         ("seam_counters", ctypes.c_uint64 * 8), ("err_bits", ctypes.c_uint64), ("cells", ctypes.c_uint64),
         ("cell_level", ctypes.c_uint64), ("spare_records", ctypes.c_uint64),
+        ("cells_exact", ctypes.c_uint64),
     ]
     STAGES = ("keys_sort_y", "sort_x", "dedupe", "frame1_sort", "kd_levels", "leaves", "merges", "extract")
 

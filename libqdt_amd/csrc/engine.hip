@@ -494,11 +494,14 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
 // ---------------------------------------------------------------------------
 
 constexpr int kCellMax = 128;             // points per cell at most
-constexpr int kCellNodes = 2 * kCellMax;  // subtree node slots (levels of 1, 2, 4, ... nodes)
 constexpr int kCellLevels = 12;
 constexpr int kCellPer = (kCellMax + 63) / 64; // positions per lane in the cell's scans
 
-using LQE = QuadEdges<short*, const double2*, false>; // cell records in LDS: local ids fit 16 bits
+// The fast pass runs the filters alone: no exact stage, no calls, fewer
+// registers; a cell where a filter cannot decide is listed and rebuilt by
+// the exact pass.
+using LQE = QuadEdges<short*, const double2*, false, false>; // cell records in LDS: local ids fit 16 bits
+using LQX = QuadEdges<short*, const double2*, false, true>;
 
 // cell c -> (level, node index): the nodes of level dc, then the roots of
 // the sets rooted below dc
@@ -528,33 +531,74 @@ __device__ __forceinline__ bool cell_node(const int* off, int dc, int D, uint32_
     return false;
 }
 
-__global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
-                                             const uint32_t* B1, const int* off, int dc, int D, uint32_t m,
-                                             int lim_e, int4* res, int* err, unsigned long long* counters)
+// LDS of a cell of at most cap points (bytes), laid out by CellLds
+__host__ __device__ constexpr size_t cell_lds_bytes(int cap)
 {
-    __shared__ double2 P[kCellMax];
-    __shared__ int gid[kCellMax];
-    __shared__ int A0[kCellMax], A1[kCellMax], T[kCellMax], Z[kCellMax + 1];
-    __shared__ uint8_t right[kCellMax];
-    __shared__ short LNX[4 * 3 * kCellMax], LOR[2 * 3 * kCellMax];
-    __shared__ int LVE[kCellMax];
-    __shared__ int loff[kCellNodes + kCellLevels]; // sub-level node offsets
-    __shared__ int lbase[kCellLevels + 1];        // first offset of each sub-level in loff
-    __shared__ short4 lres[kCellNodes];
-    __shared__ int nlev, bad;
+    return 16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 6 * (size_t)cap +
+        2 * (size_t)(2 * cap + kCellLevels + 2) + 8 * (size_t)(2 * cap) + 64;
+}
+
+struct CellLds
+{
+    double2* P;   // cap points, frame-0 order
+    short* LNX;   // 3 cap records, onext of the 4 directed edges each
+    short* LOR;   // origins of the 2 primal edges of each record
+    short *A0, *A1; // the cell's positions in frame-0 / frame-1 order
+    short* LVE;   // vertex -> an edge out of it
+    short* loff;  // sub-level node offsets
+    short4* lres; // sub-level node results (2 cap slots)
+    // partition scratch, inside the record area (unused until the leaves)
+    short *T, *Z;
+    uint8_t* right;
+    int* gid;     // global vertex ids (load phase)
+    __device__ CellLds(unsigned char* lds, int cap)
+    {
+        P = (double2*)lds;
+        LNX = (short*)(lds + 16 * (size_t)cap);
+        LOR = LNX + 12 * cap;
+        A0 = LOR + 6 * cap;
+        A1 = A0 + cap;
+        LVE = A1 + cap;
+        loff = LVE + cap;
+        lres = (short4*)(lds + ((16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 6 * (size_t)cap +
+                                 2 * (size_t)(2 * cap + kCellLevels + 2) + 7) & ~(size_t)7));
+        unsigned char* x = (unsigned char*)LNX; // 24 cap bytes
+        T = (short*)x;
+        Z = (short*)(x + 2 * (size_t)cap);
+        right = x + 4 * (size_t)cap + 4;
+        gid = (int*)(x + 8 * (size_t)cap);
+    }
+};
+
+// One cell: returns false (and writes nothing) when Q lacks the exact stage
+// and some filter could not decide.
+template <class Q>
+__device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, int* OR, const double2* PT, int* VE,
+                           const uint32_t* B0, const uint32_t* B1, const int* off, int dc, int D, int4* res,
+                           int* err, unsigned long long* counters)
+{
+    __shared__ int lbase[kCellLevels + 1]; // first offset of each sub-level in loff
+    __shared__ int bad;
+    const CellLds W(lds, cap);
+    double2* P = W.P;
+    short *A0 = W.A0, *A1 = W.A1, *T = W.T, *Z = W.Z, *loff = W.loff, *LNX = W.LNX, *LOR = W.LOR, *LVE = W.LVE;
+    uint8_t* right = W.right;
+    int* gid = W.gid;
+    short4* lres = W.lres;
+    const int kNodes = 2 * cap;
 
     const int lane = threadIdx.x;
     int dl;
     uint32_t j;
-    if (!cell_node(off, dc, D, blockIdx.x, dl, j)) return;
+    if (!cell_node(off, dc, D, cell, dl, j)) return true;
     const int* od = off + off_base(off, dl);
     const int o = od[j];
     const int s = od[j + 1] - o;
-    if (s <= 0) return;
-    if (s > kCellMax)
+    if (s <= 0) return true;
+    if (s > cap)
     {
         if (lane == 0) atomicOr(err, 2 << 1);
-        return;
+        return true;
     }
     // points in frame-0 order; vertex ids within a node ascend in that order
     for (int i = lane; i < s; i += 64)
@@ -562,7 +606,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
         const int v = (int)B0[o + i];
         gid[i] = v;
         P[i] = PT[v];
-        A0[i] = i;
+        A0[i] = (short)i;
         LVE[i] = -1;
     }
     if (lane == 0)
@@ -570,7 +614,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
         bad = 0;
         lbase[0] = 0;
         loff[0] = 0;
-        loff[1] = s;
+        loff[1] = (short)s;
     }
     __syncthreads();
     for (int i = lane; i < s; i += 64)
@@ -586,7 +630,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
                 hi = mid;
         }
         if (gid[lo] != v) bad = 1;
-        A1[i] = lo;
+        A1[i] = (short)lo;
     }
     __syncthreads();
     // kd partition of the cell, sub-level by sub-level (absolute level dl + l)
@@ -594,26 +638,26 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
     for (;;)
     {
         const int cnt = 1 << L;
-        const int* ol = loff + lbase[L];
+        const short* ol = loff + lbase[L];
         bool mine = false;
         for (int q = lane; q < cnt; q += 64) mine |= ol[q + 1] - ol[q] >= 4;
         if (__ballot(mine) == 0) break;
-        if (L + 1 >= kCellLevels || (4 << L) - 1 > kCellNodes || lbase[L] + 3 * cnt + 2 > kCellNodes + kCellLevels)
+        if (L + 1 >= kCellLevels || (4 << L) - 1 > kNodes || lbase[L] + 3 * cnt + 2 > kNodes + kCellLevels)
         {
-            bad = 1; // deeper than the slots allow: cannot happen for kCellMax points
+            bad = 1; // deeper than the slots allow: cannot happen for cap points
             break;
         }
         const int axis = (dl + L) & 1;
-        int* Af = axis ? A1 : A0;
-        int* Ao = axis ? A0 : A1;
+        short* Af = axis ? A1 : A0;
+        short* Ao = axis ? A0 : A1;
         // children offsets
         const int nb = lbase[L] + cnt + 1;
-        int* oc = loff + nb;
+        short* oc = loff + nb;
         for (int qn = lane; qn < cnt; qn += 64)
         {
             const int a = ol[qn], sz = ol[qn + 1] - a;
-            oc[2 * qn] = a;
-            oc[2 * qn + 1] = a + left_size(sz);
+            oc[2 * qn] = (short)a;
+            oc[2 * qn + 1] = (short)(a + left_size(sz));
         }
         if (lane == 0)
         {
@@ -624,8 +668,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
         // membership from the positional split of the frame array
         for (int i = lane; i < s; i += 64)
         {
-            int q = 0; // node of position i: last q with ol[q] <= i (nodes are few)
-            int lo = 0, hi = cnt - 1;
+            int lo = 0, hi = cnt - 1; // node of position i: last q with ol[q] <= i
             while (lo < hi)
             {
                 const int mid = (lo + hi + 1) >> 1;
@@ -634,8 +677,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
                 else
                     hi = mid - 1;
             }
-            q = lo;
-            const int a = ol[q], sz = ol[q + 1] - a;
+            const int a = ol[lo], sz = ol[lo + 1] - a;
             right[Af[i]] = (uint8_t)((i - a) >= left_size(sz) ? 1 : 0);
         }
         __syncthreads();
@@ -663,10 +705,10 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
             for (int r = 0; r < kCellPer; ++r)
             {
                 const int i = kCellPer * lane + r;
-                if (i < s) Z[i] = ex;
+                if (i < s) Z[i] = (short)ex;
                 ex += f[r];
             }
-            if (lane == 63) Z[s] = x;
+            if (lane == 63) Z[s] = (short)x;
         }
         __syncthreads();
         for (int i = lane; i < s; i += 64)
@@ -683,7 +725,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
             const int a = ol[lo], sz = ol[lo + 1] - a;
             const int v = Ao[i];
             const int z = Z[i] - Z[a];
-            T[right[v] ? a + left_size(sz) + (i - a - z) : a + z] = v;
+            T[right[v] ? a + left_size(sz) + (i - a - z) : a + z] = (short)v;
         }
         __syncthreads();
         for (int i = lane; i < s; i += 64) Ao[i] = T[i];
@@ -704,35 +746,44 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
             --tdl;
             tj = pj;
         }
-    LQE q;
+    Q q;
     q.init(LNX, LOR, P, LVE, 4 * 3 * s, s);
-    // leaves at sub-level L (absolute dl + L)
+    // leaves at sub-level L (absolute dl + L); their vertices are read
+    // before any record is written (the partition scratch shares the
+    // records' space)
     {
         const int cnt = 1 << L;
-        const int* ol = loff + lbase[L];
-        for (int jl = lane; jl < cnt; jl += 64)
+        const short* ol = loff + lbase[L];
+        const int jl = lane;
+        int v[3] = { 0, 0, 0 };
+        int sz = 0, a = 0, slot = 0;
+        if (jl < cnt)
         {
-            const int a = ol[jl], sz = ol[jl + 1] - a;
-            if (sz < 1) continue;
+            a = ol[jl];
+            sz = ol[jl + 1] - a;
             int db = L; // highest ancestor (sub-level) still below four points
             int jb = jl;
             while (db > 0)
             {
-                const int* op = loff + lbase[db - 1];
+                const short* op = loff + lbase[db - 1];
                 if (op[(jb >> 1) + 1] - op[jb >> 1] >= 4) break;
                 --db;
                 jb >>= 1;
             }
             const int abs_db = db > 0 ? dl + db : tdl;
             const int out_axis = abs_db > 0 ? ((abs_db - 1) & 1) : 0;
-            const int* B = out_axis ? A1 : A0;
-            int v[3];
-            for (int i = 0; i < sz; ++i) v[i] = B[a + i];
+            const short* B = out_axis ? A1 : A0;
+            for (int i = 0; i < sz && i < 3; ++i) v[i] = B[a + i];
+            slot = (1 << db) - 1 + jb;
+        }
+        if (cnt > 64 && lane == 0) bad = 1; // more leaves than lanes: cannot happen for cap <= 128
+        __syncthreads();
+        if (jl < cnt && sz >= 1)
+        {
             int first, second;
             Pool pool;
             q.leaf(v, sz, 3 * a, first, second, pool);
-            // node slot of (db, jb): sub-levels are numbered 2^l - 1 + index
-            lres[(1 << db) - 1 + jb] = make_short4((short)first, (short)second, (short)pool.head, (short)pool.tail);
+            lres[slot] = make_short4((short)first, (short)second, (short)pool.head, (short)pool.tail);
         }
     }
     __syncthreads();
@@ -740,7 +791,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
     for (int l = L - 1; l >= 0; --l)
     {
         const int cnt = 1 << l;
-        const int* ol = loff + lbase[l];
+        const short* ol = loff + lbase[l];
         for (int jl = lane; jl < cnt; jl += 64)
         {
             const int sz = ol[jl + 1] - ol[jl];
@@ -757,6 +808,8 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
         }
         __syncthreads();
     }
+    // a filter that could not decide: nothing is written, the caller lists the cell
+    if (__ballot(q.err == 5) != 0) return false;
     // write the records, vertex edges and the cell's result
     const int E0 = 12 * o; // local edge e is global E0 + e
     const int R0 = 3 * o;  // local record k is global R0 + k
@@ -773,10 +826,10 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
             continue;
         }
         for (int r = 0; r < 4; ++r) NX[4 * g + r] = E0 + LNX[4 * k + r];
-        OR[2 * g] = gid[o0];
-        OR[2 * g + 1] = gid[o1];
+        OR[2 * g] = (int)B0[o + o0];
+        OR[2 * g + 1] = (int)B0[o + o1];
     }
-    for (int i = lane; i < s; i += 64) VE[gid[i]] = LVE[i] >= 0 ? E0 + LVE[i] : -1;
+    for (int i = lane; i < s; i += 64) VE[B0[o + i]] = LVE[i] >= 0 ? E0 + LVE[i] : -1;
     if (lane == 0)
     {
         const short4 r = lres[0];
@@ -786,6 +839,37 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
     }
     flush_counts(counters, q.pc);
     if (q.err) atomicOr(err, 2 << q.err);
+    return true;
+}
+
+// every cell, filters only; undecided cells go to redo (count, then ids)
+__global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
+                                             const uint32_t* B1, const int* off, int dc, int D, int cap, int4* res,
+                                             int* redo, int* err, unsigned long long* counters)
+{
+    extern __shared__ __align__(16) unsigned char cell_lds[];
+    if (!cell_build<LQE>(cell_lds, cap, blockIdx.x, NX, OR, PT, VE, B0, B1, off, dc, D, res, err, counters) &&
+        threadIdx.x == 0)
+    {
+        const int k = atomicAdd(redo, 1);
+        redo[1 + k] = (int)blockIdx.x;
+    }
+}
+
+// the listed cells again, with the exact predicates
+__global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
+                                                   const uint32_t* B1, const int* off, int dc, int D, int cap,
+                                                   int4* res, const int* redo, int* err,
+                                                   unsigned long long* counters)
+{
+    extern __shared__ __align__(16) unsigned char cell_lds[];
+    const int n = redo[0];
+    for (int i = blockIdx.x; i < n; i += gridDim.x)
+    {
+        cell_build<LQX>(cell_lds, cap, (uint32_t)redo[1 + i], NX, OR, PT, VE, B0, B1, off, dc, D, res, err,
+                        counters);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1725,7 +1809,7 @@ struct Engine::Bufs
     // node-proportional (ensure_nodes)
     int* off; // level table, then every level's node offsets
     int4* res;
-    int *splitv, *qv, *kn, *lvlK, *segstart, *nseg, *segresume;
+    int *splitv, *qv, *kn, *lvlK, *redo, *segstart, *nseg, *segresume;
     int *arec, *astate, *aedge, *selk, *nsel, *alist, *alist_n;
     double4* acoord;
     double* akey;
@@ -1961,7 +2045,7 @@ Status Engine::ensure_nodes(const Plan& P)
     const size_t qv_nodes = std::max(P.qv_nodes, b.qv_nodes);
     const Item items[] = {
         { (void**)&b.off, 4 * (kLvlHdr + offs) }, { (void**)&b.res, 16 * nodes }, { (void**)&b.splitv, 4 * nodes },
-        { (void**)&b.kn, 4 * nodes }, { (void**)&b.lvlK, 4 * kMaxLevels },
+        { (void**)&b.kn, 4 * nodes }, { (void**)&b.lvlK, 4 * kMaxLevels }, { (void**)&b.redo, 4 * (nodes + 1) },
         { (void**)&b.qv, 4 * (size_t)kSegMaxK * qv_nodes },
     };
     const Status st = carve(items, ws_nodes_, ws_nodes_bytes_);
@@ -2436,9 +2520,17 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         st_.block_level = (uint32_t)dc;
         st_.cells = ncell;
         st_.cell_points = m_forest;
+        QDT_HIP(hipMemsetAsync(b.redo, 0, 4, s));
         if (ncell)
-            k_cell<<<ncell, 64, 0, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, dc, D, m, lim_e, b.res, b.err,
-                                        b.counters);
+        {
+            int cap = 1;
+            for (int d = dc; d <= D; ++d) cap = std::max(cap, (int)P.smax[d]);
+            const size_t lds = cell_lds_bytes(cap);
+            k_cell<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, dc, D, cap, b.res, b.redo, b.err,
+                                          b.counters);
+            k_cell_exact<<<std::min<uint32_t>(ncell, 1024), 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, dc,
+                                                                         D, cap, b.res, b.redo, b.err, b.counters);
+        }
     }
     else
     {
@@ -2518,10 +2610,12 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbx, 4, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipMemcpyAsync(hs + 1, b.err, 4, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipMemcpyAsync(hs + 2, b.spare, 4, hipMemcpyDeviceToHost, s));
+    if (use_cells) QDT_HIP(hipMemcpyAsync(hs + 3, b.redo, 4, hipMemcpyDeviceToHost, s));
     unsigned long long cnts[16];
     QDT_HIP(hipMemcpyAsync(cnts, b.counters, 8 * 16, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
     st_.spare_records = hs[2];
+    st_.cells_exact = use_cells ? hs[3] : 0;
     if (hs[1] & kSeamErr)
     {
         for (int i = 0; i < 8; ++i) st_.seam[i] = cnts[5 + i];

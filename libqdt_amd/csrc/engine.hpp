@@ -41,6 +41,7 @@ struct RunStats
     uint64_t seam[8];
     uint32_t err_bits;
     uint64_t spare_records; // records drawn from the spare region (demand, may exceed it)
+    uint64_t cells_exact;   // cells rebuilt with the exact predicates (a filter could not decide)
 };
 
 enum class Status : int

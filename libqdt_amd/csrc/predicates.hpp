@@ -401,23 +401,20 @@ QDT_HD_NOINLINE int incircle_exact(double ax, double ay, double bx, double by, d
     return incircle_general(hi, lo);
 }
 
-// Filtered predicates.  The return value's sign is the answer.  *exact is set
-// when the filter could not decide (used for the filter-failure counters).
-QDT_HD inline double orient2d(double ax, double ay, double bx, double by, double cx, double cy,
-                                           unsigned& nexact)
+// Filter stage: true when the FP64 determinant det has the sign of the exact
+// one (robust.hxx:281-289 bounds).
+QDT_HD inline bool orient2d_filter(double ax, double ay, double bx, double by, double cx, double cy, double& det)
 {
     const double detleft = (ax - cx) * (by - cy);
     const double detright = (ay - cy) * (bx - cx);
-    const double det = detleft - detright;
+    det = detleft - detright;
     const double mag = fabs(detleft) + fabs(detright);
     const double bound = kErrOrient * mag;
-    if ((det > bound || -det > bound) && mag > kFilterFloor) return det;
-    ++nexact;
-    return (double)orient2d_exact(ax, ay, bx, by, cx, cy);
+    return (det > bound || -det > bound) && mag > kFilterFloor;
 }
 
-QDT_HD inline double incircle(double ax, double ay, double bx, double by, double cx, double cy,
-                                           double dx, double dy, unsigned& nexact)
+QDT_HD inline bool incircle_filter(double ax, double ay, double bx, double by, double cx, double cy, double dx,
+                                   double dy, double& det)
 {
     const double adx = ax - dx, ady = ay - dy;
     const double bdx = bx - dx, bdy = by - dy;
@@ -428,11 +425,29 @@ QDT_HD inline double incircle(double ax, double ay, double bx, double by, double
     const double alift = adx * adx + ady * ady;
     const double blift = bdx * bdx + bdy * bdy;
     const double clift = cdx * cdx + cdy * cdy;
-    const double det = alift * (bdxcdy - cdxbdy) + blift * (cdxady - adxcdy) + clift * (adxbdy - bdxady);
+    det = alift * (bdxcdy - cdxbdy) + blift * (cdxady - adxcdy) + clift * (adxbdy - bdxady);
     const double permanent = (fabs(bdxcdy) + fabs(cdxbdy)) * alift + (fabs(cdxady) + fabs(adxcdy)) * blift +
         (fabs(adxbdy) + fabs(bdxady)) * clift;
     const double bound = kErrIncircle * permanent;
-    if ((det > bound || -det > bound) && permanent > kFilterFloor) return det;
+    return (det > bound || -det > bound) && permanent > kFilterFloor;
+}
+
+// Filtered predicates.  The return value's sign is the answer.  nexact counts
+// the calls the filter could not decide.
+QDT_HD inline double orient2d(double ax, double ay, double bx, double by, double cx, double cy,
+                                           unsigned& nexact)
+{
+    double det;
+    if (orient2d_filter(ax, ay, bx, by, cx, cy, det)) return det;
+    ++nexact;
+    return (double)orient2d_exact(ax, ay, bx, by, cx, cy);
+}
+
+QDT_HD inline double incircle(double ax, double ay, double bx, double by, double cx, double cy,
+                                           double dx, double dy, unsigned& nexact)
+{
+    double det;
+    if (incircle_filter(ax, ay, bx, by, cx, cy, dx, dy, det)) return det;
     ++nexact;
     return (double)incircle_exact(ax, ay, bx, by, cx, cy, dx, dy);
 }

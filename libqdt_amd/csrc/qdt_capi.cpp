@@ -232,6 +232,7 @@ void copy_stats(const qdt::RunStats& s, qdt_gpu_stats& o)
     o.cells = s.cells;
     o.cell_level = s.block_level;
     o.spare_records = s.spare_records;
+    o.cells_exact = s.cells_exact;
 }
 
 void record_stats(qdt_t* Q)

@@ -49,7 +49,9 @@ struct PredCount
 };
 
 // Error codes (`err`): 1 index out of range, 2 no free record, 3 loop
-// budget exhausted, 4 a seam segment ended before its stop edge.  Every access is bounds-checked (a compare next to a
+// budget exhausted, 4 a seam segment ended before its stop edge, 5 a
+// predicate's filter could not decide in an instance without the exact stage
+// (its caller redoes the work with one that has it).  Every access is bounds-checked (a compare next to a
 // dependent load costs nothing measurable) and every loop has a budget: a
 // logic error unwinds instead of touching memory outside the store or
 // spinning forever.
@@ -57,13 +59,15 @@ struct PredCount
 // out of range cannot fault (the hardware discards such writes and returns 0
 // for such reads) and the checks cost a quarter of the merge instructions.
 // The loop budgets stay, so a damaged store still cannot spin forever.
-template <class IntPtr, class PtPtr, bool kChecked = true>
+// kExact = false leaves out the exact predicate stage (and with it the calls
+// that raise every kernel's register count): an undecided filter sets err 5.
+template <class IntPtr, class PtPtr, bool kChecked = true, bool kExact = true>
 struct QuadEdges
 {
     IntPtr NX;
     IntPtr OR;
     PtPtr PT;
-    int* VE;            // vertex -> an edge out of it (may be null)
+    IntPtr VE;          // vertex -> an edge out of it (may be null)
     int lim_e;          // directed edge ids are in [0, lim_e)
     int lim_v;          // vertex ids are in [0, lim_v)
     unsigned* spare;    // spare record counter (null: no spare region)
@@ -76,7 +80,7 @@ struct QuadEdges
     int err;
     PredCount pc;
 
-    __device__ __forceinline__ void init(IntPtr nx, IntPtr orr, PtPtr pt, int* ve, int le, int lv)
+    __device__ __forceinline__ void init(IntPtr nx, IntPtr orr, PtPtr pt, IntPtr ve, int le, int lv)
     {
         NX = nx;
         OR = orr;
@@ -239,12 +243,23 @@ struct QuadEdges
         push(p, chk(e) >> 2);
     }
 
-    __device__ __forceinline__ double orient(int a, int b, int c)
+    __device__ __forceinline__ double pred_orient(double2 A, double2 B, double2 C)
     {
-        const double2 A = pt(a), B = pt(b), C = pt(c);
         ++pc.orient;
-        return orient2d(A.x, A.y, B.x, B.y, C.x, C.y, pc.orient_exact);
+        if constexpr (kExact) return orient2d(A.x, A.y, B.x, B.y, C.x, C.y, pc.orient_exact);
+        double det;
+        if (!orient2d_filter(A.x, A.y, B.x, B.y, C.x, C.y, det)) err = err ? err : 5;
+        return det;
     }
+    __device__ __forceinline__ double pred_incircle(double2 A, double2 B, double2 C, double2 D)
+    {
+        ++pc.incircle;
+        if constexpr (kExact) return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact);
+        double det;
+        if (!incircle_filter(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, det)) err = err ? err : 5;
+        return det;
+    }
+    __device__ __forceinline__ double orient(int a, int b, int c) { return pred_orient(pt(a), pt(b), pt(c)); }
     __device__ __forceinline__ bool ccw(int a, int b, int c) { return orient(a, b, c) > 0; }
     __device__ __forceinline__ bool right_of(int x, int e) { return ccw(x, dest(e), org(e)); }
     __device__ __forceinline__ bool left_of(int x, int e) { return ccw(x, org(e), dest(e)); }
@@ -253,28 +268,19 @@ struct QuadEdges
     __device__ __forceinline__ bool in_circle(int a, int b, int c, int d)
     {
         if (a == d || b == d || c == d || a == b || a == c || b == c) return false;
-        const double2 A = pt(a), B = pt(b), C = pt(c), D = pt(d);
-        ++pc.incircle;
-        return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact) > 0;
+        return pred_incircle(pt(a), pt(b), pt(c), pt(d)) > 0;
     }
     // the same tests on coordinates already loaded
-    __device__ __forceinline__ bool ccw_p(double2 A, double2 B, double2 C)
-    {
-        ++pc.orient;
-        return orient2d(A.x, A.y, B.x, B.y, C.x, C.y, pc.orient_exact) > 0;
-    }
+    __device__ __forceinline__ bool ccw_p(double2 A, double2 B, double2 C) { return pred_orient(A, B, C) > 0; }
     __device__ __forceinline__ bool in_circle_p(int a, int b, int c, int d, double2 A, double2 B, double2 C,
                                                 double2 D)
     {
         if (a == d || b == d || c == d || a == b || a == c || b == c) return false;
-        ++pc.incircle;
-        return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact) > 0;
+        return pred_incircle(A, B, C, D) > 0;
     }
     __device__ __forceinline__ double incircle_v(int a, int b, int c, int d)
     {
-        const double2 A = pt(a), B = pt(b), C = pt(c), D = pt(d);
-        ++pc.incircle;
-        return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact);
+        return pred_incircle(pt(a), pt(b), pt(c), pt(d));
     }
 
     // frame order of builder.hxx:71-76

@@ -105,6 +105,8 @@ def test_degenerate_mix(gpu):
     p = W.lattice_dups_circle(side=120, ndup=3000, ncircle=5000, seed=2)
     st = check_same("lattice+dups+circle small", p)
     assert st["incircle_exact"] > 0
+    # lattice cells have undecided filters: they go through the exact cell pass
+    assert st["cells_exact"] > 0
 
 
 def test_device_input(gpu):
