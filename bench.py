@@ -13,7 +13,9 @@ Extra fields:
                 frame orders; writes: three 24 B quad-edge records and a 4 B
                 vertex-edge entry) + 16 B per cell (its result), over the
                 launch's duration measured with HIP events on the engine's
-                stream around the k_cell launch of every timed step.  The
+                stream around the cell stage of every timed step (k_cell,
+                which runs the floating-point filters alone, and the exact
+                pass over the cells it lists: none on this input).  The
                 k_merge launches (levels above the cells merged with one
                 thread per node) are reported beside it.
   cpu_baseline  the reference's threaded qdt_execute_parallel, compiled from
