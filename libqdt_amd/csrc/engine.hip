@@ -879,9 +879,7 @@ __global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const doubl
 constexpr int kSegMaxK = 256;          // anchors per node at most (a power of two)
 constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
 constexpr int kSegMin = 1024;          // smaller nodes merge in one piece
-constexpr int kSegBlock = 64;          // pieces per block
 constexpr int kPhases = 2;             // pieces k, k+2, ... run together
-constexpr int kClaimBits = 20;         // piece id bits of a claim (the rest: launch epoch)
 constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit merges
 
 // Anchors for a node of s points whose extent across the cut is w and along
@@ -1449,37 +1447,59 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_compact(int d, const int* selk
     if (k == 0) nseg[j] = 1 + (int)total;
 }
 
-// One thread per piece of one phase: the rising bubble from cut i to cut
-// i+1.  Pieces of one phase share no cut vertex; a piece that needs a vertex
-// another piece holds stops and leaves its resume edge in segresume.
-__global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const double2* PT, int* VE, const int* off,
-                                                      int d, uint32_t m, int lim_e, unsigned* spare, int spare_base,
-                                                      int spare_cap, const int* segstart, const int* nseg,
-                                                      int parity, int hp, int epoch, int* claim, int2* segpool,
-                                                      int* segresume, int* err, unsigned long long* counters)
+// The pieces of a level: the rising bubble from cut i to cut i+1, in two
+// phases (even pieces, then odd: pieces of one phase share no cut vertex).
+// Every piece of a node runs in the same workgroup (tpn threads per node,
+// npb nodes per workgroup), so the phases are separated by a workgroup
+// barrier and the vertex claims live in an LDS hash table (cleared between
+// phases): a vertex belongs to one node of the level, and only that node's
+// pieces claim it.
+// A piece that needs a vertex another piece of its phase holds stops and
+// leaves its resume edge in segresume; after both phases one thread per node
+// finishes the stopped pieces bottom to top, with nothing else running on
+// the node (the bubble is a pure function of the store, so a resumed piece
+// does exactly what an uninterrupted one would).
+constexpr int kSegPieceBlock = 256; // threads per workgroup at most
+
+__global__ __launch_bounds__(kSegPieceBlock) void k_seg_pieces(int* NX, int* OR, const double2* PT, int* VE,
+                                                              const int* off, int d, uint32_t m, int lim_e,
+                                                              unsigned* spare, int spare_base, int spare_cap,
+                                                              const int* segstart, const int* nseg, int tpn,
+                                                              int npb, int claim_entries, int2* segpool,
+                                                              int* segresume, int* err,
+                                                              unsigned long long* counters)
 {
-    // thread (node j, h) runs piece i = kPhases * h + parity; hp pieces per
-    // node and phase at most; the claim id of the piece is the thread index
-    const uint32_t tid = blockIdx.x * kSegBlock + threadIdx.x;
+    __shared__ int nstop[kSegPieceBlock];
+    extern __shared__ unsigned long long claim_table[];
+    const int ln = threadIdx.x / tpn, h = threadIdx.x % tpn;
+    const uint32_t j = blockIdx.x * (uint32_t)npb + (uint32_t)ln;
+    const bool node = ln < npb && j < lcount(off, d);
+    int ns = 0, s = 0;
+    const int* st = nullptr;
+    if (node)
+    {
+        const int* od = off + off_base(off, d);
+        s = od[j + 1] - od[j];
+        ns = nseg[j];
+        st = segstart + (size_t)j * kSegSlots;
+    }
+    if (h == 0) nstop[ln] = 0;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
     q.spare = spare;
     q.spare_base = spare_base;
     q.spare_cap = spare_cap;
-    const uint32_t j = tid / hp;
-    const int i = kPhases * (int)(tid % hp) + parity;
-    const size_t t = (size_t)j * kSegSlots + i;
-    if (j < lcount(off, d))
+    for (int parity = 0; parity < kPhases; ++parity)
     {
-        const int ns = nseg[j];
-        if (i < ns)
+        for (int x = threadIdx.x; x < claim_entries; x += blockDim.x) claim_table[x] = kNoClaim;
+        __syncthreads();
+        for (int i = kPhases * h + parity; node && i < ns; i += kPhases * tpn)
         {
-            q.claim = claim;
-            q.claim_base = epoch << kClaimBits;
-            q.claim_me = q.claim_base | (int)tid;
-            const int* od = off + off_base(off, d);
-            const int s = od[j + 1] - od[j];
-            const int* st = segstart + (size_t)j * kSegSlots;
+            q.claim = claim_table;
+            q.claim_mask = claim_entries - 1;
+            q.claim_me = i;
+            q.h0 = q.h1 = q.h2 = q.h3 = -1;
+            const size_t t = (size_t)j * kSegSlots + i;
             int basel = st[i];
             const int stop = i + 1 < ns ? st[i + 1] : -1;
             Pool pool = { -1, -1 };
@@ -1489,6 +1509,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const d
             if (r == 2 && !q.err)
             {
                 segresume[t] = basel;
+                atomicAdd(&nstop[ln], 1);
                 atomicAdd(&counters[9], 1ull);
             }
             else
@@ -1498,69 +1519,28 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_run(int* NX, int* OR, const d
             }
             segpool[t] = make_int2(pool.head, pool.tail);
         }
+        __syncthreads();
+    }
+    if (node && h == 0 && nstop[ln] > 0)
+    {
+        // stopped pieces, bottom to top, with claims off
+        q.claim = nullptr;
+        int budget = 16 * s + 64;
+        for (int i = 0; i < ns && !q.err; ++i)
+        {
+            const size_t t = (size_t)j * kSegSlots + i;
+            int basel = segresume[t];
+            if (basel < 0) continue;
+            const int stop = i + 1 < ns ? st[i + 1] : -1;
+            const int2 sp = segpool[t];
+            Pool pool = { sp.x, sp.y };
+            const int r = q.bubble(basel, stop, pool, budget);
+            if (stop >= 0 && r != 1 && !q.err) q.err = 4;
+            segpool[t] = make_int2(pool.head, pool.tail);
+        }
+        atomicAdd(&counters[8], (unsigned long long)nstop[ln]);
     }
     flush_counts(counters, q.pc);
-    if (q.err) atomicOr(err, q.err >= 4 ? kSeamErr : (2 << q.err));
-}
-
-// one workgroup per node: its threads find the stopped pieces, one thread
-// finishes them bottom to top with no other piece of the node running
-__global__ __launch_bounds__(kSegMaxK) void k_seg_resume(int* NX, int* OR, const double2* PT, int* VE,
-                                                        const int* off, int d, uint32_t m, int lim_e,
-                                                        unsigned* spare, int spare_base, int spare_cap,
-                                                        const int* segstart, const int* nseg, int2* segpool,
-                                                        const int* segresume, int* err,
-                                                        unsigned long long* counters)
-{
-    __shared__ int stopped[kSegSlots];
-    __shared__ int nstop;
-    const uint32_t j = blockIdx.x;
-    const int k = threadIdx.x;
-    if (j >= lcount(off, d)) return;
-    const int ns = nseg[j];
-    if (k == 0) nstop = 0;
-    __syncthreads();
-    for (int i = k; i < ns; i += kSegMaxK)
-        if (segresume[(size_t)j * kSegSlots + i] >= 0) stopped[atomicAdd(&nstop, 1)] = i;
-    __syncthreads();
-    if (k != 0 || nstop == 0) return;
-    // bottom to top
-    for (int a = 1; a < nstop; ++a)
-    {
-        const int x = stopped[a];
-        int b = a;
-        while (b > 0 && stopped[b - 1] > x)
-        {
-            stopped[b] = stopped[b - 1];
-            --b;
-        }
-        stopped[b] = x;
-    }
-    GQE q;
-    q.init(NX, OR, PT, VE, lim_e, (int)m);
-    q.spare = spare;
-    q.spare_base = spare_base;
-    q.spare_cap = spare_cap;
-    const int* od = off + off_base(off, d);
-    const int s = od[j + 1] - od[j];
-    int budget = 16 * s + 64;
-    for (int a = 0; a < nstop && !q.err; ++a)
-    {
-        const int i = stopped[a];
-        const size_t t = (size_t)j * kSegSlots + i;
-        int basel = segresume[t];
-        const int stop = i + 1 < ns ? segstart[(size_t)j * kSegSlots + i + 1] : -1;
-        const int2 sp = segpool[t];
-        Pool pool = { sp.x, sp.y };
-        const int r = q.bubble(basel, stop, pool, budget);
-        if (stop >= 0 && r != 1 && !q.err) q.err = 4;
-        segpool[t] = make_int2(pool.head, pool.tail);
-    }
-    atomicAdd(&counters[8], (unsigned long long)nstop);
-    atomicAdd(&counters[0], (unsigned long long)q.pc.orient);
-    atomicAdd(&counters[1], (unsigned long long)q.pc.orient_exact);
-    atomicAdd(&counters[2], (unsigned long long)q.pc.incircle);
-    atomicAdd(&counters[3], (unsigned long long)q.pc.incircle_exact);
     if (q.err) atomicOr(err, q.err >= 4 ? kSeamErr : (2 << q.err));
 }
 
@@ -1802,7 +1782,7 @@ struct Engine::Bufs
     int *NX, *OR;
     unsigned long long* counters;
     int* err;
-    int *VE, *claim, *owner;
+    int *VE, *owner;
     unsigned* spare;
     size_t rec_cap; // quad-edge records allocated (3N + spare)
     uint32_t qcap;
@@ -1882,10 +1862,8 @@ void make_plan(Engine::Plan& P, const std::vector<uint32_t>& sizes, const std::v
     for (int d = 0; d < D; ++d)
     {
         // candidate levels (the nodes report their own anchor counts during
-        // the kd phase); the pieces of one phase must have distinct claim ids
-        const int K = seams ? seam_anchors_max(P.smax[d]) : 0;
-        const bool fits = (size_t)P.count[d] * (size_t)((K + kPhases) / kPhases) < ((size_t)1 << kClaimBits);
-        P.K[d] = fits ? K : 0;
+        // the kd phase)
+        P.K[d] = seams ? seam_anchors_max(P.smax[d]) : 0;
         if (!P.K[d]) continue;
         P.seg_nodes = std::max(P.seg_nodes, (size_t)P.count[d]);
         P.qv_nodes = std::max(P.qv_nodes, (size_t)P.nodeb[d] + (size_t)P.count[d]);
@@ -2021,7 +1999,7 @@ Status Engine::ensure(size_t n)
         { (void**)&b.vrank, 4 * N },    { (void**)&b.rin, 4 * N },
         { (void**)&b.P0, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.NX, 4 * 4 * R },
         { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
-        { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },    { (void**)&b.claim, 4 * N },
+        { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },
         { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },
     };
     const Status st = carve(items, ws_, ws_bytes_);
@@ -2405,7 +2383,6 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     mark(3);
     QDT_HIP(hipMemsetAsync(b.VE, 0xff, 4 * (size_t)m, s));
     QDT_HIP(hipMemsetAsync(b.owner, 0xff, 4 * (size_t)m, s));
-    QDT_HIP(hipMemsetAsync(b.claim, 0xff, 4 * (size_t)m, s));
     // the face flags of the extraction double as the deletion marks of the
     // seam anchors (-1 = kept) until the merges end
     QDT_HIP(hipMemsetAsync(b.face, 0xff, 4 * b.rec_cap, s));
@@ -2539,7 +2516,6 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                                                                  lim_e, b.res, b.err, b.counters);
     }
     mark(6);
-    int epoch = 0; // claim epochs: one per piece launch
     uint64_t kmerge_levels = 0;
     for (int d = Dk - 1; d >= 0; --d)
     {
@@ -2577,15 +2553,22 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                                                    b.apool);
             k_seg_compact<<<cnt, kSegMaxK, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge, b.segstart, b.nseg,
                                                    b.off);
-            for (int parity = 0; parity < kPhases; ++parity)
-                k_seg_run<<<cdiv((size_t)cnt * ((K + kPhases) / kPhases), kSegBlock), kSegBlock, 0, s>>>(
+            {
+                // threads per node: the pieces of one phase, rounded up to a
+                // power of two; nodes per workgroup: to fill a wavefront
+                const int hp = (K + kPhases) / kPhases;
+                int tpn = 1;
+                while (tpn < hp) tpn <<= 1;
+                tpn = std::min(tpn, kSegPieceBlock);
+                const int npb = std::max(1, 64 / tpn);
+                const int threads = std::max(64, tpn);
+                // claim table: ~48 vertices per piece of a phase, at most 64 KB
+                int entries = 256;
+                while (entries < 48 * npb * hp && entries < 8192) entries <<= 1;
+                k_seg_pieces<<<cdiv(cnt, (uint32_t)npb), threads, 8 * (size_t)entries, s>>>(
                     b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.spare, spare_base, spare_cap, b.segstart, b.nseg,
-                    parity, (K + kPhases) / kPhases, epoch++, b.claim, b.segpool, b.segresume, b.err,
-                    b.counters);
-            k_seg_resume<<<cnt, kSegMaxK, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
-                                                                         b.spare, spare_base, spare_cap, b.segstart,
-                                                                         b.nseg, b.segpool, b.segresume, b.err,
-                                                                         b.counters);
+                    tpn, npb, entries, b.segpool, b.segresume, b.err, b.counters);
+            }
             k_seg_final<<<cnt, kSegMaxK, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m,
                                                                         lim_e, b.nseg, b.nstate, b.segpool,
                                                                         b.astate, b.apool, b.res, b.err, b.counters);

@@ -21,11 +21,12 @@
 //
 // Claims.  Concurrent segments must never touch the ring of one vertex at
 // the same time.  A segment claims every vertex whose ring it is about to
-// read or change (claim[v] = its id, ids of one launch >= claim_base, larger
-// than every id of earlier launches, so old claims expire by themselves).
-// A segment that finds a vertex held by another one stops before touching
-// it; the operation sequence is a pure function of the store, so the
-// segment can be resumed later from the cross edge where it stopped.
+// read or change.  All segments that can meet at a vertex run in one
+// workgroup, so the claims live in an LDS hash table of (vertex, segment)
+// words, cleared between phases.  A segment that finds a vertex held by
+// another one (or the table full) stops before touching it; the operation
+// sequence is a pure function of the store, so the segment can be resumed
+// later from the cross edge where it stopped.
 #pragma once
 
 #include "predicates.hpp"
@@ -36,6 +37,8 @@ namespace qdt
 __device__ __forceinline__ int e_rot(int e) { return (e & ~3) | ((e + 1) & 3); }
 __device__ __forceinline__ int e_invrot(int e) { return (e & ~3) | ((e + 3) & 3); }
 __device__ __forceinline__ int e_sym(int e) { return e ^ 2; }
+
+constexpr unsigned long long kNoClaim = ~0ull; // empty claim-table entry
 
 struct Pool
 {
@@ -73,8 +76,9 @@ struct QuadEdges
     unsigned* spare;    // spare record counter (null: no spare region)
     int spare_base;     // first spare record
     int spare_cap;      // spare records available
-    int* claim;         // vertex claims (null: single owner, no claims)
-    int claim_me, claim_base;
+    unsigned long long* claim; // LDS claim table (null: single owner, no claims)
+    int claim_mask;            // table entries - 1 (a power of two)
+    int claim_me;
     int h0, h1, h2, h3; // recently claimed vertices (named registers: no
                         // indexed array, which would live in scratch memory)
     int err;
@@ -91,7 +95,8 @@ struct QuadEdges
         spare = nullptr;
         spare_base = spare_cap = 0;
         claim = nullptr;
-        claim_me = claim_base = 0;
+        claim_mask = 0;
+        claim_me = 0;
         h0 = h1 = h2 = h3 = -1;
         err = 0;
         pc = PredCount{ 0, 0, 0, 0, 0 };
@@ -139,13 +144,28 @@ struct QuadEdges
             err = err ? err : 1;
             return false;
         }
-        const int r = atomicMax(&claim[v], claim_me);
-        if (r >= claim_base && r != claim_me) return false;
-        h3 = h2;
-        h2 = h1;
-        h1 = h0;
-        h0 = v;
-        return true;
+        const unsigned long long mine = ((unsigned long long)(unsigned)v << 32) | (unsigned)claim_me;
+        unsigned x = ((unsigned)v * 2654435761u) & (unsigned)claim_mask;
+        for (int probe = 0; probe <= claim_mask; ++probe)
+        {
+            unsigned long long cur = claim[x];
+            if (cur == kNoClaim)
+            {
+                cur = atomicCAS(&claim[x], kNoClaim, mine);
+                if (cur == kNoClaim) cur = mine;
+            }
+            if ((unsigned)(cur >> 32) == (unsigned)v)
+            {
+                if (cur != mine) return false;
+                h3 = h2;
+                h2 = h1;
+                h1 = h0;
+                h0 = v;
+                return true;
+            }
+            x = (x + 1) & (unsigned)claim_mask;
+        }
+        return false; // table full
     }
 
     __device__ __forceinline__ int pop(Pool& p)

@@ -26,12 +26,13 @@
 //               straight walk (dry run; any degeneracy drops the anchor).
 //               Selected anchors (ordered, vertex-disjoint) are inserted in
 //               parallel by rewriting the ring of every vertex they touch.
-//  4. segments  one thread per piece, in two phases (even pieces, then odd):
-//               the rising bubble from anchor k stops when anchor k+1 is the
-//               next cross edge (QuadEdges::bubble).  A piece claims each
-//               vertex before touching its ring and stops at a vertex another
-//               piece holds; stopped pieces are finished afterwards, one
-//               thread per node.
+//  4. segments  one thread per piece, all pieces of a node in one
+//               workgroup, in two phases (even pieces, then odd, a barrier
+//               between): the rising bubble from anchor k stops when anchor
+//               k+1 is the next cross edge (QuadEdges::bubble).  A piece
+//               claims each vertex before touching its ring and stops at a
+//               vertex another piece holds; stopped pieces are finished
+//               afterwards by one thread per node.
 #pragma once
 
 #include "quadedge.hpp"
