@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
 // cell is the one the level kernels would have built.
 // ---------------------------------------------------------------------------
 
-constexpr int kCellMax = 128;             // points per cell at most
+constexpr int kCellMax = 256;             // points per cell at most
 constexpr int kCellLevels = 12;
 constexpr int kCellPer = (kCellMax + 63) / 64; // positions per lane in the cell's scans
 
@@ -748,19 +748,16 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
         }
     Q q;
     q.init(LNX, LOR, P, LVE, 4 * 3 * s, s);
-    // leaves at sub-level L (absolute dl + L); their vertices are read
-    // before any record is written (the partition scratch shares the
-    // records' space)
+    // leaves at sub-level L (absolute dl + L); the partition scratch that
+    // shares the records' space is dead by now (the loop above ends with a
+    // barrier after its last use)
     {
         const int cnt = 1 << L;
         const short* ol = loff + lbase[L];
-        const int jl = lane;
-        int v[3] = { 0, 0, 0 };
-        int sz = 0, a = 0, slot = 0;
-        if (jl < cnt)
+        for (int jl = lane; jl < cnt; jl += 64)
         {
-            a = ol[jl];
-            sz = ol[jl + 1] - a;
+            const int a = ol[jl], sz = ol[jl + 1] - a;
+            if (sz < 1) continue;
             int db = L; // highest ancestor (sub-level) still below four points
             int jb = jl;
             while (db > 0)
@@ -773,17 +770,12 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
             const int abs_db = db > 0 ? dl + db : tdl;
             const int out_axis = abs_db > 0 ? ((abs_db - 1) & 1) : 0;
             const short* B = out_axis ? A1 : A0;
+            int v[3] = { 0, 0, 0 };
             for (int i = 0; i < sz && i < 3; ++i) v[i] = B[a + i];
-            slot = (1 << db) - 1 + jb;
-        }
-        if (cnt > 64 && lane == 0) bad = 1; // more leaves than lanes: cannot happen for cap <= 128
-        __syncthreads();
-        if (jl < cnt && sz >= 1)
-        {
             int first, second;
             Pool pool;
             q.leaf(v, sz, 3 * a, first, second, pool);
-            lres[slot] = make_short4((short)first, (short)second, (short)pool.head, (short)pool.tail);
+            lres[(1 << db) - 1 + jb] = make_short4((short)first, (short)second, (short)pool.head, (short)pool.tail);
         }
     }
     __syncthreads();
