@@ -39,6 +39,7 @@ __device__ __forceinline__ int e_invrot(int e) { return (e & ~3) | ((e + 3) & 3)
 __device__ __forceinline__ int e_sym(int e) { return e ^ 2; }
 
 constexpr unsigned long long kNoClaim = ~0ull; // empty claim-table entry
+constexpr int kSpareChunk = 8;                 // spare records taken at once
 
 struct Pool
 {
@@ -175,8 +176,21 @@ struct QuadEdges
         {
             if (spare)
             {
-                const unsigned s = atomicAdd(spare, 1u);
-                if ((int)s < spare_cap) return spare_base + (int)s;
+                // kSpareChunk records per request (one device-wide atomic
+                // per chunk): the first is returned, the rest become the pool
+                const unsigned s = atomicAdd(spare, (unsigned)kSpareChunk);
+                if ((int)s + kSpareChunk <= spare_cap)
+                {
+                    const int r0 = spare_base + (int)s;
+                    for (int i = 1; i < kSpareChunk; ++i)
+                    {
+                        NX[4 * (r0 + i)] = i + 1 < kSpareChunk ? r0 + i + 1 : -1;
+                        OR[2 * (r0 + i)] = -1;
+                    }
+                    p.head = r0 + 1;
+                    p.tail = r0 + kSpareChunk - 1;
+                    return r0;
+                }
             }
             err = err ? err : 2;
             return 0;
