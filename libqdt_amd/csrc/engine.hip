@@ -69,6 +69,36 @@ __device__ __forceinline__ void flush_counts(unsigned long long* counters, const
     }
 }
 
+// Sum of v over the wavefront added to *c by one atomic (every lane of the
+// wavefront must call it; lanes with nothing to add pass 0).
+__device__ __forceinline__ void wave_add(unsigned long long* c, unsigned long long v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (__lane_id() == 0 && v) atomicAdd(c, v);
+}
+
+// Wavefront-aggregated fetch-and-add: one atomic per wavefront, each lane
+// gets the old value plus the sum of the lower lanes' v (the whole wavefront
+// must call it; lanes with nothing to take pass 0).
+template <class T>
+__device__ __forceinline__ T wave_fetch_add(T* c, T v)
+{
+    const int lane = (int)__lane_id();
+    T x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const T y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    const T total = __shfl(x, 63, 64);
+    T base = 0;
+    if (lane == 63 && total) base = atomicAdd(c, total);
+    base = __shfl(base, 63, 64);
+    return base + x - v;
+}
+
 // edges created by the merge kernel (for its algorithmic bytes)
 __device__ __forceinline__ void flush_connects(unsigned long long* counters, const PredCount& pc)
 {
@@ -970,6 +1000,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
     const uint32_t lane2 = blockIdx.x * kMergeBlock + threadIdx.x;
     const uint32_t t = lane2 >> 1; // anchor
     const int side = (int)(lane2 & 1);
+    unsigned long long certified = 0, dry_ok = 0;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
     if (t < lcount(off, d) * (uint32_t)K)
@@ -1026,7 +1057,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
             }
             if (ok)
             {
-                if (side == 0) atomicAdd(&counters[5], 1ull);
+                if (side == 0) certified = 1;
                 // dry run of the insertion on the unmerged halves: each lane
                 // finds its end's wedge and the crossings on its side
                 int* r = arec + (size_t)t * kRec;
@@ -1048,7 +1079,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
                         const double ux = frame_x(U, axis), uy = frame_y(U, axis);
                         const double vx = frame_x(V, axis), vy = frame_y(V, axis);
                         akey[t] = vx > ux ? uy + (c - ux) / (vx - ux) * (vy - uy) : 0.5 * (uy + vy);
-                        atomicAdd(&counters[10], 1ull);
+                        dry_ok = 1;
                     }
                     state = 1;
                 }
@@ -1056,6 +1087,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
         }
         if (side == 0) astate[t] = state;
     }
+    wave_add(&counters[5], certified);
+    wave_add(&counters[10], dry_ok);
     flush_counts(counters, q.pc);
 }
 
@@ -1234,30 +1267,40 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mark(int* OR, int total, co
                                                           unsigned long long* counters)
 {
     const int t = blockIdx.x * kMergeBlock + threadIdx.x;
-    if (t >= total || astate[t] != 2) return;
-    const int* r = arec + (size_t)t * kRec;
-    const unsigned k = atomicAdd(spare, 1u);
-    if ((int)k >= spare_cap)
+    const bool act = t < total && astate[t] == 2;
+    const int* r = arec + (size_t)(act ? t : 0) * kRec;
+    // one record per inserted anchor, one allocation per wavefront
+    const unsigned k = wave_fetch_add(spare, act ? 1u : 0u);
+    int e = -1;
+    if (act)
     {
-        atomicOr(err, kSeamErr);
-        aedge[t] = -1;
-        return;
+        if ((int)k >= spare_cap)
+        {
+            atomicOr(err, kSeamErr);
+            aedge[t] = -1;
+        }
+        else
+        {
+            e = 4 * (spare_base + (int)k); // e runs v -> u
+            OR[e >> 1] = r[1];
+            OR[(e ^ 2) >> 1] = r[0];
+            aedge[t] = e;
+            owner[r[0]] = t;
+            owner[r[1]] = t;
+        }
     }
-    const int e = 4 * (spare_base + (int)k); // e runs v -> u
-    OR[e >> 1] = r[1];
-    OR[(e ^ 2) >> 1] = r[0];
-    aedge[t] = e;
-    owner[r[0]] = t;
-    owner[r[1]] = t;
-    const int at = atomicAdd(alist_n, 2);
-    if (at + 2 <= acap)
+    const int at = wave_fetch_add(alist_n, e >= 0 ? 2 : 0);
+    if (e >= 0)
     {
-        alist[at] = r[0];
-        alist[at + 1] = r[1];
+        if (at + 2 <= acap)
+        {
+            alist[at] = r[0];
+            alist[at + 1] = r[1];
+        }
+        else
+            atomicOr(err, kSeamErr);
     }
-    else
-        atomicOr(err, kSeamErr);
-    atomicAdd(&counters[6], 1ull);
+    wave_add(&counters[6], e >= 0 ? 1ull : 0ull);
 }
 
 // crossed edges: deletion marks and the rings of their endpoints
@@ -1476,6 +1519,7 @@ __global__ __launch_bounds__(kSegPieceBlock) void k_seg_pieces(int* NX, int* OR,
         st = segstart + (size_t)j * kSegSlots;
     }
     if (h == 0) nstop[ln] = 0;
+    unsigned long long ran = 0, stopped = 0, resumed = 0;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
     q.spare = spare;
@@ -1497,12 +1541,12 @@ __global__ __launch_bounds__(kSegPieceBlock) void k_seg_pieces(int* NX, int* OR,
             Pool pool = { -1, -1 };
             int budget = 16 * s + 64;
             const int r = q.bubble(basel, stop, pool, budget);
-            atomicAdd(&counters[7], 1ull);
+            ++ran;
             if (r == 2 && !q.err)
             {
                 segresume[t] = basel;
                 atomicAdd(&nstop[ln], 1);
-                atomicAdd(&counters[9], 1ull);
+                ++stopped;
             }
             else
             {
@@ -1530,8 +1574,11 @@ __global__ __launch_bounds__(kSegPieceBlock) void k_seg_pieces(int* NX, int* OR,
             if (stop >= 0 && r != 1 && !q.err) q.err = 4;
             segpool[t] = make_int2(pool.head, pool.tail);
         }
-        atomicAdd(&counters[8], (unsigned long long)nstop[ln]);
+        resumed = (unsigned long long)nstop[ln];
     }
+    wave_add(&counters[7], ran);
+    wave_add(&counters[8], resumed);
+    wave_add(&counters[9], stopped);
     flush_counts(counters, q.pc);
     if (q.err) atomicOr(err, q.err >= 4 ? kSeamErr : (2 << q.err));
 }

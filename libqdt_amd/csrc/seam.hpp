@@ -171,21 +171,24 @@ __device__ int find_wedge(Q& q, int u, int e0, int p)
 {
     const int n1 = q.onext(e0);
     if (n1 == e0) return e0; // single edge: every other direction is inside
-    int e = e0, guard = 0;
+    const double2 U = q.pt(u), P = q.pt(p);
+    // the ring in order, each edge's far end and coordinates carried to the
+    // next step; the onext after that is requested before the tests
+    int e = e0, n = n1, guard = 0;
+    int a = q.dest(e0), b = q.dest(n1);
+    double2 A = q.pt(a), B = q.pt(b);
     do
     {
-        const int n = q.onext(e);
-        const int a = q.dest(e), b = q.dest(n);
-        const double oap = q.orient(u, a, p);
-        const double opb = q.orient(u, p, b);
-        if (oap == 0 && q.orient(u, a, b) != 0)
+        const int nn = q.onext(n);
+        const double oap = q.pred_orient(U, A, P);
+        const double opb = q.pred_orient(U, P, B);
+        if (oap == 0 && q.pred_orient(U, A, B) != 0)
         {
             // collinear with edge e: same direction is a coincidence, the
             // opposite direction lies in some wedge (handled below)
-            const double2 U = q.pt(u), A = q.pt(a), P = q.pt(p);
             if ((A.x - U.x) * (P.x - U.x) + (A.y - U.y) * (P.y - U.y) > 0) return -1;
         }
-        const double oab = q.orient(u, a, b);
+        const double oab = q.pred_orient(U, A, B);
         bool inside;
         if (oab > 0)
             inside = oap > 0 && opb > 0;
@@ -193,6 +196,11 @@ __device__ int find_wedge(Q& q, int u, int e0, int p)
             inside = oap > 0 || opb > 0; // reflex or straight wedge
         if (inside) return e;
         e = n;
+        a = b;
+        A = B;
+        n = nn;
+        b = q.dest(n);
+        B = q.pt(b);
     } while (e != e0 && ++guard < 256 && !q.err);
     return -1;
 }
@@ -237,38 +245,55 @@ template <class Q>
 __device__ int collect_crossings(Q& q, int u, int x, int p, int* out, int cap)
 {
     int n = 0;
-    if (q.onext(x) == x) return 0; // u has one edge: nothing around it to cross
-    // the face left of x, between x and onext(x)
-    const int k0 = face_kind(q, x);
-    if (k0 == 0) return 0;  // the segment leaves u into the outer face
-    if (k0 < 0) return -1;
-    int e;
+    const int ox = q.onext(x);
+    if (ox == x) return 0; // u has one edge: nothing around it to cross
+    const double2 U = q.pt(u), P = q.pt(p);
+    // the face left of x, between x and onext(x): a triangle (u, a, b)
+    int f = q.lnext(x);
+    int g = q.lnext(f);
+    if (q.lnext(g) != x)
+        return face_kind(q, x) == 0 ? 0 : -1; // the segment leaves u into the outer face, or a larger face
+    int a = q.org(f), b = q.org(g);
+    double2 A = q.pt(a), B = q.pt(b);
     {
-        const int f = q.lnext(x), g = q.lnext(f);
-        if (q.onext(x) != (g ^ 2)) return -1;
+        const double o = q.pred_orient(U, A, B);
+        if (o < 0) return 0;  // the outer face
+        if (o == 0) return -1;
+        if (ox != (g ^ 2)) return -1;
         // f runs a -> b opposite u; the segment leaves through it
-        const double o = q.orient(q.org(f), q.dest(f), p);
-        if (o >= 0) return -1; // p inside the triangle or on its edge
-        e = f;
+        if (q.pred_orient(A, B, P) >= 0) return -1; // p inside the triangle or on its edge
     }
+    int e = f; // runs a -> b with a right of u->p and b left of it
     for (int guard = 0; guard < cap + 4; ++guard)
     {
         if (n >= cap) return -1;
-        out[n++] = e; // e runs a -> b with a right of u->p and b left of it
-        const int h = e ^ 2; // b -> a
-        const int k = face_kind(q, h);
-        if (k == 0) return n; // crossed a hull edge: done
-        if (k < 0) return -1;
-        const int f = q.lnext(h), g = q.lnext(f);
-        const int c = q.org(g); // triangle (b, a, c)
-        const double oc = q.orient(u, p, c);
+        out[n++] = e;
+        // the face beyond e, left of h = b -> a: a triangle (b, a, c) with
+        // f = a -> c and g = c -> b, or the outer face (a hull edge: done)
+        const int h = e ^ 2;
+        f = q.lnext(h);
+        g = q.lnext(f);
+        if (q.lnext(g) != h) return face_kind(q, h) == 0 ? n : -1;
+        const int c = q.org(g);
+        const double2 C = q.pt(c);
+        const double o = q.pred_orient(B, A, C);
+        if (o < 0) return n; // the outer face
+        if (o == 0) return -1;
+        const double oc = q.pred_orient(U, P, C);
         if (oc == 0) return -1; // the segment passes through c
         if (oc > 0)
+        {
             e = f; // a -> c crosses (a right, c left)
+            b = c;
+            B = C;
+        }
         else
+        {
             e = g; // c -> b crosses (c right, b left)
-        const double op = q.orient(q.org(e), q.dest(e), p);
-        if (op >= 0) return -1; // p inside this triangle
+            a = c;
+            A = C;
+        }
+        if (q.pred_orient(A, B, P) >= 0) return -1; // p inside this triangle
     }
     return -1;
 }
