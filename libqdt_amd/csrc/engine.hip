@@ -163,7 +163,7 @@ struct StoreU32
 
 __global__ __launch_bounds__(kBlock) void k_dedupe_write(const double2* pts, const uint32_t* idx, DedupeFlag f,
                                                          const uint32_t* dist, uint32_t n, double2* P0,
-                                                         uint32_t* ID0, uint32_t* vrank)
+                                                         uint32_t* ID0, uint32_t* vrank, uint32_t* didx)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
@@ -174,8 +174,48 @@ __global__ __launch_bounds__(kBlock) void k_dedupe_write(const double2* pts, con
         P0[d] = pts[i];
         ID0[d] = i;
         if (f.kx) vrank[d] = (uint32_t)f.k_sorted[p];
+        if (didx) didx[i] = d;
     }
 }
+
+// After the sort by y (ties by input index): keeps that order, and flags a
+// y value held by two distinct points (the frame-1 order then needs the
+// x-descending tie rule, which this order does not give).
+__global__ __launch_bounds__(kBlock) void k_ytie(const uint64_t* ysorted, const uint32_t* idx, const uint64_t* kx,
+                                                 uint32_t n, uint32_t* perm, unsigned long long* tie)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t i = idx[p];
+    perm[p] = i;
+    if (p + 1 < n && ysorted[p] == ysorted[p + 1] && kx[i] != kx[idx[p + 1]]) *tie = 1;
+}
+
+// frame-1 order without a sort: the y order's representatives (a point's
+// lowest-index copy) in that order, as distinct ids
+struct RepFlag
+{
+    const uint32_t* perm;
+    const uint32_t* rep;
+    __device__ uint32_t operator()(uint32_t p) const
+    {
+        const uint32_t i = perm[p];
+        return rep[i] == i ? 1u : 0u;
+    }
+};
+
+struct RepWrite
+{
+    const uint32_t* perm;
+    const uint32_t* rep;
+    const uint32_t* didx;
+    uint32_t* out;
+    __device__ void operator()(uint32_t p, uint32_t pre) const
+    {
+        const uint32_t i = perm[p];
+        if (rep[i] == i) out[pre] = didx[i];
+    }
+};
 
 // ---- several point sets in one run ----
 
@@ -2332,6 +2372,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     QDT_HIP(hipMemcpyAsync(b.ky, b.kA, 8 * (size_t)n, hipMemcpyDeviceToDevice, s));
     Status st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
     if (st != Status::ok) return st;
+    // one set: keep the y order for frame 1 (counters[14]: a tie in y)
+    if (!multi) k_ytie<<<cdiv(n, kBlock), kBlock, 0, s>>>(kA, vA, b.kx, n, b.Bt, &b.counters[14]);
     if (!multi)
     {
         // non-finite input is reported before any further work
@@ -2358,7 +2400,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     scan_u32(df, n, StoreU32{ b.dist }, b.partials, s);
     const uint32_t nbs = cdiv(n, kScanTile);
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbs, 4, hipMemcpyDeviceToHost, s));
-    k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, vA, df, b.dist, n, b.P0, b.ID0, b.vrank);
+    k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, vA, df, b.dist, n, b.P0, b.ID0, b.vrank,
+                                                      multi ? nullptr : b.Z);
     k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, df, b.dist, n, b.ID0, b.rep);
     std::vector<uint32_t> vstart_h;
     if (multi)
@@ -2366,8 +2409,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         k_vstart<<<cdiv((size_t)k + 1, kBlock), kBlock, 0, s>>>(b.dist, b.rpos, k, n, b.partials + nbs, b.vstart);
         QDT_HIP(hipMemcpyAsync(b.host_sets, b.vstart, 4 * ((size_t)k + 1), hipMemcpyDeviceToHost, s));
     }
+    unsigned long long ytie = 1;
+    if (!multi) QDT_HIP(hipMemcpyAsync(hs + 4, &b.counters[14], 8, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
     const uint32_t m = hs[0];
+    if (!multi) memcpy(&ytie, hs + 4, 8);
     if (multi) vstart_h.assign(b.host_sets, b.host_sets + k + 1);
     res_.n = n;
     res_.rep = b.rep;
@@ -2431,19 +2477,26 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     const int lim_e = (int)(4 * b.rec_cap);
     const int spare_base = (int)(3 * (size_t)m);
     const int spare_cap = (int)(b.rec_cap - 3 * (size_t)m);
-    // 4. frame-1 order: (y asc, x desc) = stable y-sort of the reversed frame-0 order
-    k_frame1_keys<<<cdiv(m, kBlock), kBlock, 0, s>>>(b.P0, m, b.kA, b.vA);
+    // 4. frame-1 order: (y asc, x desc).  One set with no two distinct
+    // points at the same y: the representatives of the first sort's y order.
+    // Otherwise a stable y-sort of the reversed frame-0 order.
     kA = b.kA;
     kB = b.kB;
     vA = b.vA;
     vB = b.vB;
-    st = sort_pairs(kA, vA, kB, vB, m, b.hist, b.partials, b.ghist, hghist, s);
-    if (st != Status::ok) return st;
-    if (multi)
+    if (!multi && !ytie)
+        scan_u32(RepFlag{ b.Bt, b.rep }, n, RepWrite{ b.Bt, b.rep, b.Z, b.vA }, b.partials, s);
+    else
     {
-        k_gather_rank<<<cdiv(m, kBlock), kBlock, 0, s>>>(vA, b.vrank, m, kA);
+        k_frame1_keys<<<cdiv(m, kBlock), kBlock, 0, s>>>(b.P0, m, b.kA, b.vA);
         st = sort_pairs(kA, vA, kB, vB, m, b.hist, b.partials, b.ghist, hghist, s);
         if (st != Status::ok) return st;
+        if (multi)
+        {
+            k_gather_rank<<<cdiv(m, kBlock), kBlock, 0, s>>>(vA, b.vrank, m, kA);
+            st = sort_pairs(kA, vA, kB, vB, m, b.hist, b.partials, b.ghist, hghist, s);
+            if (st != Status::ok) return st;
+        }
     }
     uint32_t* B1 = vA;
     uint32_t* B0 = b.B0;
