@@ -451,17 +451,37 @@ __global__ __launch_bounds__(kBlock) void k_split(const uint32_t* Bf, const uint
     bits[Bf[p]] = (uint8_t)c;
 }
 
+// "goes left" of position p of the other frame's array: gathered once by
+// the scan's first pass and kept by position (F), which the second pass and
+// the partition read without the gather
 struct ZeroFlag
 {
     const uint32_t* B;
     const uint8_t* bits;
-    __device__ uint32_t operator()(uint32_t p) const { return bits[B[p]] ? 0u : 1u; }
+    uint8_t* F;
+    __device__ uint32_t operator()(uint32_t p) const
+    {
+        const uint32_t f = bits[B[p]] ? 0u : 1u;
+        F[p] = (uint8_t)f;
+        return f;
+    }
 };
 
-// stable two-way partition, within each node, of the array sorted in the other frame
+struct LoadU8
+{
+    const uint8_t* F;
+    __device__ uint32_t operator()(uint32_t p) const { return F[p]; }
+};
+
+// Stable two-way partition, within each node, of the array sorted in the
+// other frame (F[p]: position p goes left).  It also does the next level's
+// split: the node of every position one level down (node_next) and, as v
+// lands in its child, v's side of the child's cut (bits[v]; each v belongs
+// to one thread).  The next level's cut array is Bout, split by position, so
+// that side is whether v lands in the child's upper part.
 __global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const uint32_t* node, const int* off,
-                                                      int d, const uint8_t* bits, const uint32_t* Z, uint32_t m,
-                                                      uint32_t* Bout)
+                                                      int d, const uint8_t* F, uint8_t* bits, const uint32_t* Z,
+                                                      uint32_t m, uint32_t* Bout, uint32_t* node_next)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= m) return;
@@ -475,9 +495,14 @@ __global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const 
     const uint32_t j = node[p];
     const int o = od[j];
     const int s = od[j + 1] - o;
+    const int ls = left_size(s);
     const int z = (int)(Z[p] - Z[o]);
-    const int pos = bits[v] ? o + left_size(s) + ((int)p - o - z) : o + z;
+    const int b = F[p] ? 0 : 1;
+    const int pos = b ? o + ls + ((int)p - o - z) : o + z;
     Bout[pos] = v;
+    node_next[p] = 2 * j + (((int)p - o) >= ls ? 1u : 0u);
+    const int oc = b ? o + ls : o, sc = b ? s - ls : ls;
+    bits[v] = (uint8_t)((pos - oc) >= left_size(sc) ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -990,7 +1015,24 @@ __global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const 
     }
     kn[ni] = Kn;
     if (Kn) atomicMax(&lvlK[d], Kn);
-    for (int k = 0; k < Kn; ++k) qv[ni * kSegMaxK + k] = (int)Bo[o + (int)(((long long)(2 * k + 1) * s) / (2 * Kn))];
+}
+
+// anchor heights: the vertex at the middle of the k-th of Kn equal slices of
+// the node's order along the cut (one thread per (node, k))
+__global__ __launch_bounds__(kBlock) void k_node_qv(const uint32_t* Bo, const int* off, int d, int K, const int* kn,
+                                                    int* qv)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= lcount(off, d) * (uint32_t)K) return;
+    const uint32_t j = t / K;
+    const int k = (int)(t % K);
+    const int* od = off + off_base(off, d);
+    const int o = od[j];
+    const int s = od[j + 1] - o;
+    if (s < 4) return;
+    const size_t ni = node_index(off, d, j);
+    const int Kn = kn[ni];
+    if (k < Kn) qv[ni * kSegMaxK + k] = (int)Bo[o + (int)(((long long)(2 * k + 1) * s) / (2 * Kn))];
 }
 
 __device__ __forceinline__ bool less_frame_pt(double2 a, double2 b, int frame)
@@ -1820,6 +1862,17 @@ void scan_u32(In in, uint32_t n, Out out, uint32_t* partials, hipStream_t s)
     k_scan_down<<<nb, kBlock, 0, s>>>(in, n, partials, out);
 }
 
+// the same with separate inputs for the two passes (the first may leave its
+// values where the second reads them)
+template <class InReduce, class InDown, class Out>
+void scan_u32(InReduce in1, InDown in2, uint32_t n, Out out, uint32_t* partials, hipStream_t s)
+{
+    const uint32_t nb = cdiv(n, kScanTile);
+    k_scan_reduce<<<nb, kBlock, 0, s>>>(in1, n, partials);
+    k_scan_partials<<<1, 1024, 0, s>>>(partials, nb);
+    k_scan_down<<<nb, kBlock, 0, s>>>(in2, n, partials, out);
+}
+
 // Kd depth at which every node has at most three points.
 int kd_depth(uint32_t m)
 {
@@ -1857,7 +1910,7 @@ struct Engine::Bufs
     uint32_t *vA, *vB, *dist, *ID0, *rep, *B0, *B1, *Bt, *nodeA, *nodeB, *Z, *tris, *face, *queue;
     uint32_t *partials, *hist, *ghist, *vrank, *rin, *one_set;
     double2* P0;
-    uint8_t* bits;
+    uint8_t *bits, *flagp;
     int *NX, *OR;
     unsigned long long* counters;
     int* err;
@@ -2076,7 +2129,7 @@ Status Engine::ensure(size_t n)
         { (void**)&b.hist, 4 * 256 * (size_t)nb_sort },
         { (void**)&b.ghist, 4 * 8 * 256 },
         { (void**)&b.vrank, 4 * N },    { (void**)&b.rin, 4 * N },
-        { (void**)&b.P0, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.NX, 4 * 4 * R },
+        { (void**)&b.P0, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.flagp, N },         { (void**)&b.NX, 4 * 4 * R },
         { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
         { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },
         { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },
@@ -2546,9 +2599,15 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             k_children<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(b.off, d);
             k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, b.off, d, P.K[d], b.P0, b.splitv, b.qv,
                                                               b.kn, b.lvlK);
-            k_split<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bf, nodeA, b.off, d, m, nodeB, b.bits);
-            scan_u32(ZeroFlag{ Bo, b.bits }, m, StoreU32{ b.Z }, b.partials, s);
-            k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, nodeA, b.off, d, b.bits, b.Z, m, Bt);
+            if (P.K[d])
+                k_node_qv<<<cdiv((size_t)cnt * P.K[d], kBlock), kBlock, 0, s>>>(Bo, b.off, d, P.K[d], b.kn, b.qv);
+            // the sides of this level's cut: from the level above's partition,
+            // except on levels where point sets are rooted
+            if (d == 0 || P.ra[d] < P.ra[d + 1])
+                k_split<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bf, nodeA, b.off, d, m, nodeB, b.bits);
+            scan_u32(ZeroFlag{ Bo, b.bits, b.flagp }, LoadU8{ b.flagp }, m, StoreU32{ b.Z }, b.partials, s);
+            k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, nodeA, b.off, d, b.flagp, b.bits, b.Z, m, Bt,
+                                                           nodeB);
             std::swap(Bo, Bt);
         }
         roots_at(d + 1, nodeB);
