@@ -21,6 +21,8 @@ Extra fields:
   cpu_baseline  the reference's threaded qdt_execute_parallel, compiled from
                 the reference sources into oracle/_ref/, on the same 10M input,
                 on this host's cores (median of --cpu-reps runs).
+  verify        the exact device verifier (qdt_gpu_verify) on the last
+                step's result: its verdict and kernel time.
   dropin        qdt_execute() from host memory (H2D, pipeline, D2H, host
                 result store): the PCIe-inclusive rate of the drop-in call.
 
@@ -273,6 +275,9 @@ def main():
     ms_per_step = total_s * 1e3 / args.steps
     value = n / (total_s / args.steps) / 1e6
 
+    # the exact device verifier on the last step's result
+    ver = ctx.verify()
+
     T = int(stats["triangles"])
     m = int(stats["unique_points"])
     E = 3 * m - 3 - (2 * m - 2 - T)  # Euler: E = 3m - 3 - b with b = 2m - 2 - T
@@ -334,6 +339,10 @@ def main():
             "pipeline_device_ms": round(stats["device_ms"], 3),
             "pipeline_alg_bytes": 36 * n + 40 * m + 48 * E + 12 * T,
         },
+        "verify": {"check": "qdt_gpu_verify on the last step's result (exact orientation, manifold edges, "
+                            "local Delaunay, convex boundary, Euler, every distinct point used)",
+                   "ok": ver["ok"], "device_ms": round(ver["device_ms"], 3),
+                   "non_delaunay": ver["non_delaunay"], "boundary_edges": ver["boundary_edges"]},
     }
 
     if not args.no_dropin:

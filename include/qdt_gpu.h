@@ -81,6 +81,50 @@ QDT_API void qdt_gpu_set_seams(qdt_t* ptr, int enable);
  * 1); 0 uses one launch per level there too.  Results are identical. */
 QDT_API void qdt_gpu_set_cells(qdt_t* ptr, int enable);
 
+/* ---- Exact verifier ----
+ *
+ * No counterpart in the reference's library: its test harness checks results
+ * on the CPU (test/qdt_verify.cpp:622-696).  These run the same kind of
+ * checks as device kernels: every triangle strictly counter-clockwise (exact
+ * orient2d); every edge in one triangle (boundary) or in two with opposite
+ * directions; every interior edge locally Delaunay (exact incircle of the
+ * opposite vertex <= 0); the boundary turning counter-clockwise (or straight)
+ * at every vertex and continuing at every boundary vertex; T = 2V - 2 - b;
+ * with duplicate information, only group representatives used, and all of
+ * them; with no triangles, the distinct points collinear.  Together these make the list a Delaunay triangulation of the
+ * distinct points (not necessarily the same choice on cocircular ties). */
+typedef struct qdt_gpu_verify_report
+{
+    uint64_t triangles;
+    uint64_t boundary_edges;      /* b */
+    uint64_t vertices;            /* V: distinct vertices used */
+    uint64_t bad_index;           /* vertex index out of range or repeated in a triangle */
+    uint64_t not_representative;  /* vertex that is not the lowest index of its duplicate group */
+    uint64_t bad_orientation;     /* triangles not strictly counter-clockwise */
+    uint64_t bad_edges;           /* non-manifold edges, repeated directed edges, broken boundary */
+    uint64_t non_delaunay;        /* interior edges failing the exact incircle test */
+    uint64_t concave_boundary;    /* boundary vertices turning clockwise */
+    uint64_t unused_points;       /* representatives used by no triangle */
+    uint64_t off_line;            /* with no triangles: distinct points off the line through the
+                                     first two (a set without triangles must be collinear) */
+    int euler_ok;                 /* T = 2V - 2 - b (or T = 0) */
+    int ok;                       /* every check passed */
+    double device_ms;             /* kernel time of the check */
+} qdt_gpu_verify_report;
+
+/* Checks the context's last batch result (triangles, the duplicate rule
+ * and every distinct point used).  QDT_ERR_NULL_CONTEXT / QDT_ERR_NULL_ARGUMENT
+ * for NULL arguments; out-of-memory / internal codes otherwise.  An empty
+ * context reports ok. */
+QDT_API qdt_status qdt_gpu_verify(qdt_t* ptr, qdt_gpu_verify_report* out);
+
+/* Checks T triangles (3 * T uint32 input indices, device memory) over n
+ * points (device memory).  d_rep (n uint32, device memory: the lowest index
+ * of each point's duplicate group) may be NULL, which skips the two
+ * representative checks. */
+QDT_API qdt_status qdt_gpu_verify_triangles(const qdt_vec2d* d_pts, size_t n, const uint32_t* d_tris, size_t T,
+                                            const uint32_t* d_rep, qdt_gpu_verify_report* out);
+
 /* Number of HIP devices visible to the library (0 without a GPU). */
 QDT_API int qdt_gpu_device_count(void);
 

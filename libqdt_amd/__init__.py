@@ -38,7 +38,7 @@ EXPORTS = [
 GPU_EXPORTS = [
     "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
     "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_set_cells", "qdt_gpu_device_count",
-    "qdt_gpu_release_workspace",
+    "qdt_gpu_release_workspace", "qdt_gpu_verify", "qdt_gpu_verify_triangles",
     "qdt_gpu_batch_create", "qdt_gpu_batch_destroy", "qdt_gpu_batch_execute", "qdt_gpu_batch_execute_device",
     "qdt_gpu_batch_sets", "qdt_gpu_batch_set_status", "qdt_gpu_batch_triangles_size",
     "qdt_gpu_batch_unique_points", "qdt_gpu_batch_get_triangle", "qdt_gpu_batch_set_range",
@@ -85,6 +85,23 @@ class GpuStats(ctypes.Structure):
 
 
 _lib = None
+
+
+class VerifyReport(ctypes.Structure):
+    _fields_ = [
+        ("triangles", ctypes.c_uint64), ("boundary_edges", ctypes.c_uint64), ("vertices", ctypes.c_uint64),
+        ("bad_index", ctypes.c_uint64), ("not_representative", ctypes.c_uint64),
+        ("bad_orientation", ctypes.c_uint64), ("bad_edges", ctypes.c_uint64), ("non_delaunay", ctypes.c_uint64),
+        ("concave_boundary", ctypes.c_uint64), ("unused_points", ctypes.c_uint64), ("off_line", ctypes.c_uint64),
+        ("euler_ok", ctypes.c_int),
+        ("ok", ctypes.c_int), ("device_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["ok"] = bool(self.ok)
+        d["euler_ok"] = bool(self.euler_ok)
+        return d
 
 
 def lib():
@@ -135,6 +152,10 @@ def lib():
         L.qdt_gpu_set_seams.argtypes = [vp, ctypes.c_int]
         L.qdt_gpu_set_cells.restype = None
         L.qdt_gpu_set_cells.argtypes = [vp, ctypes.c_int]
+        L.qdt_gpu_verify.restype = st
+        L.qdt_gpu_verify.argtypes = [vp, ctypes.POINTER(VerifyReport)]
+        L.qdt_gpu_verify_triangles.restype = st
+        L.qdt_gpu_verify_triangles.argtypes = [vp, sz, vp, sz, vp, ctypes.POINTER(VerifyReport)]
         L.qdt_gpu_device_count.restype = ctypes.c_int
         L.qdt_gpu_device_count.argtypes = []
         L.qdt_gpu_release_workspace.restype = None
@@ -302,6 +323,15 @@ class Triangulation:
         s = GpuStats()
         lib().qdt_gpu_get_stats(self._h, ctypes.byref(s))
         return s.as_dict()
+
+    def verify(self):
+        """Exact device check of the last result (qdt_gpu_verify): a dict of
+        counts with 'ok'; raises on a failed call."""
+        r = VerifyReport()
+        st = lib().qdt_gpu_verify(self._h, ctypes.byref(r))
+        if st != QDT_OK:
+            raise RuntimeError("qdt_gpu_verify: " + status_string(st))
+        return r.as_dict()
 
     def set_profiling(self, on=True):
         lib().qdt_gpu_set_profiling(self._h, 1 if on else 0)
@@ -485,6 +515,17 @@ class Server:
         b, r = ctypes.c_uint64(), ctypes.c_uint64()
         lib().qdt_gpu_server_counters(self._h, ctypes.byref(b), ctypes.byref(r))
         return b.value, r.value
+
+
+def verify_triangles(d_pts, n, d_tris, T, d_rep=None):
+    """Exact device check of T triangles (device uint32 array, 3 T input
+    indices) over n device points (qdt_gpu_verify_triangles); pointers as
+    integers (e.g. torch data_ptr()).  Returns a dict of counts with 'ok'."""
+    r = VerifyReport()
+    st = lib().qdt_gpu_verify_triangles(d_pts, n, d_tris, T, d_rep, ctypes.byref(r))
+    if st != QDT_OK:
+        raise RuntimeError("qdt_gpu_verify_triangles: " + status_string(st))
+    return r.as_dict()
 
 
 def triangulate(pts):

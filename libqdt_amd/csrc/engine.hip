@@ -1850,6 +1850,159 @@ __global__ __launch_bounds__(kBlock) void k_face_write(const uint32_t* face, con
 }
 
 // ---------------------------------------------------------------------------
+// 9: exact verifier of a triangle list
+// ---------------------------------------------------------------------------
+
+// verifier counters
+enum : int
+{
+    kvBoundary,
+    kvVertices,
+    kvBadIndex,
+    kvNotRep,
+    kvBadOrient,
+    kvBadEdges,
+    kvNonDelaunay,
+    kvConcave,
+    kvUnused,
+    kvSecond,  // lowest representative index with coordinates other than point 0's
+    kvOffLine, // representatives off the line through point 0 and that one
+    kvCount
+};
+
+constexpr uint64_t kNoEdge = ~0ull;
+
+// per triangle: index checks, exact orientation, used vertices, and its
+// three edges keyed by the unordered vertex pair (value 3t + i: edge i runs
+// from corner i to corner i + 1)
+__global__ __launch_bounds__(kBlock) void k_ver_tris(const double2* P, uint32_t n, const uint32_t* tris, uint32_t T,
+                                                     const uint32_t* rep, uint64_t* keys, uint32_t* vals,
+                                                     uint8_t* used, unsigned long long* cnt)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t v[3] = { tris[3 * (size_t)t], tris[3 * (size_t)t + 1], tris[3 * (size_t)t + 2] };
+    const bool idx_ok = v[0] < n && v[1] < n && v[2] < n && v[0] != v[1] && v[1] != v[2] && v[0] != v[2];
+    if (!idx_ok)
+    {
+        atomicAdd(&cnt[kvBadIndex], 1ull);
+        for (int i = 0; i < 3; ++i)
+        {
+            keys[3 * (size_t)t + i] = kNoEdge;
+            vals[3 * (size_t)t + i] = 3 * t + i;
+        }
+        return;
+    }
+    if (rep && (rep[v[0]] != v[0] || rep[v[1]] != v[1] || rep[v[2]] != v[2])) atomicAdd(&cnt[kvNotRep], 1ull);
+    const double2 A = P[v[0]], B = P[v[1]], C = P[v[2]];
+    unsigned nex = 0;
+    if (!(orient2d(A.x, A.y, B.x, B.y, C.x, C.y, nex) > 0)) atomicAdd(&cnt[kvBadOrient], 1ull);
+    for (int i = 0; i < 3; ++i)
+    {
+        used[v[i]] = 1;
+        const uint32_t a = v[i], b = v[(i + 1) % 3];
+        keys[3 * (size_t)t + i] = ((uint64_t)min(a, b) << 32) | max(a, b);
+        vals[3 * (size_t)t + i] = 3 * t + i;
+    }
+}
+
+// per group of equal edge keys (sorted): one edge = boundary (its direction
+// recorded as the boundary successor of its origin), two = interior (opposite
+// directions, exact incircle of the far corner), more = not a manifold
+__global__ __launch_bounds__(kBlock) void k_ver_edges(const double2* P, const uint32_t* tris, const uint64_t* keys,
+                                                      const uint32_t* vals, uint32_t E, int* nextb,
+                                                      unsigned long long* cnt)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= E) return;
+    const uint64_t key = keys[p];
+    if (key == kNoEdge || (p > 0 && keys[p - 1] == key)) return;
+    uint32_t g = 1;
+    while (p + g < E && keys[p + g] == key && g < 3) ++g;
+    auto corner = [&](uint32_t e, int k) { return tris[3 * (size_t)(e / 3) + (e % 3 + k) % 3]; };
+    const uint32_t e1 = vals[p];
+    const uint32_t u1 = corner(e1, 0), v1 = corner(e1, 1);
+    if (g >= 3)
+    {
+        atomicAdd(&cnt[kvBadEdges], 1ull);
+        return;
+    }
+    if (g == 1)
+    {
+        atomicAdd(&cnt[kvBoundary], 1ull);
+        if (atomicCAS(&nextb[u1], -1, (int)v1) != -1) atomicAdd(&cnt[kvBadEdges], 1ull);
+        return;
+    }
+    const uint32_t e2 = vals[p + 1];
+    const uint32_t u2 = corner(e2, 0), v2 = corner(e2, 1);
+    if (u1 != v2 || v1 != u2)
+    {
+        atomicAdd(&cnt[kvBadEdges], 1ull);
+        return;
+    }
+    const double2 A = P[u1], B = P[v1], C = P[corner(e1, 2)], Dp = P[corner(e2, 2)];
+    unsigned nex = 0;
+    if (incircle(A.x, A.y, B.x, B.y, C.x, C.y, Dp.x, Dp.y, nex) > 0) atomicAdd(&cnt[kvNonDelaunay], 1ull);
+}
+
+// With no triangles the distinct points must be collinear: a second point
+// (the lowest representative not equal to point 0; point 0 is always a
+// representative) fixes the line, and every representative is tested.
+__global__ __launch_bounds__(kBlock) void k_ver_second(const double2* P, uint32_t n, const uint32_t* rep,
+                                                       unsigned long long* cnt)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n || (rep && rep[i] != i)) return;
+    const double2 a = P[0], b = P[i];
+    if (a.x != b.x || a.y != b.y) atomicMin(&cnt[kvSecond], (unsigned long long)i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ver_line(const double2* P, uint32_t n, const uint32_t* rep,
+                                                     unsigned long long* cnt)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const unsigned long long s = cnt[kvSecond];
+    if (i >= n || s >= n || (rep && rep[i] != i)) return;
+    const double2 a = P[0], b = P[s], c = P[i];
+    unsigned nex = 0;
+    if (orient2d(a.x, a.y, b.x, b.y, c.x, c.y, nex) != 0) atomicAdd(&cnt[kvOffLine], 1ull);
+}
+
+// per vertex: its boundary turn (exact orient2d of predecessor edge and
+// successor), used-vertex count, unused representatives
+__global__ __launch_bounds__(kBlock) void k_ver_vertices(const double2* P, uint32_t n, const uint32_t* rep,
+                                                         const uint8_t* used, const int* nextb,
+                                                         unsigned long long* cnt)
+{
+    const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
+    bool is_used = false, unused_rep = false;
+    if (u < n)
+    {
+        is_used = used[u] != 0;
+        unused_rep = rep && !is_used && rep[u] == u;
+        const int v = nextb[u];
+        if (v >= 0)
+        {
+            const int w = nextb[v];
+            if (w < 0)
+                atomicAdd(&cnt[kvBadEdges], 1ull); // the boundary does not continue at v
+            else
+            {
+                const double2 A = P[u], B = P[v], C = P[w];
+                unsigned nex = 0;
+                if (orient2d(A.x, A.y, B.x, B.y, C.x, C.y, nex) < 0) atomicAdd(&cnt[kvConcave], 1ull);
+            }
+        }
+    }
+    const unsigned long long nu = __popcll(__ballot(is_used)), nr = __popcll(__ballot(unused_rep));
+    if (__lane_id() == 0)
+    {
+        if (nu) atomicAdd(&cnt[kvVertices], nu);
+        if (nr) atomicAdd(&cnt[kvUnused], nr);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 
@@ -2884,6 +3037,119 @@ Status Engine::batch_output(uint32_t T, int k, int kg, uint32_t m)
     batch_.rep = b.rep;
     batch_.n = res_.n;
     batch_.triangles = T;
+    return Status::ok;
+}
+
+// ---------------------------------------------------------------------------
+// exact verifier (host side)
+// ---------------------------------------------------------------------------
+
+namespace
+{
+
+// device allocations of one call, freed when it returns
+struct TempBuffers
+{
+    std::vector<void*> ptrs;
+    ~TempBuffers()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class X>
+    Status get(X*& p, size_t count)
+    {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(X));
+        if (e != hipSuccess) return e == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal;
+        ptrs.push_back(q);
+        p = (X*)q;
+        return Status::ok;
+    }
+};
+
+} // namespace
+
+Status verify_triangulation(const double* pts_raw, uint32_t n, const uint32_t* tris, uint32_t T, const uint32_t* rep,
+                            hipStream_t s, VerifyReport& out)
+{
+    out = VerifyReport();
+    out.triangles = T;
+    const double2* P = (const double2*)pts_raw;
+    const size_t E = 3 * (size_t)T;
+    if (E > 0xffffffffull) return Status::internal;
+    const uint32_t nb = cdiv(std::max<size_t>(E, 1), kSortTile);
+    TempBuffers tmp;
+    uint64_t *kA = nullptr, *kB = nullptr;
+    uint32_t *vA = nullptr, *vB = nullptr, *hist = nullptr, *partials = nullptr, *ghist = nullptr;
+    uint8_t* used = nullptr;
+    int* nextb = nullptr;
+    unsigned long long* cnt = nullptr;
+    Status st;
+    if ((st = tmp.get(kA, E)) != Status::ok || (st = tmp.get(kB, E)) != Status::ok ||
+        (st = tmp.get(vA, E)) != Status::ok || (st = tmp.get(vB, E)) != Status::ok ||
+        (st = tmp.get(hist, 256 * (size_t)nb)) != Status::ok ||
+        (st = tmp.get(partials, cdiv(256 * (size_t)nb, kScanTile) + 2 + 256 * (size_t)nb)) != Status::ok ||
+        (st = tmp.get(ghist, 8 * 256)) != Status::ok || (st = tmp.get(used, n)) != Status::ok ||
+        (st = tmp.get(nextb, n)) != Status::ok || (st = tmp.get(cnt, kvCount)) != Status::ok)
+        return st;
+    uint32_t* host_ghist = nullptr;
+    if (hipHostMalloc((void**)&host_ghist, 4 * 8 * 256, hipHostMallocDefault) != hipSuccess) return Status::out_of_memory;
+    struct HostFree
+    {
+        void* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{ host_ghist };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    QDT_HIP(hipEventCreate(&e0));
+    QDT_HIP(hipEventCreate(&e1));
+    struct EvFree
+    {
+        hipEvent_t a, b;
+        ~EvFree()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ef{ e0, e1 };
+    QDT_HIP(hipEventRecord(e0, s));
+    QDT_HIP(hipMemsetAsync(used, 0, std::max<size_t>(n, 1), s));
+    QDT_HIP(hipMemsetAsync(nextb, 0xff, 4 * std::max<size_t>(n, 1), s));
+    QDT_HIP(hipMemsetAsync(cnt, 0, 8 * kvCount, s));
+    QDT_HIP(hipMemsetAsync(cnt + kvSecond, 0xff, 8, s));
+    if (T)
+    {
+        k_ver_tris<<<cdiv(T, kBlock), kBlock, 0, s>>>(P, n, tris, T, rep, kA, vA, used, cnt);
+        st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, hist, partials, ghist, host_ghist, s);
+        if (st != Status::ok) return st;
+        k_ver_edges<<<cdiv(E, kBlock), kBlock, 0, s>>>(P, tris, kA, vA, (uint32_t)E, nextb, cnt);
+    }
+    if (n) k_ver_vertices<<<cdiv(n, kBlock), kBlock, 0, s>>>(P, n, rep, used, nextb, cnt);
+    if (n && !T)
+    {
+        k_ver_second<<<cdiv(n, kBlock), kBlock, 0, s>>>(P, n, rep, cnt);
+        k_ver_line<<<cdiv(n, kBlock), kBlock, 0, s>>>(P, n, rep, cnt);
+    }
+    QDT_HIP(hipEventRecord(e1, s));
+    unsigned long long c[kvCount];
+    QDT_HIP(hipMemcpyAsync(c, cnt, 8 * kvCount, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    out.device_ms = ms;
+    out.boundary_edges = c[kvBoundary];
+    out.vertices = c[kvVertices];
+    out.bad_index = c[kvBadIndex];
+    out.not_representative = c[kvNotRep];
+    out.bad_orientation = c[kvBadOrient];
+    out.bad_edges = c[kvBadEdges];
+    out.non_delaunay = c[kvNonDelaunay];
+    out.concave_boundary = c[kvConcave];
+    out.unused_points = c[kvUnused];
+    out.off_line = T ? 0 : c[kvOffLine];
+    out.euler_ok = T == 0 ? true : (uint64_t)T + 2 + out.boundary_edges == 2 * out.vertices;
+    // no triangles: right exactly when the distinct points are collinear
+    out.ok = out.euler_ok && !out.bad_index && !out.not_representative && !out.bad_orientation && !out.bad_edges &&
+        !out.non_delaunay && !out.concave_boundary && (T ? !out.unused_points : !out.off_line);
     return Status::ok;
 }
 

@@ -75,6 +75,36 @@ struct DeviceResult
     size_t n = 0;
 };
 
+// Result of the exact check of a triangle list (verify_triangulation).
+struct VerifyReport
+{
+    uint64_t triangles = 0;
+    uint64_t boundary_edges = 0;    // edges of one triangle only
+    uint64_t vertices = 0;          // distinct vertices used
+    uint64_t bad_index = 0;         // vertex index >= n, or repeated inside a triangle
+    uint64_t not_representative = 0; // a vertex that is not its duplicate group's lowest index
+    uint64_t bad_orientation = 0;   // triangles not strictly counter-clockwise (exact orient2d)
+    uint64_t bad_edges = 0;         // edges of three or more triangles, twice the same direction, or a
+                                    // boundary vertex with two outgoing boundary edges or none
+    uint64_t non_delaunay = 0;      // interior edges whose opposite vertex lies strictly inside the
+                                    // other triangle's circumcircle (exact incircle)
+    uint64_t concave_boundary = 0;  // boundary vertices where the boundary turns clockwise
+    uint64_t unused_points = 0;     // duplicate-group representatives used by no triangle
+    uint64_t off_line = 0;          // no triangles: representatives off the line of the first two
+    bool euler_ok = false;          // T = 2V - 2 - b
+    bool ok = false;                // all of the above clean
+    double device_ms = 0;
+};
+
+// Exact check, on the device, that tris (T triangles, 3 uint32 input indices
+// each) is a Delaunay triangulation of the distinct points of pts (n
+// points): orientation, edge manifoldness, every interior edge locally
+// Delaunay, a convex boundary, the Euler count, every representative used
+// (rep: lowest index of each point's duplicate group, or null to skip the
+// representative checks).  Temporary device memory is allocated and freed.
+Status verify_triangulation(const double* pts, uint32_t n, const uint32_t* tris, uint32_t T, const uint32_t* rep,
+                            hipStream_t s, VerifyReport& out);
+
 class Engine
 {
 public:

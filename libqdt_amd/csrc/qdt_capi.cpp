@@ -679,6 +679,111 @@ int qdt_gpu_device_count(void)
     return n;
 }
 
+namespace
+{
+
+void copy_report(const qdt::VerifyReport& r, qdt_gpu_verify_report* o)
+{
+    o->triangles = r.triangles;
+    o->boundary_edges = r.boundary_edges;
+    o->vertices = r.vertices;
+    o->bad_index = r.bad_index;
+    o->not_representative = r.not_representative;
+    o->bad_orientation = r.bad_orientation;
+    o->bad_edges = r.bad_edges;
+    o->non_delaunay = r.non_delaunay;
+    o->concave_boundary = r.concave_boundary;
+    o->unused_points = r.unused_points;
+    o->off_line = r.off_line;
+    o->euler_ok = r.euler_ok ? 1 : 0;
+    o->ok = r.ok ? 1 : 0;
+    o->device_ms = r.device_ms;
+}
+
+qdt_status status_of(qdt::Status s)
+{
+    return s == qdt::Status::ok ? QDT_OK
+                                : (s == qdt::Status::out_of_memory ? QDT_ERR_OUT_OF_MEMORY : QDT_ERR_INTERNAL);
+}
+
+} // namespace
+
+qdt_status qdt_gpu_verify_triangles(const qdt_vec2d* d_pts, size_t n, const uint32_t* d_tris, size_t T,
+                                    const uint32_t* d_rep, qdt_gpu_verify_report* out)
+{
+    if (!out || (n && !d_pts) || (T && !d_tris)) return QDT_ERR_NULL_ARGUMENT;
+    if (n > 100000000u || T > 400000000u) return QDT_ERR_TOO_MANY_POINTS;
+    try
+    {
+        qdt::VerifyReport r;
+        const qdt::Status s = qdt::verify_triangulation((const double*)d_pts, (uint32_t)n, d_tris, (uint32_t)T,
+                                                        d_rep, nullptr, r);
+        if (s == qdt::Status::ok) copy_report(r, out);
+        return status_of(s);
+    }
+    catch (...)
+    {
+        return QDT_ERR_OUT_OF_MEMORY;
+    }
+}
+
+qdt_status qdt_gpu_verify(qdt_t* ptr, qdt_gpu_verify_report* out)
+{
+    if (!ptr) return QDT_ERR_NULL_CONTEXT;
+    if (!out) return QDT_ERR_NULL_ARGUMENT;
+    qdt_t* Q = ptr;
+    try
+    {
+        qdt::VerifyReport r;
+        if (Q->npts == 0 || !Q->eng)
+        {
+            r.ok = r.euler_ok = true;
+            copy_report(r, out);
+            return QDT_OK;
+        }
+        qdt::Status s;
+        if (Q->tris_on_device_only)
+        {
+            // the last run's results are still only on the device
+            const qdt::DeviceResult& d = Q->eng->result();
+            s = qdt::verify_triangulation(Q->d_in, (uint32_t)Q->npts, d.tris, d.triangles, d.rep,
+                                          Q->eng->stream(), r);
+        }
+        else
+        {
+            // the host store (the device workspace may have been released)
+            qdt_status hs = materialize_points(Q);
+            if (hs != QDT_OK) return hs;
+            void *dp = nullptr, *dt = nullptr, *dr = nullptr;
+            struct Free
+            {
+                void** p[3];
+                ~Free()
+                {
+                    for (void** q : p)
+                        if (*q) (void)hipFree(*q);
+                }
+            } fr{ { &dp, &dt, &dr } };
+            const size_t np = Q->npts, nt = Q->ntris;
+            if (hipMalloc(&dp, 16 * np) != hipSuccess || hipMalloc(&dt, 12 * std::max<size_t>(nt, 1)) != hipSuccess ||
+                hipMalloc(&dr, 4 * np) != hipSuccess)
+                return QDT_ERR_OUT_OF_MEMORY;
+            if (hipMemcpy(dp, Q->pts.p, 16 * np, hipMemcpyHostToDevice) != hipSuccess ||
+                (nt && hipMemcpy(dt, Q->tris.p, 12 * nt, hipMemcpyHostToDevice) != hipSuccess) ||
+                hipMemcpy(dr, Q->rep.p, 4 * np, hipMemcpyHostToDevice) != hipSuccess)
+                return QDT_ERR_INTERNAL;
+            s = qdt::verify_triangulation((const double*)dp, (uint32_t)np, (const uint32_t*)dt, (uint32_t)nt,
+                                          (const uint32_t*)dr, Q->eng->stream(), r);
+        }
+        if (s == qdt::Status::ok) copy_report(r, out);
+        return status_of(s);
+    }
+    catch (...)
+    {
+        return QDT_ERR_OUT_OF_MEMORY;
+    }
+}
+
 void qdt_gpu_release_workspace(qdt_t* ptr)
 {
     if (!ptr || !ptr->eng) return;
