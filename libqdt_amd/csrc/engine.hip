@@ -1254,6 +1254,7 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const do
     __shared__ int su[kSegMaxK], sv[kSegMaxK];
     __shared__ double4 sc[kSegMaxK];
     __shared__ int hset[kSelHash];
+    __shared__ uint8_t ok_tan[kSegMaxK], ok_prev[kSegMaxK];
     const uint32_t j = blockIdx.x;
     const int k = threadIdx.x;
     if (j == 0 && k == 0) *alist_n = 0;
@@ -1300,12 +1301,35 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const do
             }
             __syncthreads();
         }
+    // the ordering tests of every candidate against the tangent and against
+    // its predecessor in height order, in parallel (the greedy pass below
+    // needs others only after a rejection)
+    double2 PU = PT[pu0], PV = PT[pv0];
+    unsigned dummy = 0;
+    {
+        const int sl = sslot[k];
+        uint8_t t_ok = 0, p_ok = 0;
+        if (sl >= 0)
+        {
+            const double4 c = sc[sl];
+            t_ok = orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0 &&
+                orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
+            const int sp = k > 0 ? sslot[k - 1] : -1;
+            if (sp >= 0)
+            {
+                const double4 q = sc[sp];
+                p_ok = orient2d(c.x, c.y, q.x, q.y, q.z, q.w, dummy) > 0 &&
+                    orient2d(c.z, c.w, q.x, q.y, q.z, q.w, dummy) > 0;
+            }
+        }
+        ok_tan[k] = t_ok;
+        ok_prev[k] = p_ok;
+    }
+    __syncthreads();
     if (k != 0) return;
     hset_add(hset, pu0);
     hset_add(hset, pv0);
-    double2 PU = PT[pu0], PV = PT[pv0];
-    unsigned dummy = 0;
-    int cnt = 0;
+    int cnt = 0, last = -1; // sorted position of the last cut taken (-1: the tangent)
     for (int i = 0; i < K; ++i)
     {
         const int sl = sslot[i];
@@ -1314,8 +1338,16 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const do
         const int u = su[sl], v = sv[sl];
         const double4 c = sc[sl];
         bool ok = u != v && !hset_has(hset, u) && !hset_has(hset, v);
-        ok = ok && orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
-        ok = ok && orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
+        if (ok)
+        {
+            if (last < 0)
+                ok = ok_tan[i];
+            else if (last == i - 1)
+                ok = ok_prev[i];
+            else
+                ok = orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0 &&
+                    orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
+        }
         if (!ok)
         {
             astate[tt] = 0;
@@ -1327,6 +1359,7 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const do
         selk[(size_t)j * kSegSlots + cnt++] = (int)tt;
         PU = make_double2(c.x, c.y);
         PV = make_double2(c.z, c.w);
+        last = i;
     }
     nsel[j] = cnt;
     if (cnt) atomicAdd(&counters[11], (unsigned long long)cnt);

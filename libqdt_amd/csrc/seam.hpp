@@ -248,13 +248,15 @@ __device__ int collect_crossings(Q& q, int u, int x, int p, int* out, int cap)
     const int ox = q.onext(x);
     if (ox == x) return 0; // u has one edge: nothing around it to cross
     const double2 U = q.pt(u), P = q.pt(p);
-    // the face left of x, between x and onext(x): a triangle (u, a, b)
+    // the face left of x, between x and onext(x): a triangle (u, a, b).  Its
+    // corners are read as dest of its edges, so that the corner loads and
+    // the closing test of the 3-cycle are in flight together.
     int f = q.lnext(x);
     int g = q.lnext(f);
+    int a = q.dest(x), b = q.dest(f);
+    double2 A = q.pt(a), B = q.pt(b);
     if (q.lnext(g) != x)
         return face_kind(q, x) == 0 ? 0 : -1; // the segment leaves u into the outer face, or a larger face
-    int a = q.org(f), b = q.org(g);
-    double2 A = q.pt(a), B = q.pt(b);
     {
         const double o = q.pred_orient(U, A, B);
         if (o < 0) return 0;  // the outer face
@@ -273,9 +275,9 @@ __device__ int collect_crossings(Q& q, int u, int x, int p, int* out, int cap)
         const int h = e ^ 2;
         f = q.lnext(h);
         g = q.lnext(f);
-        if (q.lnext(g) != h) return face_kind(q, h) == 0 ? n : -1;
-        const int c = q.org(g);
+        const int c = q.dest(f);
         const double2 C = q.pt(c);
+        if (q.lnext(g) != h) return face_kind(q, h) == 0 ? n : -1;
         const double o = q.pred_orient(B, A, C);
         if (o < 0) return n; // the outer face
         if (o == 0) return -1;
