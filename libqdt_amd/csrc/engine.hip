@@ -963,8 +963,10 @@ __global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const doubl
 // 7b: large merges cut into seam pieces (see seam.hpp)
 // ---------------------------------------------------------------------------
 
-constexpr int kSegMaxK = 256;          // anchors per node at most (a power of two)
+constexpr int kSegMaxK = 512;          // anchor slots per node (a power of two)
+constexpr int kSegCap = kSegMaxK - 2;  // anchors per node at most: a phase's pieces fit 256 threads
 constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
+constexpr int kSegThreads = 256;       // threads of the per-node compaction and joining kernels
 constexpr int kSegMin = 1024;          // smaller nodes merge in one piece
 constexpr int kPhases = 2;             // pieces k, k+2, ... run together
 constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit merges
@@ -978,7 +980,7 @@ __host__ __device__ inline int seam_anchors(double s, double w, double h)
     const double seam = w > 0 ? std::sqrt(s * h / w) : 1e9;
     const double k = seam / 3.5;
     if (k < 3) return 0;
-    return k > kSegMaxK ? kSegMaxK : (int)k;
+    return k > kSegCap ? kSegCap : (int)k;
 }
 
 // upper bound of seam_anchors for nodes of up to s points (elongation up to
@@ -1570,10 +1572,11 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_pools(int* NX, int* OR, int
 
 // cut list of every node: the tangent, then the inserted anchors in order
 // (one workgroup per node, a block scan compacts the inserted ones)
-__global__ __launch_bounds__(kSegMaxK) void k_seg_compact(int d, const int* selk, const int* nsel,
-                                                         const int* astate, const int* aedge, int* segstart,
-                                                         int* nseg, const int* off)
+__global__ __launch_bounds__(kSegThreads) void k_seg_compact(int d, const int* selk, const int* nsel,
+                                                            const int* astate, const int* aedge, int* segstart,
+                                                            int* nseg, const int* off)
 {
+    constexpr int kPerSel = kSegMaxK / kSegThreads; // candidates per thread, consecutive
     const uint32_t j = blockIdx.x;
     const int k = threadIdx.x;
     if (j >= lcount(off, d)) return;
@@ -1585,15 +1588,25 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_compact(int d, const int* selk
         return;
     }
     const int ns = nsel[j];
-    int v = -1;
-    if (k < ns)
+    int v[kPerSel];
+    uint32_t c = 0;
+#pragma unroll
+    for (int r = 0; r < kPerSel; ++r)
     {
-        const int t = selk[(size_t)j * kSegSlots + k];
-        if (astate[t] == 2 && aedge[t] >= 0) v = aedge[t];
+        const int x = k * kPerSel + r;
+        v[r] = -1;
+        if (x < ns)
+        {
+            const int t = selk[(size_t)j * kSegSlots + x];
+            if (astate[t] == 2 && aedge[t] >= 0) v[r] = aedge[t];
+        }
+        c += v[r] >= 0 ? 1u : 0u;
     }
     uint32_t total;
-    const uint32_t pos = block_exclusive_scan(v >= 0 ? 1u : 0u, total);
-    if (v >= 0) segstart[(size_t)j * kSegSlots + 1 + pos] = v;
+    uint32_t pos = block_exclusive_scan(c, total);
+#pragma unroll
+    for (int r = 0; r < kPerSel; ++r)
+        if (v[r] >= 0) segstart[(size_t)j * kSegSlots + 1 + pos++] = v[r];
     if (k == 0) nseg[j] = 1 + (int)total;
 }
 
@@ -1701,14 +1714,15 @@ __global__ __launch_bounds__(kSegPieceBlock) void k_seg_pieces(int* NX, int* OR,
 // one workgroup per node: the free lists of the node, its pieces and its
 // anchors are joined in parallel (compacted by a block scan, each list's
 // tail linked to the next list's head); then the hull re-walk
-__global__ __launch_bounds__(kSegMaxK) void k_seg_final(int* NX, int* OR, const double2* PT, int* VE,
+__global__ __launch_bounds__(kSegThreads) void k_seg_final(int* NX, int* OR, const double2* PT, int* VE,
                                                        const int* off, int d, int K, uint32_t m, int lim_e,
                                                        const int* nseg, const int4* nstate, const int2* segpool,
                                                        const int* astate, const int2* apool, int4* res, int* err,
                                                        unsigned long long* counters)
 {
-    constexpr int kPer = 3; // lists per thread: 1 + kSegSlots + kSegMaxK <= 3 * kSegMaxK
-    __shared__ int2 lists[kPer * kSegMaxK];
+    // lists per thread: 1 + kSegSlots + kSegMaxK of them at most
+    constexpr int kPer = (1 + kSegSlots + kSegMaxK + kSegThreads - 1) / kSegThreads;
+    __shared__ int2 lists[kPer * kSegThreads];
     const uint32_t j = blockIdx.x;
     const int k = threadIdx.x;
     if (j >= lcount(off, d)) return;
@@ -1743,7 +1757,7 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_final(int* NX, int* OR, const 
         if (mine[r].x >= 0) lists[pos++] = mine[r];
     __syncthreads();
     const int lim_r = lim_e >> 2;
-    for (uint32_t x = k; x + 1 < total; x += kSegMaxK)
+    for (uint32_t x = k; x + 1 < total; x += kSegThreads)
     {
         const int tail = lists[x].y;
         if ((unsigned)tail < (unsigned)lim_r)
@@ -2888,7 +2902,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                 b.err, b.counters);
             k_seg_pools<<<ga, kMergeBlock, 0, s>>>(b.NX, b.OR, total, b.arec, b.astate, b.aedge, lim_e, (int*)b.face,
                                                    b.apool);
-            k_seg_compact<<<cnt, kSegMaxK, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge, b.segstart, b.nseg,
+            k_seg_compact<<<cnt, kSegThreads, 0, s>>>(d, b.selk, b.nsel, b.astate, b.aedge, b.segstart, b.nseg,
                                                    b.off);
             {
                 // threads per node: the pieces of one phase, rounded up to a
@@ -2906,7 +2920,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                     b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.spare, spare_base, spare_cap, b.segstart, b.nseg,
                     tpn, npb, entries, b.segpool, b.segresume, b.err, b.counters);
             }
-            k_seg_final<<<cnt, kSegMaxK, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m,
+            k_seg_final<<<cnt, kSegThreads, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m,
                                                                         lim_e, b.nseg, b.nstate, b.segpool,
                                                                         b.astate, b.apool, b.res, b.err, b.counters);
         }
