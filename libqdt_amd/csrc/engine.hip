@@ -1209,40 +1209,76 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_tangent(int* NX, int* OR, c
 }
 
 // Anchors in chain order, each strictly above the previous cut, and no two
-// cuts with a common end vertex.  One workgroup per node: the candidates and
-// their cached coordinates go to LDS and are sorted by crossing height
-// (bitonic); one thread then takes them in order (exact orient2d against the
-// last cut; end vertices kept in an LDS hash set).  The tangent's rings
-// changed after the anchors' dry runs, so its two vertices are excluded as
-// anchor ends too.
+// cuts with a common end vertex.  One workgroup per node (a power of two of
+// threads, at least the node's candidate count): the candidates and their
+// cached coordinates go to LDS and are sorted by crossing height (bitonic).
+// The choice is the greedy one: in height order a candidate is taken when
+// neither end is an end of the tangent or of a cut taken before it, and both
+// ends lie strictly left of the last cut taken (exact orient2d).  The tangent's
+// rings changed after the anchors' dry runs, so its two vertices are excluded
+// as anchor ends too.
+//
+// The greedy pass runs in parallel as a fixed-point iteration.  Each round,
+// every position recomputes its decision from the previous round's decisions
+// of the positions before it: the last taken position comes from a prefix
+// max, and "an end already used" from an LDS table holding, per vertex, the
+// lowest taken position that has it as an end.  A position's decision depends
+// only on earlier positions, so after round r the first r positions are
+// final; the rounds stop when nothing changes (two or three rounds when
+// rejections are rare, as they are).
 constexpr int kSelHash = 2048; // > 2 * (kSegMaxK + 1), a power of two
 
-__device__ __forceinline__ bool hset_add(int* h, int v) // false if present
+// per-vertex lowest taken position: insert (v, pos) / look up (INT_MAX: none)
+__device__ __forceinline__ void vmin_put(int* hk, int* hp, int v, int pos)
 {
     unsigned x = ((unsigned)v * 2654435761u) & (kSelHash - 1);
     for (int probe = 0; probe < kSelHash; ++probe)
     {
-        if (h[x] == v) return false;
-        if (h[x] < 0)
+        int cur = hk[x];
+        if (cur < 0)
         {
-            h[x] = v;
-            return true;
+            cur = atomicCAS(&hk[x], -1, v);
+            if (cur < 0) cur = v;
+        }
+        if (cur == v)
+        {
+            atomicMin(&hp[x], pos);
+            return;
         }
         x = (x + 1) & (kSelHash - 1);
     }
-    return false;
 }
 
-__device__ __forceinline__ bool hset_has(const int* h, int v)
+__device__ __forceinline__ int vmin_get(const int* hk, const int* hp, int v)
 {
     unsigned x = ((unsigned)v * 2654435761u) & (kSelHash - 1);
     for (int probe = 0; probe < kSelHash; ++probe)
     {
-        if (h[x] == v) return true;
-        if (h[x] < 0) return false;
+        const int cur = hk[x];
+        if (cur == v) return hp[x];
+        if (cur < 0) return INT_MAX;
         x = (x + 1) & (kSelHash - 1);
     }
-    return false;
+    return INT_MAX;
+}
+
+// inclusive block scan (max or sum) for blocks of a multiple of 64 threads
+template <bool kMax>
+__device__ __forceinline__ int sel_scan(int v, int* wtmp)
+{
+    const int lane = (int)__lane_id(), w = (int)threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v = kMax ? max(v, t) : v + t;
+    }
+    if (lane == 63) wtmp[w] = v;
+    __syncthreads();
+    int pre = kMax ? -1 : 0;
+    for (int i = 0; i < w; ++i) pre = kMax ? max(pre, wtmp[i]) : pre + wtmp[i];
+    __syncthreads();
+    return kMax ? max(pre, v) : pre + v;
 }
 
 __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const double2* PT, const int* off, int d,
@@ -1255,10 +1291,11 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const do
     __shared__ int sslot[kSegMaxK];
     __shared__ int su[kSegMaxK], sv[kSegMaxK];
     __shared__ double4 sc[kSegMaxK];
-    __shared__ int hset[kSelHash];
-    __shared__ uint8_t ok_tan[kSegMaxK], ok_prev[kSegMaxK];
+    __shared__ int hk[kSelHash], hp[kSelHash];
+    __shared__ int lastv[kSegMaxK];
+    __shared__ int wtmp[kSegMaxK / 64];
     const uint32_t j = blockIdx.x;
-    const int k = threadIdx.x;
+    const int k = threadIdx.x, nt = blockDim.x;
     if (j == 0 && k == 0) *alist_n = 0;
     if (j >= lcount(off, d)) return;
     const int* od = off + off_base(off, d);
@@ -1269,20 +1306,21 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const do
     if (b0 < 0 || b0 >= lim_e) return; // the tangent failed: the run's error flag is set
     const int pu0 = OR[(b0 ^ 2) >> 1], pv0 = OR[b0 >> 1];
     if ((unsigned)pu0 >= m || (unsigned)pv0 >= m) return;
-    const size_t t = (size_t)j * K + k;
-    const bool valid = k < K && astate[t] == 1;
-    skey[k] = valid ? akey[t] : INFINITY;
-    sslot[k] = valid ? k : -1;
-    if (valid)
     {
-        su[k] = arec[t * kRec];
-        sv[k] = arec[t * kRec + 1];
-        sc[k] = acoord[t];
+        const size_t t = (size_t)j * K + k;
+        const bool valid = k < K && astate[t] == 1;
+        skey[k] = valid ? akey[t] : INFINITY;
+        sslot[k] = valid ? k : -1;
+        if (valid)
+        {
+            su[k] = arec[t * kRec];
+            sv[k] = arec[t * kRec + 1];
+            sc[k] = acoord[t];
+        }
     }
-    for (int x = k; x < kSelHash; x += kSegMaxK) hset[x] = -1;
     __syncthreads();
     // bitonic sort of (key, slot), ascending; invalid slots (infinite keys) last
-    for (int size = 2; size <= kSegMaxK; size <<= 1)
+    for (int size = 2; size <= nt; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1)
         {
             const int partner = k ^ stride;
@@ -1303,68 +1341,77 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const do
             }
             __syncthreads();
         }
-    // the ordering tests of every candidate against the tangent and against
-    // its predecessor in height order, in parallel (the greedy pass below
-    // needs others only after a rejection)
-    double2 PU = PT[pu0], PV = PT[pv0];
+    // the candidate at sorted position k, its tests against the tangent and
+    // against the candidate at position k - 1
+    const int sl = sslot[k];
+    int u = -1, v = -1;
+    double4 c = make_double4(0, 0, 0, 0);
+    bool base = false, ok_tan = false, ok_prev = false;
     unsigned dummy = 0;
+    if (sl >= 0)
     {
-        const int sl = sslot[k];
-        uint8_t t_ok = 0, p_ok = 0;
-        if (sl >= 0)
+        u = su[sl];
+        v = sv[sl];
+        c = sc[sl];
+        base = u != v && u != pu0 && u != pv0 && v != pu0 && v != pv0;
+        const double2 PU = PT[pu0], PV = PT[pv0];
+        ok_tan = orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0 &&
+            orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
+        const int sp = k > 0 ? sslot[k - 1] : -1;
+        if (sp >= 0)
         {
-            const double4 c = sc[sl];
-            t_ok = orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0 &&
-                orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
-            const int sp = k > 0 ? sslot[k - 1] : -1;
-            if (sp >= 0)
+            const double4 q = sc[sp];
+            ok_prev = orient2d(c.x, c.y, q.x, q.y, q.z, q.w, dummy) > 0 &&
+                orient2d(c.z, c.w, q.x, q.y, q.z, q.w, dummy) > 0;
+        }
+    }
+    bool take = base;
+    for (int round = 0; round <= nt; ++round)
+    {
+        lastv[k] = sel_scan<true>(take ? k : -1, wtmp);
+        for (int x = k; x < kSelHash; x += nt)
+        {
+            hk[x] = -1;
+            hp[x] = INT_MAX;
+        }
+        __syncthreads();
+        if (take)
+        {
+            vmin_put(hk, hp, u, k);
+            vmin_put(hk, hp, v, k);
+        }
+        __syncthreads();
+        bool now = false;
+        if (base && vmin_get(hk, hp, u) >= k && vmin_get(hk, hp, v) >= k)
+        {
+            const int L = k > 0 ? lastv[k - 1] : -1;
+            if (L < 0)
+                now = ok_tan;
+            else if (L == k - 1)
+                now = ok_prev;
+            else
             {
-                const double4 q = sc[sp];
-                p_ok = orient2d(c.x, c.y, q.x, q.y, q.z, q.w, dummy) > 0 &&
+                const double4 q = sc[sslot[L]];
+                now = orient2d(c.x, c.y, q.x, q.y, q.z, q.w, dummy) > 0 &&
                     orient2d(c.z, c.w, q.x, q.y, q.z, q.w, dummy) > 0;
             }
         }
-        ok_tan[k] = t_ok;
-        ok_prev[k] = p_ok;
+        const bool changed = __syncthreads_or(now != take);
+        take = now;
+        if (!changed) break;
     }
-    __syncthreads();
-    if (k != 0) return;
-    hset_add(hset, pu0);
-    hset_add(hset, pv0);
-    int cnt = 0, last = -1; // sorted position of the last cut taken (-1: the tangent)
-    for (int i = 0; i < K; ++i)
+    const int rank = sel_scan<false>(take ? 1 : 0, wtmp);
+    if (sl >= 0)
     {
-        const int sl = sslot[i];
-        if (sl < 0) continue;
         const size_t tt = (size_t)j * K + sl;
-        const int u = su[sl], v = sv[sl];
-        const double4 c = sc[sl];
-        bool ok = u != v && !hset_has(hset, u) && !hset_has(hset, v);
-        if (ok)
-        {
-            if (last < 0)
-                ok = ok_tan[i];
-            else if (last == i - 1)
-                ok = ok_prev[i];
-            else
-                ok = orient2d(c.x, c.y, PU.x, PU.y, PV.x, PV.y, dummy) > 0 &&
-                    orient2d(c.z, c.w, PU.x, PU.y, PV.x, PV.y, dummy) > 0;
-        }
-        if (!ok)
-        {
-            astate[tt] = 0;
-            continue;
-        }
-        astate[tt] = 2;
-        hset_add(hset, u);
-        hset_add(hset, v);
-        selk[(size_t)j * kSegSlots + cnt++] = (int)tt;
-        PU = make_double2(c.x, c.y);
-        PV = make_double2(c.z, c.w);
-        last = i;
+        astate[tt] = take ? 2 : 0;
+        if (take) selk[(size_t)j * kSegSlots + rank - 1] = (int)tt;
     }
-    nsel[j] = cnt;
-    if (cnt) atomicAdd(&counters[11], (unsigned long long)cnt);
+    if (k == nt - 1)
+    {
+        nsel[j] = rank;
+        if (rank) atomicAdd(&counters[11], (unsigned long long)rank);
+    }
 }
 
 // Parallel anchor insertion.  Several anchors may cross the same edge (long
@@ -2888,7 +2935,9 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
                                                                           b.spare, spare_base, spare_cap, b.res,
                                                                           b.segstart, b.nstate, b.err, b.counters);
-            k_seg_select<<<cnt, kSegMaxK, 0, s>>>(b.OR, b.P0, b.off, d, K, m, lim_e, b.segstart,
+            int sel_threads = 64;
+            while (sel_threads < K) sel_threads <<= 1;
+            k_seg_select<<<cnt, sel_threads, 0, s>>>(b.OR, b.P0, b.off, d, K, m, lim_e, b.segstart,
                                                                          b.arec, b.acoord, b.akey, b.astate, b.selk,
                                                                          b.nsel, b.alist_n, b.counters);
             const int acap = (int)(b.anchor_cap * (2 + 4 * kCross));
