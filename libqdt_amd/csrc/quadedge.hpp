@@ -404,6 +404,212 @@ struct QuadEdges
         return basel;
     }
 
+    // Deletion of e when oprev(e) = po and onext(e) = na are known (eo, ed:
+    // its ends).  Same result as delete_edge: the two splices of
+    // builder.hxx:161-166, with every word they read back fixed by
+    // onext(oprev(x)) = x and rot(x).onext = invrot(oprev(x)); only the ring
+    // at the far end (ed) is read.
+    __device__ __forceinline__ void delete_known(int e, int po, int na, int eo, int ed, Pool& p)
+    {
+        const int s = e ^ 2;
+        const int na2 = onext(s);
+        const int r2 = onext(e_rot(s));
+        const int pd = e_rot(r2);
+        set_next(e, e);
+        set_next(po, na);
+        set_next(e_rot(na), e_invrot(po));
+        set_next(e_rot(e), e_invrot(e));
+        set_next(s, s);
+        set_next(pd, na2);
+        set_next(e_rot(na2), r2);
+        set_next(e_rot(s), e_invrot(s));
+        set_ve(eo, po != e ? po : -1);
+        set_ve(ed, pd != s ? pd : -1);
+        push(p, chk(e) >> 2);
+    }
+
+    // New edge o -> d placed in o's ring right after x (whose onext was nx)
+    // and in d's ring right after y (whose onext was ny): the result of
+    // connect (builder.hxx:151-159) with every word it reads known.
+    __device__ __forceinline__ int connect_known(int x, int nx, int y, int ny, int o, int d, Pool& p)
+    {
+        const int e = 4 * pop(p);
+        ++pc.connects;
+        set_org(e, o);
+        set_org(e ^ 2, d);
+        set_next(e, nx);
+        set_next(x, e);
+        set_next(e + 1, e_invrot(x));
+        set_next(e_rot(nx), e + 3);
+        set_next(e + 2, ny);
+        set_next(y, e + 2);
+        set_next(e + 3, e_invrot(y));
+        set_next(e_rot(ny), e + 1);
+        set_ve(o, e);
+        set_ve(d, e ^ 2);
+        return e;
+    }
+
+    // The rising bubble in global memory: the predicates, claims and store
+    // updates of the LDS form below, in the same order, with each step's
+    // candidates carried over instead of read again, so that a step waits
+    // on as few dependent loads (each a trip to L2 or HBM) as possible.
+    //  * The side not chosen keeps its candidate, the candidate after it and
+    //    their ends and coordinates: the new edge changes neither ring.
+    //  * The chosen side's new candidate comes from the connect: on the
+    //    right oprev(new base) = lnext(rc), on the left onext(new base.sym)
+    //    = the old onext(lc.sym).
+    //  * Deleting lc: oprev(lc) = basel.sym, onext(lc) = ln; deleting rc:
+    //    oprev(rc) = rp, onext(rc) = basel; the next candidate's ring word is
+    //    read before the deletion's stores.
+    // Where a carried word is one the step itself rewrote (an end vertex of
+    // degree one or two), the value written is used instead.
+    __device__ int bubble_carry(int& basel, int stop, Pool& pool, int& budget)
+    {
+        const int srec = stop >= 0 ? (stop >> 2) : -1;
+        const int so = stop >= 0 ? org(stop) : -1;
+        const int sd = stop >= 0 ? dest(stop) : -1;
+        int bo = org(basel), bd = dest(basel);
+        double2 Po = pt(bo), Pd = pt(bd);
+        bool fresh = true;
+        int lc = 0, ln = 0, lcd = 0, lnd = 0, rc = 0, rp = 0, rcd = 0, rpd = 0;
+        double2 Pl = Po, Pln = Po, Pr = Po, Prp = Po;
+        for (;;)
+        {
+            if (err) return 0;
+            if (--budget < 0)
+            {
+                err = 3;
+                return 0;
+            }
+            if (!take(bo) || !take(bd)) return 2;
+            if (fresh)
+            {
+                lc = onext(basel ^ 2);
+                rc = oprev(basel);
+                ln = onext(lc);
+                rp = oprev(rc);
+                lcd = dest(lc);
+                rcd = dest(rc);
+                lnd = dest(ln);
+                rpd = dest(rp);
+                Pl = pt(lcd);
+                Pr = pt(rcd);
+                Pln = pt(lnd);
+                Prp = pt(rpd);
+                fresh = false;
+            }
+            const int b2 = basel ^ 2;
+            bool lv = false;
+            if ((lc >> 2) != srec)
+            {
+                lv = ccw_p(Pl, Pd, Po);
+                if (lv)
+                {
+                    bool deleted = false;
+                    for (;;)
+                    {
+                        if (err || --budget < 0) break;
+                        if ((ln >> 2) == srec) break;
+                        if (!in_circle_p(bd, bo, lcd, lnd, Pd, Po, Pl, Pln)) break;
+                        if (!take(lcd)) return 2;
+                        const int nl = onext(ln);
+                        delete_known(lc, b2, ln, bd, lcd, pool);
+                        lc = ln;
+                        lcd = lnd;
+                        Pl = Pln;
+                        ln = lc == b2 ? b2 : nl; // bd's ring was {basel.sym, lc}
+                        lnd = dest(ln);
+                        Pln = pt(lnd);
+                        deleted = true;
+                    }
+                    if (deleted) lv = ccw_p(Pl, Pd, Po);
+                }
+            }
+            bool rv = false;
+            if ((rc >> 2) != srec)
+            {
+                rv = ccw_p(Pr, Pd, Po);
+                if (rv)
+                {
+                    bool deleted = false;
+                    for (;;)
+                    {
+                        if (err || --budget < 0) break;
+                        if ((rp >> 2) == srec) break;
+                        if (!in_circle_p(bd, bo, rcd, rpd, Pd, Po, Pr, Prp)) break;
+                        if (!take(rcd)) return 2;
+                        const int pr = onext(e_rot(rp));
+                        delete_known(rc, rp, basel, bo, rcd, pool);
+                        rc = rp;
+                        rcd = rpd;
+                        Pr = Prp;
+                        rp = rc == basel ? basel : e_rot(pr); // bo's ring was {basel, rc}
+                        rpd = dest(rp);
+                        Prp = pt(rpd);
+                        deleted = true;
+                    }
+                    if (deleted) rv = ccw_p(Pr, Pd, Po);
+                }
+            }
+            if (!lv && !rv) return 0;
+            const bool right = !lv || (rv && in_circle_p(lcd, bd, bo, rcd, Pl, Pd, Po, Pr));
+            // the new cross edge would run rcd -> bd or bo -> lcd
+            if (stop >= 0)
+            {
+                const int no = right ? rcd : bo;
+                const int nd = right ? bd : lcd;
+                if (no == so && nd == sd) return 1;
+            }
+            if (!take(right ? rcd : lcd)) return 2;
+            if (right)
+            {
+                // e = rcd -> bd after lnext(rc) at rcd and after basel.sym at bd
+                const int x = lnext(rc);
+                const int nx = onext(x), xr = onext(e_rot(x)), xd = dest(x);
+                const int e = connect_known(x, nx, b2, lc, rcd, bd, pool);
+                if (lc == b2) // bd's ring was {basel.sym}: now {basel.sym, e.sym}
+                {
+                    ln = e ^ 2;
+                    lnd = rcd;
+                    Pln = Pr;
+                }
+                basel = e;
+                bo = rcd;
+                Po = Pr;
+                rc = x;
+                rcd = xd;
+                rp = nx == x ? e : e_rot(xr); // rcd's ring was {x}: now {x, e}
+                rpd = dest(rp);
+                Pr = pt(rcd);
+                Prp = pt(rpd);
+            }
+            else
+            {
+                // e = bo -> lcd after rc (= lnext(basel.sym)) at bo and after lc.sym at lcd
+                const int ls = lc ^ 2;
+                const int ny = onext(ls);
+                const int nl = onext(ny), nd = dest(ny);
+                const int e = connect_known(rc, basel, ls, ny, bo, lcd, pool);
+                if (rc == basel) // bo's ring was {basel}: now {basel, e}
+                {
+                    rp = e;
+                    rpd = lcd;
+                    Prp = Pl;
+                }
+                basel = e;
+                bd = lcd;
+                Pd = Pl;
+                lc = ny;
+                lcd = nd;
+                ln = ny == ls ? (e ^ 2) : nl; // lcd's ring was {lc.sym}: now {lc.sym, e.sym}
+                lnd = dest(ln);
+                Pl = pt(lcd);
+                Pln = pt(lnd);
+            }
+        }
+    }
+
     // The rising bubble of builder.hxx:263-283 from cross edge `basel`.  With
     // stop < 0 it runs to the upper common tangent.  With a stop edge (a
     // cross edge already in the store, directed right -> left like basel) it
@@ -416,6 +622,7 @@ struct QuadEdges
     // the cross edge to resume from.
     __device__ int bubble(int& basel, int stop, Pool& pool, int& budget)
     {
+        if constexpr (kChecked) return bubble_carry(basel, stop, pool, budget);
         const int srec = stop >= 0 ? (stop >> 2) : -1;
         const int so = stop >= 0 ? org(stop) : -1;
         const int sd = stop >= 0 ? dest(stop) : -1;
