@@ -450,10 +450,22 @@ struct QuadEdges
         return e;
     }
 
-    // The rising bubble in global memory: the predicates, claims and store
-    // updates of the LDS form below, in the same order, with each step's
-    // candidates carried over instead of read again, so that a step waits
-    // on as few dependent loads (each a trip to L2 or HBM) as possible.
+    // The rising bubble of builder.hxx:263-283 from cross edge `basel`.  With
+    // stop < 0 it runs to the upper common tangent.  With a stop edge (a
+    // cross edge already in the store, directed right -> left like basel) it
+    // ends when the next cross edge to connect is the stop edge.  Where the
+    // stop edge is the first candidate around a base vertex, that side has no
+    // candidate inside this piece (between two cross edges sharing a vertex
+    // only further cross edges can lie: an edge of one side there would cross
+    // an edge of the other side).  Returns 1 at the stop edge, 0 at the top,
+    // 2 when a vertex it needs is claimed by another segment: `basel` is then
+    // the cross edge to resume from.
+    //
+    // The left scan deletes only edges at bd between left vertices and the
+    // right scan only edges at bo between right vertices, so neither changes
+    // what the other side reads.  Each step's candidates are carried over
+    // instead of read again, so that a step waits on as few dependent loads
+    // (each a trip to L2 or HBM, or an LDS access in a cell) as possible:
     //  * The side not chosen keeps its candidate, the candidate after it and
     //    their ends and coordinates: the new edge changes neither ring.
     //  * The chosen side's new candidate comes from the connect: on the
@@ -464,7 +476,7 @@ struct QuadEdges
     //    read before the deletion's stores.
     // Where a carried word is one the step itself rewrote (an end vertex of
     // degree one or two), the value written is used instead.
-    __device__ int bubble_carry(int& basel, int stop, Pool& pool, int& budget)
+    __device__ int bubble(int& basel, int stop, Pool& pool, int& budget)
     {
         const int srec = stop >= 0 ? (stop >> 2) : -1;
         const int so = stop >= 0 ? org(stop) : -1;
@@ -606,154 +618,6 @@ struct QuadEdges
                 lnd = dest(ln);
                 Pl = pt(lcd);
                 Pln = pt(lnd);
-            }
-        }
-    }
-
-    // The rising bubble of builder.hxx:263-283 from cross edge `basel`.  With
-    // stop < 0 it runs to the upper common tangent.  With a stop edge (a
-    // cross edge already in the store, directed right -> left like basel) it
-    // ends when the next cross edge to connect is the stop edge.  Where the
-    // stop edge is the first candidate around a base vertex, that side has no
-    // candidate inside this piece (between two cross edges sharing a vertex
-    // only further cross edges can lie: an edge of one side there would cross
-    // an edge of the other side).  Returns 1 at the stop edge, 0 at the top,
-    // 2 when a vertex it needs is claimed by another segment: `basel` is then
-    // the cross edge to resume from.
-    __device__ int bubble(int& basel, int stop, Pool& pool, int& budget)
-    {
-        if constexpr (kChecked) return bubble_carry(basel, stop, pool, budget);
-        const int srec = stop >= 0 ? (stop >> 2) : -1;
-        const int so = stop >= 0 ? org(stop) : -1;
-        const int sd = stop >= 0 ? dest(stop) : -1;
-        // the base edge's ends and their coordinates carry over to the next
-        // step (one end stays, the other is the chosen candidate's)
-        int bo = org(basel), bd = dest(basel);
-        double2 Po = pt(bo), Pd = pt(bd);
-        for (;;)
-        {
-            if (err) return 0;
-            if (--budget < 0)
-            {
-                err = 3;
-                return 0;
-            }
-            if (!take(bo) || !take(bd)) return 2;
-            // The left scan deletes only edges at bd between left vertices
-            // and the right scan only edges at bo between right vertices, so
-            // neither changes what the other side reads.  In global memory
-            // the first two candidates of both sides are loaded before any
-            // is used (their latencies overlap); in LDS the second ones are
-            // loaded only when needed.
-            int lc = onext(basel ^ 2);
-            int rc = oprev(basel);
-            int ln = -1, rp = -1, lnd = -1, rpd = -1;
-            if (kChecked)
-            {
-                ln = onext(lc);
-                rp = oprev(rc);
-            }
-            int lcd = dest(lc), rcd = dest(rc);
-            if (kChecked)
-            {
-                lnd = dest(ln);
-                rpd = dest(rp);
-            }
-            double2 Pl = pt(lcd), Pr = pt(rcd), Pln, Prp;
-            if (kChecked)
-            {
-                Pln = pt(lnd);
-                Prp = pt(rpd);
-            }
-            bool lv = false;
-            if ((lc >> 2) != srec)
-            {
-                lv = ccw_p(Pl, Pd, Po);
-                if (lv)
-                {
-                    bool deleted = false;
-                    for (;;)
-                    {
-                        if (err || --budget < 0) break;
-                        if (!kChecked) ln = onext(lc);
-                        if ((ln >> 2) == srec) break;
-                        if (!kChecked)
-                        {
-                            lnd = dest(ln);
-                            Pln = pt(lnd);
-                        }
-                        if (!in_circle_p(bd, bo, lcd, lnd, Pd, Po, Pl, Pln)) break;
-                        if (!take(lcd)) return 2;
-                        delete_edge(lc, pool);
-                        lc = ln;
-                        lcd = lnd;
-                        Pl = Pln;
-                        if (kChecked)
-                        {
-                            ln = onext(lc);
-                            lnd = dest(ln);
-                            Pln = pt(lnd);
-                        }
-                        deleted = true;
-                    }
-                    if (deleted) lv = ccw_p(Pl, Pd, Po);
-                }
-            }
-            bool rv = false;
-            if ((rc >> 2) != srec)
-            {
-                rv = ccw_p(Pr, Pd, Po);
-                if (rv)
-                {
-                    bool deleted = false;
-                    for (;;)
-                    {
-                        if (err || --budget < 0) break;
-                        if (!kChecked) rp = oprev(rc);
-                        if ((rp >> 2) == srec) break;
-                        if (!kChecked)
-                        {
-                            rpd = dest(rp);
-                            Prp = pt(rpd);
-                        }
-                        if (!in_circle_p(bd, bo, rcd, rpd, Pd, Po, Pr, Prp)) break;
-                        if (!take(rcd)) return 2;
-                        delete_edge(rc, pool);
-                        rc = rp;
-                        rcd = rpd;
-                        Pr = Prp;
-                        if (kChecked)
-                        {
-                            rp = oprev(rc);
-                            rpd = dest(rp);
-                            Prp = pt(rpd);
-                        }
-                        deleted = true;
-                    }
-                    if (deleted) rv = ccw_p(Pr, Pd, Po);
-                }
-            }
-            if (!lv && !rv) return 0;
-            const bool right = !lv || (rv && in_circle_p(lcd, bd, bo, rcd, Pl, Pd, Po, Pr));
-            // the new cross edge would run rcd -> bd or bo -> lcd
-            if (stop >= 0)
-            {
-                const int no = right ? rcd : bo;
-                const int nd = right ? bd : lcd;
-                if (no == so && nd == sd) return 1;
-            }
-            if (!take(right ? rcd : lcd)) return 2;
-            if (right)
-            {
-                basel = connect(rc, basel ^ 2, pool);
-                bo = rcd;
-                Po = Pr;
-            }
-            else
-            {
-                basel = connect(basel ^ 2, lc ^ 2, pool);
-                bd = lcd;
-                Pd = Pl;
             }
         }
     }
