@@ -255,10 +255,14 @@ __device__ int collect_crossings(Q& q, int u, int x, int p, int* out, int cap)
     int g = q.lnext(f);
     int a = q.dest(x), b = q.dest(f);
     double2 A = q.pt(a), B = q.pt(b);
-    if (q.lnext(g) != x)
-        return face_kind(q, x) == 0 ? 0 : -1; // the segment leaves u into the outer face, or a larger face
     {
+        const bool tri = q.lnext(g) == x;
         const double o = q.pred_orient(U, A, B);
+        // Not a 3-cycle: the halves are complete triangulations whose bounded
+        // faces are counter-clockwise triangles, so this is the outer face,
+        // whose boundary turns clockwise at a corner of the hull; only a
+        // straight turn (collinear points) needs the walk of face_kind.
+        if (!tri) return (o < 0 || face_kind(q, x) == 0) ? 0 : -1;
         if (o < 0) return 0;  // the outer face
         if (o == 0) return -1;
         if (ox != (g ^ 2)) return -1;
@@ -277,8 +281,9 @@ __device__ int collect_crossings(Q& q, int u, int x, int p, int* out, int cap)
         g = q.lnext(f);
         const int c = q.dest(f);
         const double2 C = q.pt(c);
-        if (q.lnext(g) != h) return face_kind(q, h) == 0 ? n : -1;
+        const bool tri = q.lnext(g) == h;
         const double o = q.pred_orient(B, A, C);
+        if (!tri) return (o < 0 || face_kind(q, h) == 0) ? n : -1; // as for the first face
         if (o < 0) return n; // the outer face
         if (o == 0) return -1;
         const double oc = q.pred_orient(U, P, C);
