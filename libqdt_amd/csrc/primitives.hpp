@@ -209,17 +209,21 @@ __device__ __forceinline__ uint64_t lanemask_lt()
 // Stable scatter.  Each wave owns a contiguous run of 64*kSortItems keys and
 // walks it in 64-key steps; the 8 ballots give, for each lane, the mask of
 // lanes holding the same digit.  offsets[d * nblocks + b] is the exclusive
-// scan of the per-block histograms.
+// scan of the per-block histograms.  The block's keys are first placed in
+// LDS in their sorted order, so that the global writes of each digit's run
+// are contiguous (coalesced) instead of one scattered 8-byte store per key.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
                                                           uint32_t* vout, uint32_t n, int shift,
                                                           const uint32_t* offsets)
 {
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t bbase[256];
+    __shared__ uint64_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / 64;
     for (int w = 0; w < kWaves; ++w) wcnt[w][threadIdx.x] = 0;
-    bbase[threadIdx.x] = offsets[threadIdx.x * gridDim.x + blockIdx.x];
+    const uint32_t gbase = offsets[threadIdx.x * gridDim.x + blockIdx.x];
     __syncthreads();
 
     const uint32_t wbase = blockIdx.x * kSortTile + wave * 64 * kSortItems;
@@ -252,7 +256,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, c
     }
     __syncthreads();
     {
-        // exclusive prefix of the per-wave counts, per digit
+        // per digit: exclusive prefix of the per-wave counts, then where the
+        // digit's run starts in the block (a scan over the digits); bbase is
+        // the global position of the block's run minus that local start
         uint32_t run = 0;
         for (int w = 0; w < kWaves; ++w)
         {
@@ -260,6 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, c
             wcnt[w][threadIdx.x] = run;
             run += c;
         }
+        uint32_t total;
+        const uint32_t start = block_exclusive_scan(run, total);
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) wcnt[w][threadIdx.x] += start;
+        bbase[threadIdx.x] = gbase - start;
     }
     __syncthreads();
 #pragma unroll
@@ -269,10 +280,20 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, c
         if (i < n)
         {
             const uint32_t dg = (uint32_t)(key[k] >> shift) & 255;
-            const uint32_t pos = bbase[dg] + wcnt[wave][dg] + rank[k];
-            kout[pos] = key[k];
-            vout[pos] = val[k];
+            const uint32_t pos = wcnt[wave][dg] + rank[k];
+            skey[pos] = key[k];
+            sval[pos] = val[k];
         }
+    }
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * kSortTile;
+    const uint32_t cnt = n - b0 < (uint32_t)kSortTile ? n - b0 : (uint32_t)kSortTile;
+    for (uint32_t i = threadIdx.x; i < cnt; i += kBlock)
+    {
+        const uint64_t k = skey[i];
+        const uint32_t pos = bbase[(uint32_t)(k >> shift) & 255] + i;
+        kout[pos] = k;
+        vout[pos] = sval[i];
     }
 }
 
