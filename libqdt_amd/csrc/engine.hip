@@ -193,27 +193,44 @@ __global__ __launch_bounds__(kBlock) void k_ytie(const uint64_t* ysorted, const 
 
 // frame-1 order without a sort: the y order's representatives (a point's
 // lowest-index copy) in that order, as distinct ids
+// distinct-point id of every input index that represents its point (the
+// first of its duplicates), ~0 for the others; in input order, so that the
+// scan below gathers one word per position
+__global__ __launch_bounds__(kBlock) void k_rep_id(const uint32_t* rep, const uint32_t* didx, uint32_t n, uint32_t* w)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) w[i] = rep[i] == i ? didx[i] : ~0u;
+}
+
+// the scan's first pass gathers w in y order once and keeps it by position
+// (g), which the second pass reads back without the gather
 struct RepFlag
 {
     const uint32_t* perm;
-    const uint32_t* rep;
+    const uint32_t* w;
+    uint32_t* g;
     __device__ uint32_t operator()(uint32_t p) const
     {
-        const uint32_t i = perm[p];
-        return rep[i] == i ? 1u : 0u;
+        const uint32_t v = w[perm[p]];
+        g[p] = v;
+        return v != ~0u ? 1u : 0u;
     }
+};
+
+struct RepLoad
+{
+    const uint32_t* g;
+    __device__ uint32_t operator()(uint32_t p) const { return g[p] != ~0u ? 1u : 0u; }
 };
 
 struct RepWrite
 {
-    const uint32_t* perm;
-    const uint32_t* rep;
-    const uint32_t* didx;
+    const uint32_t* g;
     uint32_t* out;
     __device__ void operator()(uint32_t p, uint32_t pre) const
     {
-        const uint32_t i = perm[p];
-        if (rep[i] == i) out[pre] = didx[i];
+        const uint32_t v = g[p];
+        if (v != ~0u) out[pre] = v;
     }
 };
 
@@ -2785,7 +2802,13 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     vA = b.vA;
     vB = b.vB;
     if (!multi && !ytie)
-        scan_u32(RepFlag{ b.Bt, b.rep }, n, RepWrite{ b.Bt, b.rep, b.Z, b.vA }, b.partials, s);
+    {
+        // kB and vB are free here (they serve the sorts of the other branch)
+        uint32_t* w = b.vB;
+        uint32_t* g = reinterpret_cast<uint32_t*>(b.kB);
+        k_rep_id<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.rep, b.Z, (uint32_t)n, w);
+        scan_u32(RepFlag{ b.Bt, w, g }, RepLoad{ g }, n, RepWrite{ g, b.vA }, b.partials, s);
+    }
     else
     {
         k_frame1_keys<<<cdiv(m, kBlock), kBlock, 0, s>>>(b.P0, m, b.kA, b.vA);
