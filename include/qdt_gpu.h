@@ -125,6 +125,40 @@ QDT_API qdt_status qdt_gpu_verify(qdt_t* ptr, qdt_gpu_verify_report* out);
 QDT_API qdt_status qdt_gpu_verify_triangles(const qdt_vec2d* d_pts, size_t n, const uint32_t* d_tris, size_t T,
                                             const uint32_t* d_rep, qdt_gpu_verify_report* out);
 
+/* ---- Half-edge mesh ----
+ *
+ * The half-edge mesh libqdt builds from a batch result before its first
+ * insertion or constraint (GuibasStolfiDwyer_Wrapper.cpp:99-202, ensure_mesh_;
+ * fields of wrapper.hxx:44-71), built on the device.  The reference keeps it
+ * private; here it is an export.  For T triangles (3 T input indices < n,
+ * device memory):
+ *   d_he   (3 T x 4 int32, device): half-edge 3t+i = {origin, twin, next,
+ *          triangle}: it runs from corner i of triangle t (origin: its local
+ *          vertex) to corner (i+1)%3; next = 3t+(i+1)%3; twin = the half-edge
+ *          running the other way along the same edge, -1 on the boundary.
+ *   d_vert (up to min(n, 3 T) x 2 int32, device): local vertex k = {input
+ *          index, half-edge}; local vertices are numbered in order of first
+ *          appearance (triangle t, then corner i), and a vertex's half-edge is
+ *          that first appearance.
+ *   d_orig_to_local (n int32, device): the local vertex of each input
+ *          index, -1 for indices no triangle uses.
+ * The reference pairs twins by scanning each vertex's out-going half-edges;
+ * here twins come from a sort of the undirected edge keys, with the same
+ * result on any triangulation (each edge in one triangle, or in two with
+ * opposite directions).  Edges that are neither stay without twins and are
+ * counted in `unmatched`.  An index >= n: QDT_ERR_INVALID_INDEX. */
+typedef struct qdt_gpu_mesh_info
+{
+    uint64_t vertices;   /* local vertices */
+    uint64_t halfedges;  /* 3 T */
+    uint64_t boundary;   /* half-edges without a twin (one per boundary edge) */
+    uint64_t unmatched;  /* half-edges of edges in more than two triangles or in two with one direction */
+    double device_ms;    /* kernel time of the build */
+} qdt_gpu_mesh_info;
+
+QDT_API qdt_status qdt_gpu_build_mesh(const uint32_t* d_tris, size_t T, size_t n, int32_t* d_he, int32_t* d_vert,
+                                      int32_t* d_orig_to_local, qdt_gpu_mesh_info* out);
+
 /* Number of HIP devices visible to the library (0 without a GPU). */
 QDT_API int qdt_gpu_device_count(void);
 

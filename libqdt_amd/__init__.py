@@ -38,7 +38,7 @@ EXPORTS = [
 GPU_EXPORTS = [
     "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
     "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_set_cells", "qdt_gpu_device_count",
-    "qdt_gpu_release_workspace", "qdt_gpu_verify", "qdt_gpu_verify_triangles",
+    "qdt_gpu_release_workspace", "qdt_gpu_verify", "qdt_gpu_verify_triangles", "qdt_gpu_build_mesh",
     "qdt_gpu_batch_create", "qdt_gpu_batch_destroy", "qdt_gpu_batch_execute", "qdt_gpu_batch_execute_device",
     "qdt_gpu_batch_sets", "qdt_gpu_batch_set_status", "qdt_gpu_batch_triangles_size",
     "qdt_gpu_batch_unique_points", "qdt_gpu_batch_get_triangle", "qdt_gpu_batch_set_range",
@@ -104,6 +104,16 @@ class VerifyReport(ctypes.Structure):
         return d
 
 
+class MeshInfo(ctypes.Structure):
+    _fields_ = [
+        ("vertices", ctypes.c_uint64), ("halfedges", ctypes.c_uint64), ("boundary", ctypes.c_uint64),
+        ("unmatched", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def lib():
     """The loaded C library; raises if it has not been built."""
     global _lib
@@ -156,6 +166,8 @@ def lib():
         L.qdt_gpu_verify.argtypes = [vp, ctypes.POINTER(VerifyReport)]
         L.qdt_gpu_verify_triangles.restype = st
         L.qdt_gpu_verify_triangles.argtypes = [vp, sz, vp, sz, vp, ctypes.POINTER(VerifyReport)]
+        L.qdt_gpu_build_mesh.restype = st
+        L.qdt_gpu_build_mesh.argtypes = [vp, sz, sz, vp, vp, vp, ctypes.POINTER(MeshInfo)]
         L.qdt_gpu_device_count.restype = ctypes.c_int
         L.qdt_gpu_device_count.argtypes = []
         L.qdt_gpu_release_workspace.restype = None
@@ -332,6 +344,21 @@ class Triangulation:
         if st != QDT_OK:
             raise RuntimeError("qdt_gpu_verify: " + status_string(st))
         return r.as_dict()
+
+    def mesh(self):
+        """Half-edge mesh of the last result, built on the device
+        (qdt_gpu_build_mesh): (he (3T, 4) int32 origin/twin/next/triangle,
+        vert (V, 2) int32 input index/half-edge, orig_to_local (n,) int32,
+        info dict).  Device buffers come from torch."""
+        import torch
+        d_tris, T = self.device_triangles()
+        n = self.points_size()
+        he = torch.empty((max(3 * T, 1), 4), dtype=torch.int32, device="cuda")
+        vert = torch.empty((max(min(n, 3 * T), 1), 2), dtype=torch.int32, device="cuda")
+        o2l = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+        info = build_mesh(d_tris, T, n, he.data_ptr(), vert.data_ptr(), o2l.data_ptr())
+        torch.cuda.synchronize()
+        return (he[:3 * T].cpu().numpy(), vert[:info["vertices"]].cpu().numpy(), o2l[:n].cpu().numpy(), info)
 
     def set_profiling(self, on=True):
         lib().qdt_gpu_set_profiling(self._h, 1 if on else 0)
@@ -525,6 +552,17 @@ def verify_triangles(d_pts, n, d_tris, T, d_rep=None):
     st = lib().qdt_gpu_verify_triangles(d_pts, n, d_tris, T, d_rep, ctypes.byref(r))
     if st != QDT_OK:
         raise RuntimeError("qdt_gpu_verify_triangles: " + status_string(st))
+    return r.as_dict()
+
+
+def build_mesh(d_tris, T, n, d_he, d_vert, d_orig_to_local):
+    """Half-edge mesh of T device triangles over n input indices
+    (qdt_gpu_build_mesh) into caller-allocated device arrays (3T x 4, up to
+    min(n, 3T) x 2 and n int32); pointers as integers.  Returns the counts."""
+    r = MeshInfo()
+    st = lib().qdt_gpu_build_mesh(d_tris, T, n, d_he, d_vert, d_orig_to_local, ctypes.byref(r))
+    if st != QDT_OK:
+        raise RuntimeError("qdt_gpu_build_mesh: " + status_string(st))
     return r.as_dict()
 
 

@@ -3163,6 +3163,107 @@ Status Engine::batch_output(uint32_t T, int k, int kg, uint32_t m)
 // exact verifier (host side)
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Half-edge mesh of a triangle list (GuibasStolfiDwyer_Wrapper.cpp:99-202):
+// half-edge 3t+i runs from corner i of triangle t to corner i+1; twins are
+// found by sorting the undirected edge keys (the reference scans each
+// vertex's list of out-going half-edges); local vertices are numbered in
+// order of first appearance by a scan over the half-edges.
+// ---------------------------------------------------------------------------
+
+enum MeshCount
+{
+    kmBoundary = 0,
+    kmUnmatched,
+    kmBadIndex,
+    kmCount
+};
+
+__global__ __launch_bounds__(kBlock) void k_mesh_keys(const uint32_t* tris, uint32_t T, uint32_t n, uint64_t* keys,
+                                                      uint32_t* vals, uint32_t* first, unsigned long long* cnt)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t v[3] = { tris[3 * (size_t)t], tris[3 * (size_t)t + 1], tris[3 * (size_t)t + 2] };
+    const bool ok = v[0] < n && v[1] < n && v[2] < n;
+    if (!ok) atomicAdd(&cnt[kmBadIndex], 1ull);
+    for (int i = 0; i < 3; ++i)
+    {
+        const uint32_t h = 3 * t + i;
+        const uint32_t a = v[i], b = v[(i + 1) % 3];
+        keys[h] = ok ? (((uint64_t)min(a, b) << 32) | max(a, b)) : kNoEdge;
+        vals[h] = h;
+        if (ok) atomicMin(&first[a], h);
+    }
+}
+
+// half-edge h is its origin's first appearance
+struct FirstFlag
+{
+    const uint32_t* tris;
+    const uint32_t* first;
+    uint32_t n;
+    __device__ uint32_t operator()(uint32_t h) const
+    {
+        const uint32_t u = tris[h];
+        return u < n && first[u] == h ? 1u : 0u;
+    }
+};
+
+struct FirstWrite
+{
+    FirstFlag f;
+    int2* vert;
+    int* o2l;
+    __device__ void operator()(uint32_t h, uint32_t pre) const
+    {
+        if (!f(h)) return;
+        const uint32_t u = f.tris[h];
+        vert[pre] = make_int2((int)u, (int)h);
+        o2l[u] = (int)pre;
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void k_mesh_fill(const uint32_t* tris, uint32_t E, uint32_t n, const int* o2l,
+                                                      int4* he)
+{
+    const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
+    if (h >= E) return;
+    const uint32_t t = h / 3, i = h - 3 * t;
+    const uint32_t u = tris[h];
+    he[h] = make_int4(u < n ? o2l[u] : -1, -1, (int)(3 * t + (i + 1) % 3), (int)t);
+}
+
+// per group of equal edge keys: one half-edge is a boundary one, two with
+// opposite directions are twins; anything else stays unmatched
+__global__ __launch_bounds__(kBlock) void k_mesh_twins(const uint32_t* tris, const uint64_t* keys,
+                                                       const uint32_t* vals, uint32_t E, int4* he,
+                                                       unsigned long long* cnt)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= E) return;
+    const uint64_t key = keys[p];
+    if (key == kNoEdge || (p > 0 && keys[p - 1] == key)) return;
+    uint32_t g = 1;
+    while (p + g < E && keys[p + g] == key) ++g;
+    if (g == 1)
+    {
+        atomicAdd(&cnt[kmBoundary], 1ull);
+        return;
+    }
+    if (g == 2)
+    {
+        const uint32_t a = vals[p], b = vals[p + 1];
+        if (tris[a] != tris[b])
+        {
+            he[a].y = (int)b;
+            he[b].y = (int)a;
+            return;
+        }
+    }
+    atomicAdd(&cnt[kmUnmatched], (unsigned long long)g);
+}
+
 namespace
 {
 
@@ -3269,6 +3370,80 @@ Status verify_triangulation(const double* pts_raw, uint32_t n, const uint32_t* t
     // no triangles: right exactly when the distinct points are collinear
     out.ok = out.euler_ok && !out.bad_index && !out.not_representative && !out.bad_orientation && !out.bad_edges &&
         !out.non_delaunay && !out.concave_boundary && (T ? !out.unused_points : !out.off_line);
+    return Status::ok;
+}
+
+Status build_mesh(const uint32_t* tris, uint32_t T, uint32_t n, int4* he, int2* vert, int* orig_to_local,
+                  hipStream_t s, MeshReport& out)
+{
+    out = MeshReport();
+    const size_t E = 3 * (size_t)T;
+    if (E > 0xffffffffull) return Status::internal;
+    out.halfedges = E;
+    const uint32_t nb = cdiv(std::max<size_t>(E, 1), kSortTile);
+    TempBuffers tmp;
+    uint64_t *kA = nullptr, *kB = nullptr;
+    uint32_t *vA = nullptr, *vB = nullptr, *hist = nullptr, *partials = nullptr, *ghist = nullptr, *first = nullptr;
+    unsigned long long* cnt = nullptr;
+    Status st;
+    if ((st = tmp.get(kA, E)) != Status::ok || (st = tmp.get(kB, E)) != Status::ok ||
+        (st = tmp.get(vA, E)) != Status::ok || (st = tmp.get(vB, E)) != Status::ok ||
+        (st = tmp.get(hist, 256 * (size_t)nb)) != Status::ok ||
+        (st = tmp.get(partials, cdiv(std::max<size_t>(256 * (size_t)nb, E), kScanTile) + 2 +
+                                    std::max<size_t>(256 * (size_t)nb, E))) != Status::ok ||
+        (st = tmp.get(ghist, 8 * 256)) != Status::ok || (st = tmp.get(first, n)) != Status::ok ||
+        (st = tmp.get(cnt, kmCount)) != Status::ok)
+        return st;
+    uint32_t* host_ghist = nullptr;
+    if (hipHostMalloc((void**)&host_ghist, 4 * 8 * 256, hipHostMallocDefault) != hipSuccess) return Status::out_of_memory;
+    struct HostFree
+    {
+        void* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{ host_ghist };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    QDT_HIP(hipEventCreate(&e0));
+    QDT_HIP(hipEventCreate(&e1));
+    struct EvFree
+    {
+        hipEvent_t a, b;
+        ~EvFree()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ef{ e0, e1 };
+    QDT_HIP(hipEventRecord(e0, s));
+    QDT_HIP(hipMemsetAsync(cnt, 0, 8 * kmCount, s));
+    if (n)
+    {
+        QDT_HIP(hipMemsetAsync(first, 0xff, 4 * (size_t)n, s));
+        QDT_HIP(hipMemsetAsync(orig_to_local, 0xff, 4 * (size_t)n, s));
+    }
+    uint32_t nv = 0;
+    if (T)
+    {
+        k_mesh_keys<<<cdiv(T, kBlock), kBlock, 0, s>>>(tris, T, n, kA, vA, first, cnt);
+        const FirstFlag ff{ tris, first, n };
+        scan_u32(ff, (uint32_t)E, FirstWrite{ ff, vert, orig_to_local }, partials, s);
+        const uint32_t nbs = cdiv((uint32_t)E, kScanTile);
+        QDT_HIP(hipMemcpyAsync(&nv, partials + nbs, 4, hipMemcpyDeviceToHost, s));
+        k_mesh_fill<<<cdiv(E, kBlock), kBlock, 0, s>>>(tris, (uint32_t)E, n, orig_to_local, he);
+        st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, hist, partials, ghist, host_ghist, s);
+        if (st != Status::ok) return st;
+        k_mesh_twins<<<cdiv(E, kBlock), kBlock, 0, s>>>(tris, kA, vA, (uint32_t)E, he, cnt);
+    }
+    QDT_HIP(hipEventRecord(e1, s));
+    unsigned long long c[kmCount];
+    QDT_HIP(hipMemcpyAsync(c, cnt, 8 * kmCount, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    out.device_ms = ms;
+    out.vertices = nv;
+    out.boundary = c[kmBoundary];
+    out.unmatched = c[kmUnmatched];
+    out.bad_index = c[kmBadIndex];
     return Status::ok;
 }
 

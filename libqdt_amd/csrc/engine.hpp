@@ -105,6 +105,26 @@ struct VerifyReport
 Status verify_triangulation(const double* pts, uint32_t n, const uint32_t* tris, uint32_t T, const uint32_t* rep,
                             hipStream_t s, VerifyReport& out);
 
+// Counts of a half-edge mesh build (build_mesh).
+struct MeshReport
+{
+    uint64_t vertices = 0;  // local vertices (input indices some triangle uses)
+    uint64_t halfedges = 0; // 3 T
+    uint64_t boundary = 0;  // half-edges without a twin
+    uint64_t unmatched = 0; // half-edges of edges in more than two triangles, or in two with the same direction
+    uint64_t bad_index = 0; // triangles with an index >= n
+    double device_ms = 0;
+};
+
+// Half-edge mesh of T triangles (3 T input indices < n, device memory), as
+// built before the first insertion (GuibasStolfiDwyer_Wrapper.cpp:99-202).
+// he: 3 T int4 {origin (local vertex), twin (-1: none), next, triangle};
+// vert: int2 {input index, first half-edge out of it} per local vertex, in
+// order of first appearance (triangle, then corner); orig_to_local: n ints
+// (-1: unused).  Temporary device memory is allocated and freed.
+Status build_mesh(const uint32_t* tris, uint32_t T, uint32_t n, int4* he, int2* vert, int* orig_to_local,
+                  hipStream_t s, MeshReport& out);
+
 class Engine
 {
 public:

@@ -727,6 +727,30 @@ qdt_status qdt_gpu_verify_triangles(const qdt_vec2d* d_pts, size_t n, const uint
     }
 }
 
+qdt_status qdt_gpu_build_mesh(const uint32_t* d_tris, size_t T, size_t n, int32_t* d_he, int32_t* d_vert,
+                              int32_t* d_orig_to_local, qdt_gpu_mesh_info* out)
+{
+    if (!out || (T && (!d_tris || !d_he || !d_vert)) || (n && !d_orig_to_local)) return QDT_ERR_NULL_ARGUMENT;
+    if (n > 100000000u || T > 400000000u) return QDT_ERR_TOO_MANY_POINTS;
+    try
+    {
+        qdt::MeshReport r;
+        const qdt::Status s = qdt::build_mesh(d_tris, (uint32_t)T, (uint32_t)n, (int4*)d_he, (int2*)d_vert,
+                                              (int*)d_orig_to_local, nullptr, r);
+        if (s != qdt::Status::ok) return status_of(s);
+        out->vertices = r.vertices;
+        out->halfedges = r.halfedges;
+        out->boundary = r.boundary;
+        out->unmatched = r.unmatched;
+        out->device_ms = r.device_ms;
+        return r.bad_index ? QDT_ERR_INVALID_INDEX : QDT_OK;
+    }
+    catch (...)
+    {
+        return QDT_ERR_OUT_OF_MEMORY;
+    }
+}
+
 qdt_status qdt_gpu_verify(qdt_t* ptr, qdt_gpu_verify_report* out)
 {
     if (!ptr) return QDT_ERR_NULL_CONTEXT;

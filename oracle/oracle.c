@@ -1131,3 +1131,73 @@ done:
     free(bnext);
     return rc;
 }
+
+/* Triangulation::ensure_mesh_, libqdt/GuibasStolfiDwyer_Wrapper.cpp:99-202. */
+long orc_build_mesh(const uint32_t* tris, size_t ntri, size_t n, int32_t* he, int32_t* vert, int32_t* o2l)
+{
+    const size_t nhe = 3 * ntri;
+    for (size_t i = 0; i < n; ++i) o2l[i] = -1;
+    for (size_t i = 0; i < nhe; ++i)
+        if (tris[i] >= n) return -1;
+    /* local vertices by first appearance (:111-131) */
+    long nv = 0;
+    for (size_t i = 0; i < nhe; ++i)
+    {
+        const uint32_t orig = tris[i];
+        if (o2l[orig] < 0)
+        {
+            o2l[orig] = (int32_t)nv;
+            vert[2 * nv] = (int32_t)orig;
+            vert[2 * nv + 1] = -1;
+            ++nv;
+        }
+    }
+    /* half-edges of each triangle, first half-edge of each vertex (:133-154) */
+    for (size_t t = 0; t < ntri; ++t)
+        for (int i = 0; i < 3; ++i)
+        {
+            const size_t h = 3 * t + (size_t)i;
+            const int32_t v = o2l[tris[h]];
+            he[4 * h] = v;
+            he[4 * h + 1] = -1;
+            he[4 * h + 2] = (int32_t)(3 * t + (size_t)((i + 1) % 3));
+            he[4 * h + 3] = (int32_t)t;
+            if (vert[2 * v + 1] < 0) vert[2 * v + 1] = (int32_t)h;
+        }
+    /* out-going half-edges of each vertex, CSR (:156-172) */
+    int32_t* off = (int32_t*)calloc((size_t)nv + 1, sizeof(int32_t));
+    int32_t* adj = (int32_t*)malloc((nhe ? nhe : 1) * sizeof(int32_t));
+    int32_t* cur = (int32_t*)malloc(((size_t)nv + 1) * sizeof(int32_t));
+    if (!off || !adj || !cur)
+    {
+        free(off);
+        free(adj);
+        free(cur);
+        return -2;
+    }
+    for (size_t i = 0; i < nhe; ++i) ++off[he[4 * i] + 1];
+    for (long v = 0; v < nv; ++v) off[v + 1] += off[v];
+    for (long v = 0; v < nv; ++v) cur[v] = off[v];
+    for (size_t i = 0; i < nhe; ++i) adj[cur[he[4 * i]]++] = (int32_t)i;
+    /* twins: the half-edge out of the far end that comes back (:174-195) */
+    for (size_t i = 0; i < nhe; ++i)
+    {
+        if (he[4 * i + 1] >= 0) continue;
+        const int32_t u = he[4 * i], v = he[4 * he[4 * i + 2]];
+        for (int32_t k = off[v]; k < off[v + 1]; ++k)
+        {
+            const int32_t h2 = adj[k];
+            if (he[4 * h2] != v) continue;
+            if (he[4 * he[4 * h2 + 2]] == u)
+            {
+                he[4 * i + 1] = h2;
+                he[4 * h2 + 1] = (int32_t)i;
+                break;
+            }
+        }
+    }
+    free(off);
+    free(adj);
+    free(cur);
+    return nv;
+}

@@ -72,6 +72,17 @@ size_t orc_hull(const double* xy, size_t n, uint32_t* out);
 int orc_check(const double* xy, size_t n, const uint32_t* tris, size_t ntri,
               char* msg, size_t msglen, int64_t info[4]);
 
+/* Half-edge mesh of a triangle list, restating Triangulation::ensure_mesh_
+ * (libqdt/GuibasStolfiDwyer_Wrapper.cpp:99-202): local vertices in order of
+ * first appearance (triangle, then corner), half-edge 3t+i from corner i to
+ * corner i+1 with next 3t+(i+1)%3 and triangle t, twins found by scanning the
+ * out-going half-edges of the far vertex (CSR adjacency by origin).
+ * he: 4 * 3 ntri int32 {origin, twin, next, triangle}; vert: 2 * (vertices)
+ * int32 {input index, half-edge}, room for min(n, 3 ntri); o2l: n int32
+ * (-1: unused).  Returns the number of local vertices, or -1 for an index
+ * >= n. */
+long orc_build_mesh(const uint32_t* tris, size_t ntri, size_t n, int32_t* he, int32_t* vert, int32_t* o2l);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,9 @@ def lib():
         L.orc_check.restype = ctypes.c_int
         L.orc_check.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
                                 ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int64)]
+        L.orc_build_mesh.restype = ctypes.c_long
+        L.orc_build_mesh.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -240,3 +243,18 @@ def ref_time(pts, parallel=True):
         return dt, R.qdt_triangles_size(q)
     finally:
         R.qdt_destroy(q)
+
+
+def build_mesh(tris, n):
+    """The oracle's half-edge mesh of a triangle list (restates ensure_mesh_):
+    (he (3T, 4) int32 origin/twin/next/triangle, vert (V, 2) int32 input
+    index/half-edge, o2l (n,) int32)."""
+    t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+    T = len(t)
+    he = np.zeros((3 * T, 4), np.int32)
+    vert = np.zeros((max(min(n, 3 * T), 1), 2), np.int32)
+    o2l = np.zeros(max(n, 1), np.int32)
+    nv = lib().orc_build_mesh(t.ctypes.data if T else None, T, n, he.ctypes.data, vert.ctypes.data, o2l.ctypes.data)
+    if nv < 0:
+        raise ValueError("orc_build_mesh: index out of range" if nv == -1 else "orc_build_mesh: out of memory")
+    return he, vert[:nv].copy(), o2l[:n].copy()
