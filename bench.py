@@ -25,6 +25,13 @@ Extra fields:
                 step's result: its verdict and kernel time.
   dropin        qdt_execute() from host memory (H2D, pipeline, D2H, host
                 result store): the PCIe-inclusive rate of the drop-in call.
+  mesh          the half-edge mesh of the last step's result built on the
+                device (qdt_gpu_build_mesh: half-edges, twins, local
+                vertices), median kernel time of 3 builds, and as its CPU
+                baseline the reference's own mesh build (ensure_mesh_parallel_,
+                after its batch call on the same input).  Algorithmic bytes:
+                12 B per triangle read, 16 B per half-edge, 8 B per local
+                vertex and 4 B per input index written.
 
   batched       the batched form: 4,096 sets, sizes log-uniform in [64, 65536]
                 (seed 3), each uniform / Gaussian (sigma 0.1) / 32-cluster
@@ -216,6 +223,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--batch-sets", type=int, default=4096)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -344,6 +352,34 @@ def main():
                    "ok": ver["ok"], "device_ms": round(ver["device_ms"], 3),
                    "non_delaunay": ver["non_delaunay"], "boundary_edges": ver["boundary_edges"]},
     }
+
+    if not args.no_mesh:
+        d_tris, Tm = ctx.device_triangles()
+        he = torch.empty((3 * Tm, 4), dtype=torch.int32, device="cuda")
+        vert = torch.empty((min(n, 3 * Tm), 2), dtype=torch.int32, device="cuda")
+        o2l = torch.empty(n, dtype=torch.int32, device="cuda")
+        infos = [Q.build_mesh(d_tris, Tm, n, he.data_ptr(), vert.data_ptr(), o2l.data_ptr()) for _ in range(4)]
+        mesh_ms = statistics.median(i["device_ms"] for i in infos[1:])
+        nv = int(infos[-1]["vertices"])
+        mesh_bytes = 12.0 * Tm + 16.0 * 3 * Tm + 8.0 * nv + 4.0 * n
+        line["mesh"] = {"call": "qdt_gpu_build_mesh on the last step's triangles (device arrays)",
+                        "ms": round(mesh_ms, 3), "halfedges": 3 * Tm, "vertices": nv,
+                        "boundary": int(infos[-1]["boundary"]), "unmatched": int(infos[-1]["unmatched"]),
+                        "alg_bytes": mesh_bytes, "achieved_gbs": round(mesh_bytes / (mesh_ms * 1e-3) / 1e9, 2)}
+        del he, vert, o2l
+        if not args.no_cpu_baseline:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            try:
+                import pyoracle
+
+                if pyoracle.ref_available():
+                    dt, he_ref = pyoracle.ref_mesh_time(pts, parallel=True)
+                    line["mesh"]["cpu_baseline"] = {
+                        "ms": round(dt * 1e3, 1), "kind": "reference", "cores": min(os.cpu_count() or 1, 64),
+                        "sample": "reference ensure_mesh_parallel_ on its own result of the same input (after an "
+                                  "untimed qdt_execute_parallel), one run; half-edges %d" % he_ref}
+            except Exception as e:
+                line["mesh"]["cpu_baseline"] = {"error": str(e)}
 
     if not args.no_dropin:
         st = ctx.execute(pts)  # warm the host path (pinned buffers)

@@ -227,6 +227,20 @@ def ref_batch_time(pts, offsets, threads):
     return dt, tot.value
 
 
+def ref_mesh_time(pts, parallel=True):
+    """Wall time (s) of the reference's half-edge mesh build (ensure_mesh_ /
+    ensure_mesh_parallel_) after an untimed batch call on pts, and 3 T."""
+    R = ref()
+    a = _xy(pts)
+    he = ctypes.c_uint64()
+    R.ref_mesh_seconds.restype = ctypes.c_double
+    R.ref_mesh_seconds.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+    dt = R.ref_mesh_seconds(a.ctypes.data, a.shape[0], 1 if parallel else 0, ctypes.byref(he))
+    if dt < 0:
+        raise RuntimeError("reference mesh build failed")
+    return dt, he.value
+
+
 def ref_time(pts, parallel=True):
     """Wall time (s) of one reference batch call, without reading results."""
     import time

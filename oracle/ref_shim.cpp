@@ -67,3 +67,29 @@ extern "C" double ref_batch_seconds(const double* xy, const uint64_t* offs, size
     if (total_tris) *total_tris = tris.load();
     return bad.load() ? -1.0 : dt;
 }
+
+// Seconds of the reference's half-edge mesh build (Triangulation::ensure_mesh_
+// or ensure_mesh_parallel_, GuibasStolfiDwyer_Wrapper.cpp:99-398; public
+// members of the context type) on a context that already holds the batch
+// result of xy (qdt_execute_parallel, not timed): the CPU baseline of the
+// device mesh build.  halfedges receives 3 T; -1 on failure.
+extern "C" double ref_mesh_seconds(const double* xy, size_t n, int parallel, uint64_t* halfedges)
+{
+    qdt_t* q = qdt_create();
+    if (!q) return -1.0;
+    if (qdt_execute_parallel(q, reinterpret_cast<const qdt_vec2d*>(xy), n) != QDT_OK)
+    {
+        qdt_destroy(q);
+        return -1.0;
+    }
+    wrapper::Triangulation* T = reinterpret_cast<wrapper::Triangulation*>(q);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (parallel)
+        T->ensure_mesh_parallel_();
+    else
+        T->ensure_mesh_();
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (halfedges) *halfedges = 3 * (uint64_t)T->triangles.size();
+    qdt_destroy(q);
+    return dt;
+}
