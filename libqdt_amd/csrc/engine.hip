@@ -1085,6 +1085,7 @@ __device__ int kd_descend(const int* off, const int* splitv, const double2* PT, 
 
 constexpr int kCross = 16;                     // crossing edges per side of an anchor at most
 constexpr int kRec = 6 + 2 * kCross;           // ints per anchor record: u v xu xv nl nr crossings
+constexpr int kMarkSlotsMax = 4096;            // anchors per level up to which crossings get a thread each
 
 // Two lanes per anchor (adjacent, so in one wavefront): lane side 0 works in
 // the lower child L, lane side 1 in the upper child R.  Every decision both
@@ -1520,6 +1521,52 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mark_cross(const int* OR, i
                 }
             }
         }
+    }
+}
+
+// The same with one thread per (anchor, side, crossing slot), for levels of
+// few anchors: an anchor's atomics then do not wait on each other (the vertex
+// list grows by one atomic per wavefront).
+__global__ __launch_bounds__(kMergeBlock) void k_seg_mark_cross_slots(const int* OR, int total, const int* arec,
+                                                                const int* astate, const int* aedge, uint32_t m,
+                                                                int lim_e, int* owner, int* del, int* alist,
+                                                                int* alist_n, int acap, int* err)
+{
+    const size_t x = (size_t)blockIdx.x * kMergeBlock + threadIdx.x;
+    const size_t t = x / (2 * kCross);
+    const int side = (int)(x / kCross) & 1, i = (int)(x % kCross);
+    bool has = false;
+    int e = 0;
+    if (t < (size_t)total && astate[t] == 2 && aedge[t] >= 0)
+    {
+        const int* r = arec + t * kRec;
+        if (i < r[4 + side])
+        {
+            has = true;
+            e = r[6 + side * kCross + i];
+        }
+    }
+    int w0 = -1, w1 = -1;
+    if (has && (e < 0 || e >= lim_e))
+        atomicOr(err, kSeamErr);
+    else if (has)
+    {
+        atomicCAS(&del[e >> 2], -1, (int)t);
+        const int a = OR[e >> 1], b = OR[(e ^ 2) >> 1];
+        if ((unsigned)a < m && atomicCAS(&owner[a], -1, -3) == -1) w0 = a;
+        if ((unsigned)b < m && atomicCAS(&owner[b], -1, -3) == -1) w1 = b;
+    }
+    // the whole wavefront takes its list slots together
+    const int want = (w0 >= 0 ? 1 : 0) + (w1 >= 0 ? 1 : 0);
+    int at = wave_fetch_add(alist_n, want);
+    for (int w : { w0, w1 })
+    {
+        if (w < 0) continue;
+        if (at < acap)
+            alist[at] = w;
+        else
+            atomicOr(err, kSeamErr);
+        ++at;
     }
 }
 
@@ -2967,8 +3014,13 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             const uint32_t ga = cdiv((size_t)total, kMergeBlock);
             k_seg_mark<<<ga, kMergeBlock, 0, s>>>(b.OR, total, b.arec, b.astate, b.spare, spare_base, spare_cap,
                                                   b.owner, b.aedge, b.alist, b.alist_n, acap, b.err, b.counters);
-            k_seg_mark_cross<<<ga, kMergeBlock, 0, s>>>(b.OR, total, b.arec, b.astate, b.aedge, m, lim_e, b.owner,
-                                                        (int*)b.face, b.alist, b.alist_n, acap, b.err);
+            if (total <= kMarkSlotsMax)
+                k_seg_mark_cross_slots<<<cdiv((size_t)total * 2 * kCross, kMergeBlock), kMergeBlock, 0, s>>>(
+                    b.OR, total, b.arec, b.astate, b.aedge, m, lim_e, b.owner, (int*)b.face, b.alist, b.alist_n,
+                    acap, b.err);
+            else
+                k_seg_mark_cross<<<ga, kMergeBlock, 0, s>>>(b.OR, total, b.arec, b.astate, b.aedge, m, lim_e,
+                                                            b.owner, (int*)b.face, b.alist, b.alist_n, acap, b.err);
             k_seg_rebuild<<<cdiv((size_t)total * 8, kMergeBlock), kMergeBlock, 0, s>>>(
                 b.NX, b.OR, b.VE, m, lim_e, b.arec, b.aedge, b.owner, (const int*)b.face, b.alist, b.alist_n, acap,
                 b.err, b.counters);
