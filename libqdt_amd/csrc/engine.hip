@@ -744,6 +744,31 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
         if (gid[lo] != v) bad = 1;
         A1[i] = (short)lo;
     }
+    // the cell's extent: every coordinate difference of two of its points is
+    // at most wx in x and wy in y, which fixes the filters' bounds for the cell
+    double wx = 0, wy = 0;
+    if constexpr (!Q::kHasExact)
+    {
+        double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+        for (int i = lane; i < s; i += 64)
+        {
+            const double2 p = P[i];
+            x0 = fmin(x0, p.x);
+            x1 = fmax(x1, p.x);
+            y0 = fmin(y0, p.y);
+            y1 = fmax(y1, p.y);
+        }
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1)
+        {
+            x0 = fmin(x0, __shfl_xor(x0, dd, 64));
+            x1 = fmax(x1, __shfl_xor(x1, dd, 64));
+            y0 = fmin(y0, __shfl_xor(y0, dd, 64));
+            y1 = fmax(y1, __shfl_xor(y1, dd, 64));
+        }
+        wx = x1 - x0;
+        wy = y1 - y0;
+    }
     __syncthreads();
     // kd partition of the cell, sub-level by sub-level (absolute level dl + l)
     int L = 0;
@@ -860,6 +885,8 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
         }
     Q q;
     q.init(LNX, LOR, P, LVE, 4 * 3 * s, s);
+    if constexpr (!Q::kHasExact)
+        if (wx > 0 && wy > 0 && wx < INFINITY && wy < INFINITY) static_bounds(wx, wy, q.sb_orient, q.sb_incircle);
     // leaves at sub-level L (absolute dl + L); the partition scratch that
     // shares the records' space is dead by now (the loop above ends with a
     // barrier after its last use)

@@ -432,6 +432,39 @@ QDT_HD inline bool incircle_filter(double ax, double ay, double bx, double by, d
     return (det > bound || -det > bound) && permanent > kFilterFloor;
 }
 
+// The filters' determinants with a bound fixed in advance: valid for points
+// whose coordinate differences are at most wx in x and wy in y in magnitude
+// (a cell's extent).  The permanents are then at most 2 wx wy (orient2d) and
+// 6 wx wy (wx^2 + wy^2) (incircle), so kErr * that bound is at least the
+// run-time bound of the filters above; static_bounds returns both (0: not
+// usable, the run-time filters apply) with a margin that covers the rounding
+// of the computed differences, products and of the bound itself.
+QDT_HD inline void static_bounds(double wx, double wy, double& b_orient, double& b_incircle)
+{
+    const double vx = wx * (1.0 + 0x1p-30), vy = wy * (1.0 + 0x1p-30);
+    const double o = 2.0 * vx * vy, i = 6.0 * (vx * vy) * (vx * vx + vy * vy);
+    const bool ok = o > kFilterFloor && i > kFilterFloor && i < 1e300;
+    b_orient = ok ? kErrOrient * o * (1.0 + 0x1p-30) : 0.0;
+    b_incircle = ok ? kErrIncircle * i * (1.0 + 0x1p-30) : 0.0;
+}
+
+QDT_HD inline double orient2d_det(double ax, double ay, double bx, double by, double cx, double cy)
+{
+    return (ax - cx) * (by - cy) - (ay - cy) * (bx - cx);
+}
+
+QDT_HD inline double incircle_det(double ax, double ay, double bx, double by, double cx, double cy, double dx,
+                                  double dy)
+{
+    const double adx = ax - dx, ady = ay - dy;
+    const double bdx = bx - dx, bdy = by - dy;
+    const double cdx = cx - dx, cdy = cy - dy;
+    const double alift = adx * adx + ady * ady;
+    const double blift = bdx * bdx + bdy * bdy;
+    const double clift = cdx * cdx + cdy * cdy;
+    return alift * (bdx * cdy - cdx * bdy) + blift * (cdx * ady - adx * cdy) + clift * (adx * bdy - bdx * ady);
+}
+
 // Filtered predicates.  The return value's sign is the answer.  nexact counts
 // the calls the filter could not decide.
 QDT_HD inline double orient2d(double ax, double ay, double bx, double by, double cx, double cy,

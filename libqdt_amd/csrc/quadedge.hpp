@@ -68,6 +68,7 @@ struct PredCount
 template <class IntPtr, class PtPtr, bool kChecked = true, bool kExact = true>
 struct QuadEdges
 {
+    static constexpr bool kHasExact = kExact;
     IntPtr NX;
     IntPtr OR;
     PtPtr PT;
@@ -84,6 +85,7 @@ struct QuadEdges
                         // indexed array, which would live in scratch memory)
     int err;
     PredCount pc;
+    double sb_orient, sb_incircle; // static filter bounds (0: run-time bounds), used without the exact stage
 
     __device__ __forceinline__ void init(IntPtr nx, IntPtr orr, PtPtr pt, IntPtr ve, int le, int lv)
     {
@@ -101,6 +103,7 @@ struct QuadEdges
         h0 = h1 = h2 = h3 = -1;
         err = 0;
         pc = PredCount{ 0, 0, 0, 0, 0 };
+        sb_orient = sb_incircle = 0.0;
     }
 
     __device__ __forceinline__ int chk(int e)
@@ -282,6 +285,12 @@ struct QuadEdges
         ++pc.orient;
         if constexpr (kExact) return orient2d(A.x, A.y, B.x, B.y, C.x, C.y, pc.orient_exact);
         double det;
+        if (sb_orient > 0)
+        {
+            det = orient2d_det(A.x, A.y, B.x, B.y, C.x, C.y);
+            if (!(det > sb_orient || -det > sb_orient)) err = err ? err : 5;
+            return det;
+        }
         if (!orient2d_filter(A.x, A.y, B.x, B.y, C.x, C.y, det)) err = err ? err : 5;
         return det;
     }
@@ -290,6 +299,12 @@ struct QuadEdges
         ++pc.incircle;
         if constexpr (kExact) return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact);
         double det;
+        if (sb_incircle > 0)
+        {
+            det = incircle_det(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y);
+            if (!(det > sb_incircle || -det > sb_incircle)) err = err ? err : 5;
+            return det;
+        }
         if (!incircle_filter(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, det)) err = err ? err : 5;
         return det;
     }
