@@ -48,4 +48,20 @@ double ph_incircle(double ax, double ay, double bx, double by, double cx, double
 {
     return qdt::incircle(ax, ay, bx, by, cx, cy, dx, dy, *nexact);
 }
+
+// the filters with bounds fixed from an extent (wx, wy), as the LDS cells use them
+void ph_static_bounds(double wx, double wy, double* b_orient, double* b_incircle)
+{
+    qdt::static_bounds(wx, wy, *b_orient, *b_incircle);
+}
+
+double ph_orient2d_det(double ax, double ay, double bx, double by, double cx, double cy)
+{
+    return qdt::orient2d_det(ax, ay, bx, by, cx, cy);
+}
+
+double ph_incircle_det(double ax, double ay, double bx, double by, double cx, double cy, double dx, double dy)
+{
+    return qdt::incircle_det(ax, ay, bx, by, cx, cy, dx, dy);
+}
 }

@@ -85,3 +85,78 @@ def test_filtered_matches_exact_and_counts():
         assert sgn(L.ph_orient2d(*a, *b, *c, ctypes.byref(cnt))) == orient_true(a, b, c)
         assert sgn(L.ph_incircle(*a, *b, *c, *d, ctypes.byref(cnt))) == incircle_true(a, b, c, d)
     assert cnt.value > 0  # the small-integer half sends ties to the exact path
+
+
+# ---- bounds fixed per cell (the LDS cells' filters) ----
+
+K_EPS = 2.0 ** -53
+K_ERR_ORIENT = (3.0 + 16.0 * K_EPS) * K_EPS
+K_ERR_INCIRCLE = (10.0 + 96.0 * K_EPS) * K_EPS
+
+
+def static_lib():
+    L = lib()
+    d = ctypes.c_double
+    L.ph_static_bounds.restype = None
+    L.ph_static_bounds.argtypes = [d, d, ctypes.POINTER(d), ctypes.POINTER(d)]
+    L.ph_orient2d_det.restype = d
+    L.ph_orient2d_det.argtypes = [d] * 6
+    L.ph_incircle_det.restype = d
+    L.ph_incircle_det.argtypes = [d] * 8
+    return L
+
+
+def box_points(rng, k, x0, y0, wx, wy, near):
+    """k points in [x0, x0 + wx] x [y0, y0 + wy]; with `near`, nearly
+    collinear / cocircular constructions (points of a line or circle, rounded)."""
+    import math
+    if not near:
+        return [(x0 + rng.random() * wx, y0 + rng.random() * wy) for _ in range(k)]
+    cx, cy = x0 + wx / 2, y0 + wy / 2
+    r = min(wx, wy) / 2
+    if rng.random() < 0.5:
+        return [(cx + r * math.cos(t), cy + r * math.sin(t)) for t in (rng.random() * 6.283 for _ in range(k))]
+    return [(x0 + t * wx, y0 + t * wy) for t in (rng.random() for _ in range(k))]
+
+
+def test_static_bounds_dominate_and_decide_right():
+    """Over boxes of many sizes and aspect ratios, with random and nearly
+    degenerate points: the fixed bound is at least the run-time filter's
+    bound (computed here in IEEE doubles as the filter does), and every sign
+    it lets through is the exact one (rationals)."""
+    L = static_lib()
+    rng = random.Random(7)
+    bo, bi = ctypes.c_double(), ctypes.c_double()
+    decided = 0
+    for trial in range(3000):
+        e = rng.choice([1e-9, 1e-3, 1.0, 1e3, 1e12])
+        wx = e * rng.choice([1.0, 0.5, 0.01, 1e-4])
+        wy = e * rng.choice([1.0, 0.3, 0.01, 1e-4])
+        x0, y0 = rng.uniform(-10, 10) * e, rng.uniform(-10, 10) * e
+        pts = box_points(rng, 4, x0, y0, wx, wy, near=trial % 2 == 1)
+        # the extent as the cell computes it
+        xs, ys = [p[0] for p in pts], [p[1] for p in pts]
+        L.ph_static_bounds(max(xs) - min(xs), max(ys) - min(ys), ctypes.byref(bo), ctypes.byref(bi))
+        (ax, ay), (bx, by), (cx, cy), (dx, dy) = pts
+        # orient2d of a, b, c
+        dl = (ax - cx) * (by - cy)
+        dr = (ay - cy) * (bx - cx)
+        det = L.ph_orient2d_det(ax, ay, bx, by, cx, cy)
+        assert det == dl - dr
+        if bo.value > 0:
+            assert bo.value >= K_ERR_ORIENT * (abs(dl) + abs(dr))
+            if det > bo.value or -det > bo.value:
+                decided += 1
+                assert sgn(det) == orient_true(pts[0], pts[1], pts[2])
+        # incircle of a, b, c, d
+        adx, ady, bdx, bdy, cdx, cdy = ax - dx, ay - dy, bx - dx, by - dy, cx - dx, cy - dy
+        al, bl, cl = adx * adx + ady * ady, bdx * bdx + bdy * bdy, cdx * cdx + cdy * cdy
+        perm = (abs(bdx * cdy) + abs(cdx * bdy)) * al + (abs(cdx * ady) + abs(adx * cdy)) * bl + \
+            (abs(adx * bdy) + abs(bdx * ady)) * cl
+        det = L.ph_incircle_det(ax, ay, bx, by, cx, cy, dx, dy)
+        if bi.value > 0:
+            assert bi.value >= K_ERR_INCIRCLE * perm
+            if det > bi.value or -det > bi.value:
+                decided += 1
+                assert sgn(det) == incircle_true(*pts)
+    assert decided > 2000
