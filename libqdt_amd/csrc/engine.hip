@@ -1951,9 +1951,15 @@ __global__ __launch_bounds__(kBlock) void k_extract_test(const int* NX, const in
         for (int r = 0; r < 4; r += 2)
         {
             const int e = 4 * (int)k + r;
-            int f, g, a, b, c;
-            if (!face_of(NX, OR, e, lim_e, (int)m, f, g, a, b, c)) continue;
-            if (e > f || e > g) continue;
+            // the face is reported by its smallest edge: stop at the first
+            // smaller edge of the cycle before reading further
+            const int f = e_rot(NX[e_invrot(e)]);
+            if (f < e || f >= lim_e) continue;
+            const int g = e_rot(NX[e_invrot(f)]);
+            if (g < e || g >= lim_e) continue;
+            if (e_rot(NX[e_invrot(g)]) != e) continue;
+            const int a = OR[e >> 1], b = OR[f >> 1], c = OR[g >> 1];
+            if ((unsigned)a >= m || (unsigned)b >= m || (unsigned)c >= m) continue;
             const double2 A = PT[a], B = PT[b], C = PT[c];
             const double detleft = (A.x - C.x) * (B.y - C.y);
             const double detright = (A.y - C.y) * (B.x - C.x);
