@@ -1118,13 +1118,17 @@ constexpr int kMarkSlotsMax = 4096;            // anchors per level up to which 
 // the lower child L, lane side 1 in the upper child R.  Every decision both
 // lanes take depends only on values they exchange, so a pair never diverges
 // and its shuffles always find the partner active; the two walks, ring scans
-// and dry runs of an anchor overlap their memory latencies.
+// and dry runs of an anchor overlap their memory latencies.  One more thread
+// per node, after all the anchors' lanes, walks the node's lower common
+// tangent (reads only, like the dry runs; its result goes to nstate, and
+// k_seg_tangent makes the connect), so that walk overlaps the anchors.
 __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, const double2* PT, int* VE,
                                                             const int* off, const int* splitv, const int* qv,
                                                             const uint32_t* B0, const uint32_t* B1, const int* kn,
                                                             int D, int d, int K, uint32_t m,
                                                             int lim_e, int* arec, double4* acoord, double* akey,
-                                                            int* astate, unsigned long long* counters)
+                                                            int* astate, const int4* res, int4* nstate, int* err,
+                                                            unsigned long long* counters)
 {
     const uint32_t lane2 = blockIdx.x * kMergeBlock + threadIdx.x;
     const uint32_t t = lane2 >> 1; // anchor
@@ -1132,7 +1136,24 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
     unsigned long long certified = 0, dry_ok = 0;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
-    if (t < lcount(off, d) * (uint32_t)K)
+    const uint32_t nanch = lcount(off, d) * (uint32_t)K;
+    if (lane2 >= 2 * nanch && lane2 < 2 * nanch + lcount(off, d))
+    {
+        const uint32_t j = lane2 - 2 * nanch;
+        const int* od = off + off_base(off, d);
+        const int s = od[j + 1] - od[j];
+        if (s >= 4)
+        {
+            const int4 L = res[node_index(off, d + 1, 2 * j)];
+            const int4 R = res[node_index(off, d + 1, 2 * j + 1)];
+            int ldi = L.y, rdi = R.x;
+            int budget = 16 * s + 64;
+            q.tangent_walk(ldi, rdi, budget);
+            nstate[j] = make_int4(ldi, rdi, 0, 0);
+            if (q.err) atomicOr(err, 2 << q.err);
+        }
+    }
+    else if (t < nanch)
     {
         const uint32_t j = t / K;
         const int k = (int)(t % K);
@@ -1221,7 +1242,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
     flush_counts(counters, q.pc);
 }
 
-// lower common tangent of every segmented node (the first cut)
+// lower common tangent of every segmented node (the first cut): the connect
+// of the walk that k_seg_anchor made
 __global__ __launch_bounds__(kMergeBlock) void k_seg_tangent(int* NX, int* OR, const double2* PT, int* VE,
                                                              const int* off, int d, uint32_t m, int lim_e,
                                                              unsigned* spare, int spare_base, int spare_cap,
@@ -1244,8 +1266,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_tangent(int* NX, int* OR, c
             const int4 R = res[node_index(off, d + 1, 2 * j + 1)];
             Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
             int ldo = L.x, rdo = R.y;
-            int budget = 16 * s + 64;
-            segstart[(size_t)j * kSegSlots] = q.tangent(ldo, L.y, R.x, rdo, pool, budget);
+            const int4 w = nstate[j]; // the tangent's walk, done by k_seg_anchor
+            segstart[(size_t)j * kSegSlots] = q.tangent_connect(ldo, w.x, w.y, rdo, pool);
             nstate[j] = make_int4(ldo, rdo, pool.head, pool.tail);
         }
     }
@@ -3032,9 +3054,9 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         else
         {
             const int total = (int)(cnt * (uint32_t)K);
-            k_seg_anchor<<<cdiv(2 * (size_t)total, kMergeBlock), kMergeBlock, 0, s>>>(
+            k_seg_anchor<<<cdiv(2 * (size_t)total + cnt, kMergeBlock), kMergeBlock, 0, s>>>(
                 b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, b.kn, Dk, d, K, m, lim_e, b.arec, b.acoord, b.akey,
-                b.astate, b.counters);
+                b.astate, b.res, b.nstate, b.err, b.counters);
             k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
                                                                           b.spare, spare_base, spare_cap, b.res,
                                                                           b.segstart, b.nstate, b.err, b.counters);

@@ -400,8 +400,9 @@ struct QuadEdges
         }
     }
 
-    // Lower common tangent and first cross edge of a merge (builder.hxx:253-261).
-    __device__ __forceinline__ int tangent(int& ldo, int ldi, int rdi, int& rdo, Pool& pool, int& budget)
+    // Lower common tangent and first cross edge of a merge (builder.hxx:253-261):
+    // the walk (reads only) and the connect.
+    __device__ __forceinline__ void tangent_walk(int& ldi, int& rdi, int& budget)
     {
         for (;;)
         {
@@ -413,10 +414,18 @@ struct QuadEdges
             else
                 break;
         }
+    }
+    __device__ __forceinline__ int tangent_connect(int& ldo, int ldi, int rdi, int& rdo, Pool& pool)
+    {
         const int basel = connect(rdi ^ 2, ldi, pool);
         if (org(ldi) == org(ldo)) ldo = basel ^ 2;
         if (org(rdi) == org(rdo)) rdo = basel;
         return basel;
+    }
+    __device__ __forceinline__ int tangent(int& ldo, int ldi, int rdi, int& rdo, Pool& pool, int& budget)
+    {
+        tangent_walk(ldi, rdi, budget);
+        return tangent_connect(ldo, ldi, rdi, rdo, pool);
     }
 
     // Deletion of e when oprev(e) = po and onext(e) = na are known (eo, ed:
