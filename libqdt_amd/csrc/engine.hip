@@ -3299,8 +3299,10 @@ __global__ __launch_bounds__(kBlock) void k_mesh_keys(const uint32_t* tris, uint
         const uint32_t h = 3 * t + i;
         const uint32_t a = v[i], b = v[(i + 1) % 3];
         keys[h] = ok ? (((uint64_t)min(a, b) << 32) | max(a, b)) : kNoEdge;
-        vals[h] = h;
-        if (ok) atomicMin(&first[a], h);
+        vals[h] = h | (a > b ? 0x80000000u : 0u); // the half-edge and its direction
+        // first appearances only decrease, so a value read as already lower
+        // (possibly stale, then higher than the current one) needs no atomic
+        if (ok && first[a] > h) atomicMin(&first[a], h);
     }
 }
 
@@ -3343,7 +3345,7 @@ __global__ __launch_bounds__(kBlock) void k_mesh_fill(const uint32_t* tris, uint
 
 // per group of equal edge keys: one half-edge is a boundary one, two with
 // opposite directions are twins; anything else stays unmatched
-__global__ __launch_bounds__(kBlock) void k_mesh_twins(const uint32_t* tris, const uint64_t* keys,
+__global__ __launch_bounds__(kBlock) void k_mesh_twins(const uint64_t* keys,
                                                        const uint32_t* vals, uint32_t E, int4* he,
                                                        unsigned long long* cnt)
 {
@@ -3361,10 +3363,11 @@ __global__ __launch_bounds__(kBlock) void k_mesh_twins(const uint32_t* tris, con
     if (g == 2)
     {
         const uint32_t a = vals[p], b = vals[p + 1];
-        if (tris[a] != tris[b])
+        if ((a ^ b) >> 31) // opposite directions
         {
-            he[a].y = (int)b;
-            he[b].y = (int)a;
+            const uint32_t ha = a & 0x7fffffffu, hb = b & 0x7fffffffu;
+            he[ha].y = (int)hb;
+            he[hb].y = (int)ha;
             return;
         }
     }
@@ -3485,7 +3488,7 @@ Status build_mesh(const uint32_t* tris, uint32_t T, uint32_t n, int4* he, int2* 
 {
     out = MeshReport();
     const size_t E = 3 * (size_t)T;
-    if (E > 0xffffffffull) return Status::internal;
+    if (E >= 0x80000000ull) return Status::internal; // half-edge ids and a direction bit share 32 bits
     out.halfedges = E;
     const uint32_t nb = cdiv(std::max<size_t>(E, 1), kSortTile);
     TempBuffers tmp;
@@ -3538,7 +3541,7 @@ Status build_mesh(const uint32_t* tris, uint32_t T, uint32_t n, int4* he, int2* 
         k_mesh_fill<<<cdiv(E, kBlock), kBlock, 0, s>>>(tris, (uint32_t)E, n, orig_to_local, he);
         st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, hist, partials, ghist, host_ghist, s);
         if (st != Status::ok) return st;
-        k_mesh_twins<<<cdiv(E, kBlock), kBlock, 0, s>>>(tris, kA, vA, (uint32_t)E, he, cnt);
+        k_mesh_twins<<<cdiv(E, kBlock), kBlock, 0, s>>>(kA, vA, (uint32_t)E, he, cnt);
     }
     QDT_HIP(hipEventRecord(e1, s));
     unsigned long long c[kmCount];
