@@ -567,14 +567,19 @@ __global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const 
     if (q.err) atomicOr(err, 2 << q.err);
 }
 
+// One merge per lane on the first kMergeLanes lanes of each wavefront: the
+// merges are diverging chains of dependent loads, and a level has only a few
+// thousand of them, so fewer per wavefront spread them over more SIMDs.
+constexpr uint32_t kMergeLanes = 16;
+
 __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const double2* PT, int* VE, const int* off,
                                                        int d, uint32_t m, int lim_e, int4* res, int* err,
                                                        unsigned long long* counters)
 {
-    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
+    const uint32_t j = blockIdx.x * kMergeLanes + threadIdx.x;
     GQE q;
     q.init(NX, OR, PT, VE, lim_e, (int)m);
-    if (j < lcount(off, d))
+    if (threadIdx.x < kMergeLanes && j < lcount(off, d))
     {
         const int* od = off + off_base(off, d);
         const int s = od[j + 1] - od[j];
@@ -3049,7 +3054,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         if (!cnt)
             ;
         else if (!K)
-            k_merge<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res,
+            k_merge<<<cdiv(cnt, kMergeLanes), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res,
                                                                     b.err, b.counters);
         else
         {
