@@ -4,8 +4,9 @@ Correct results of the GPU path pass on every input kind (random, clustered,
 lattice with duplicates and cocircular points, collinear, tiny); lists with a
 mistake planted on the host (a reversed triangle, a flipped diagonal, a
 dropped or repeated triangle, an index out of range, a duplicate's
-non-lowest index) fail with the matching count.  Every case prints its name
-before it runs."""
+non-lowest index) fail with the matching count.  The largest input a context
+accepts (100M points) passes too.  Every case prints its name before it
+runs."""
 import numpy as np
 import pytest
 
@@ -138,3 +139,25 @@ def test_large_inputs_pass(gpu):
             st = t.stats()
         assert r["ok"], (name, r)
         assert r["vertices"] == st["unique_points"] and r["triangles"] == st["triangles"]
+
+
+def test_maximum_size_passes(gpu):
+    # the largest input a context accepts (wrapper.hxx:15, kMaxPoints =
+    # 100,000,000): the whole path at that size, checked exactly on the device
+    n = 100_000_000
+    print("case: uniform", n, flush=True)
+    import torch
+
+    pts = W.uniform(n, 7)
+    d = dev(pts)
+    del pts
+    with Q.Triangulation() as t:
+        assert t.execute_device(d.data_ptr(), n) == Q.QDT_OK
+        r = t.verify()
+        st = t.stats()
+    del d
+    torch.cuda.empty_cache()
+    assert r["ok"], r
+    assert st["unique_points"] == n and r["vertices"] == n
+    # Euler with b boundary vertices: T = 2n - 2 - b
+    assert r["triangles"] == st["triangles"] == 2 * n - 2 - r["boundary_edges"]
