@@ -651,7 +651,7 @@ __device__ __forceinline__ bool cell_node(const int* off, int dc, int D, uint32_
 // LDS of a cell of at most cap points (bytes), laid out by CellLds
 __host__ __device__ constexpr size_t cell_lds_bytes(int cap)
 {
-    return 16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 6 * (size_t)cap +
+    return 16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 4 * (size_t)cap +
         2 * (size_t)(2 * cap + kCellLevels + 2) + 8 * (size_t)(2 * cap) + 64;
 }
 
@@ -660,8 +660,8 @@ struct CellLds
     double2* P;   // cap points, frame-0 order
     short* LNX;   // 3 cap records, onext of the 4 directed edges each
     short* LOR;   // origins of the 2 primal edges of each record
-    short *A0, *A1; // the cell's positions in frame-0 / frame-1 order
-    short* LVE;   // vertex -> an edge out of it
+    short *A0, *A1; // the cell's positions in frame-0 / frame-1 order (after the
+                    // merges: an int per vertex, its lowest out-going edge)
     short* loff;  // sub-level node offsets
     short4* lres; // sub-level node results (2 cap slots)
     // partition scratch, inside the record area (unused until the leaves)
@@ -675,9 +675,8 @@ struct CellLds
         LOR = LNX + 12 * cap;
         A0 = LOR + 6 * cap;
         A1 = A0 + cap;
-        LVE = A1 + cap;
-        loff = LVE + cap;
-        lres = (short4*)(lds + ((16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 6 * (size_t)cap +
+        loff = A1 + cap;
+        lres = (short4*)(lds + ((16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 4 * (size_t)cap +
                                  2 * (size_t)(2 * cap + kCellLevels + 2) + 7) & ~(size_t)7));
         unsigned char* x = (unsigned char*)LNX; // 24 cap bytes
         T = (short*)x;
@@ -698,7 +697,7 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
     __shared__ int bad;
     const CellLds W(lds, cap);
     double2* P = W.P;
-    short *A0 = W.A0, *A1 = W.A1, *T = W.T, *Z = W.Z, *loff = W.loff, *LNX = W.LNX, *LOR = W.LOR, *LVE = W.LVE;
+    short *A0 = W.A0, *A1 = W.A1, *T = W.T, *Z = W.Z, *loff = W.loff, *LNX = W.LNX, *LOR = W.LOR;
     uint8_t* right = W.right;
     int* gid = W.gid;
     short4* lres = W.lres;
@@ -724,7 +723,6 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
         gid[i] = v;
         P[i] = PT[v];
         A0[i] = (short)i;
-        LVE[i] = -1;
     }
     if (lane == 0)
     {
@@ -889,7 +887,9 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
             tj = pj;
         }
     Q q;
-    q.init(LNX, LOR, P, LVE, 4 * 3 * s, s);
+    // no vertex-edge upkeep inside the cell: the entries are derived from the
+    // final records below
+    q.init(LNX, LOR, P, nullptr, 4 * 3 * s, s);
     if constexpr (!Q::kHasExact)
         if (wx > 0 && wy > 0 && wx < INFINITY && wy < INFINITY) static_bounds(wx, wy, q.sb_orient, q.sb_incircle);
     // leaves at sub-level L (absolute dl + L); the partition scratch that
@@ -946,6 +946,20 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
     }
     // a filter that could not decide: nothing is written, the caller lists the cell
     if (__ballot(q.err == 5) != 0) return false;
+    // an edge out of every vertex, the lowest id among its edges (so the
+    // choice does not depend on lane timing), in the frame orders' space
+    // (A0, A1: 2 cap shorts, dead since the leaves)
+    int* ve = (int*)A0;
+    for (int i = lane; i < s; i += 64) ve[i] = INT_MAX;
+    __syncthreads();
+    for (int k = lane; k < 3 * s; k += 64)
+    {
+        const int o0 = LOR[2 * k];
+        if (o0 < 0) continue;
+        atomicMin(&ve[o0], 4 * k);
+        atomicMin(&ve[LOR[2 * k + 1]], 4 * k + 2);
+    }
+    __syncthreads();
     // write the records, vertex edges and the cell's result
     const int E0 = 12 * o; // local edge e is global E0 + e
     const int R0 = 3 * o;  // local record k is global R0 + k
@@ -965,7 +979,7 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
         OR[2 * g] = (int)B0[o + o0];
         OR[2 * g + 1] = (int)B0[o + o1];
     }
-    for (int i = lane; i < s; i += 64) VE[B0[o + i]] = LVE[i] >= 0 ? E0 + LVE[i] : -1;
+    for (int i = lane; i < s; i += 64) VE[B0[o + i]] = ve[i] != INT_MAX ? E0 + ve[i] : -1;
     if (lane == 0)
     {
         const short4 r = lres[0];
