@@ -62,7 +62,8 @@ struct PredCount
 // kChecked = false drops the index checks: for stores in LDS, where an index
 // out of range cannot fault (the hardware discards such writes and returns 0
 // for such reads) and the checks cost a quarter of the merge instructions.
-// The loop budgets stay, so a damaged store still cannot spin forever.
+// The loop budgets stay, so a damaged store still cannot spin forever.  Such
+// stores (the LDS cells) also take no claims and keep no vertex-edge entries.
 // kExact = false leaves out the exact predicate stage (and with it the calls
 // that raise every kernel's register count): an undecided filter sets err 5.
 template <class IntPtr, class PtPtr, bool kChecked = true, bool kExact = true>
@@ -134,13 +135,18 @@ struct QuadEdges
         }
         return PT[v];
     }
+    // (stores in LDS, the cells, keep no vertex-edge entries and no claims:
+    // their entries are derived from the final records, and one thread owns
+    // each merge)
     __device__ __forceinline__ void set_ve(int v, int e)
     {
+        if constexpr (!kChecked) return;
         if (VE && (unsigned)v < (unsigned)lim_v) VE[v] = e;
     }
     // true when this segment may touch v's ring (claims it on first use)
     __device__ __forceinline__ bool take(int v)
     {
+        if constexpr (!kChecked) return true;
         if (!claim) return true;
         if (v == h0 || v == h1 || v == h2 || v == h3) return true;
         if ((unsigned)v >= (unsigned)lim_v)
