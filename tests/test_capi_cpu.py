@@ -124,3 +124,16 @@ def test_server_needs_a_gpu():
     assert Q.lib().qdt_gpu_server_create(1 << 30, 2) is None
     assert Q.lib().qdt_gpu_server_poll(None, 1) == -1
     assert Q.lib().qdt_gpu_server_wait(None, 1, None) == Q.QDT_ERR_NULL_CONTEXT
+
+
+def test_mesh_argument_checks_need_no_gpu():
+    """qdt_gpu_build_mesh: NULL and size checks come before any device work."""
+    import ctypes
+
+    L = Q.lib()
+    info = Q.MeshInfo()
+    assert L.qdt_gpu_build_mesh(None, 0, 0, None, None, None, None) == Q.QDT_ERR_NULL_ARGUMENT
+    assert L.qdt_gpu_build_mesh(None, 5, 10, None, None, None, ctypes.byref(info)) == Q.QDT_ERR_NULL_ARGUMENT
+    assert L.qdt_gpu_build_mesh(None, 0, 10, None, None, None, ctypes.byref(info)) == Q.QDT_ERR_NULL_ARGUMENT
+    big = 500_000_000
+    assert L.qdt_gpu_build_mesh(8, big, 10, 8, 8, 8, ctypes.byref(info)) == Q.QDT_ERR_TOO_MANY_POINTS
