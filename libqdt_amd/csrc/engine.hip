@@ -522,6 +522,36 @@ __global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const 
     bits[v] = (uint8_t)((pos - oc) >= left_size(sc) ? 1 : 0);
 }
 
+// Vertex ids renumbered into the order of the frame-0 array at the cell
+// level: every kd node's points get a contiguous range of ids (their
+// positions), so the coordinates that the cells, merges, anchor walks and
+// seam pieces read lie together in memory (as x-ranks, a node's points are
+// spread over the whole width of its slab).  newid[v] = p for v = B0[p];
+// coordinates and original indices are written in the new order (into the
+// spare buffers, which then swap roles with the old ones) and B0 becomes the
+// identity.  B1 keeps the old ids: its readers map them through newid.
+__global__ __launch_bounds__(kBlock) void k_renum(uint32_t* B0, uint32_t m, const double2* P0, const uint32_t* ID0,
+                                                  uint32_t* newid, double2* P0n, uint32_t* ID0n)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const uint32_t v = B0[p];
+    newid[v] = p;
+    P0n[p] = P0[v];
+    ID0n[p] = ID0[v];
+    B0[p] = p;
+}
+
+// vertex ids in a[0, na) mapped (entries that are not ids are left alone)
+__global__ __launch_bounds__(kBlock) void k_renum_ids(const uint32_t* newid, uint32_t m, int* a, size_t na)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < na; i += (size_t)gridDim.x * kBlock)
+    {
+        const int v = a[i];
+        if ((unsigned)v < m) a[i] = (int)newid[v];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 6-7: leaves and merges
 // ---------------------------------------------------------------------------
@@ -690,8 +720,8 @@ struct CellLds
 // and some filter could not decide.
 template <class Q>
 __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, int* OR, const double2* PT, int* VE,
-                           const uint32_t* B0, const uint32_t* B1, const int* off, int dc, int D, int4* res,
-                           int* err, unsigned long long* counters)
+                           const uint32_t* B0, const uint32_t* B1, const uint32_t* newid, const int* off, int dc,
+                           int D, int4* res, int* err, unsigned long long* counters)
 {
     __shared__ int lbase[kCellLevels + 1]; // first offset of each sub-level in loff
     __shared__ int bad;
@@ -732,20 +762,13 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
         loff[1] = (short)s;
     }
     __syncthreads();
+    // frame 1 holds the ids from before the renumbering: newid maps them to
+    // positions in frame 0, which are the cell's ids
     for (int i = lane; i < s; i += 64)
     {
-        const int v = (int)B1[o + i];
-        int lo = 0, hi = s - 1;
-        while (lo < hi)
-        {
-            const int mid = (lo + hi) >> 1;
-            if (gid[mid] < v)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        if (gid[lo] != v) bad = 1;
-        A1[i] = (short)lo;
+        const int lo = (int)newid[B1[o + i]] - o;
+        if (lo < 0 || lo >= s || gid[lo] != o + lo) bad = 1;
+        A1[i] = (short)(lo < 0 || lo >= s ? 0 : lo);
     }
     // the cell's extent: every coordinate difference of two of its points is
     // at most wx in x and wy in y, which fixes the filters' bounds for the cell
@@ -994,11 +1017,12 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
 
 // every cell, filters only; undecided cells go to redo (count, then ids)
 __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
-                                             const uint32_t* B1, const int* off, int dc, int D, int cap, int4* res,
+                                             const uint32_t* B1, const uint32_t* newid, const int* off, int dc, int D,
+                                             int cap, int4* res,
                                              int* redo, int* err, unsigned long long* counters)
 {
     extern __shared__ __align__(16) unsigned char cell_lds[];
-    if (!cell_build<LQE>(cell_lds, cap, blockIdx.x, NX, OR, PT, VE, B0, B1, off, dc, D, res, err, counters) &&
+    if (!cell_build<LQE>(cell_lds, cap, blockIdx.x, NX, OR, PT, VE, B0, B1, newid, off, dc, D, res, err, counters) &&
         threadIdx.x == 0)
     {
         const int k = atomicAdd(redo, 1);
@@ -1008,7 +1032,8 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
 
 // the listed cells again, with the exact predicates
 __global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
-                                                   const uint32_t* B1, const int* off, int dc, int D, int cap,
+                                                   const uint32_t* B1, const uint32_t* newid, const int* off, int dc,
+                                                   int D, int cap,
                                                    int4* res, const int* redo, int* err,
                                                    unsigned long long* counters)
 {
@@ -1016,7 +1041,7 @@ __global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const doubl
     const int n = redo[0];
     for (int i = blockIdx.x; i < n; i += gridDim.x)
     {
-        cell_build<LQX>(cell_lds, cap, (uint32_t)redo[1 + i], NX, OR, PT, VE, B0, B1, off, dc, D, res, err,
+        cell_build<LQX>(cell_lds, cap, (uint32_t)redo[1 + i], NX, OR, PT, VE, B0, B1, newid, off, dc, D, res, err,
                         counters);
         __syncthreads();
     }
@@ -1111,7 +1136,8 @@ __device__ __forceinline__ bool less_frame_pt(double2 a, double2 b, int frame)
 // side (0: the lower child, which faces the cut with its largest points in
 // that frame; 1: the upper child), so that the walks toward p start close.
 __device__ int kd_descend(const int* off, const int* splitv, const double2* PT, const uint32_t* B0,
-                          const uint32_t* B1, int D, int dd, uint32_t jj, double2 p, int axis, int side)
+                          const uint32_t* B1, const uint32_t* newid, int D, int dd, uint32_t jj, double2 p, int axis,
+                          int side)
 {
     while (dd < D)
     {
@@ -1126,7 +1152,8 @@ __device__ int kd_descend(const int* off, const int* splitv, const double2* PT, 
     const int o = od[jj], e = od[jj + 1];
     if (e <= o) return -1;
     const uint32_t* B = axis ? B1 : B0;
-    return (int)B[side ? o : e - 1];
+    const uint32_t v = B[side ? o : e - 1];
+    return (int)(axis && newid ? newid[v] : v); // B1 holds ids of the order before renumbering
 }
 
 constexpr int kCross = 16;                     // crossing edges per side of an anchor at most
@@ -1143,7 +1170,8 @@ constexpr int kMarkSlotsMax = 4096;            // anchors per level up to which 
 // k_seg_tangent makes the connect), so that walk overlaps the anchors.
 __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, const double2* PT, int* VE,
                                                             const int* off, const int* splitv, const int* qv,
-                                                            const uint32_t* B0, const uint32_t* B1, const int* kn,
+                                                            const uint32_t* B0, const uint32_t* B1,
+                                                            const uint32_t* newid, const int* kn,
                                                             int D, int d, int K, uint32_t m,
                                                             int lim_e, int* arec, double4* acoord, double* akey,
                                                             int* astate, const int4* res, int4* nstate, int* err,
@@ -1187,7 +1215,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
             const double Y = frame_y(PT[qv[ni * kSegMaxK + k]], axis);
             double2 p = from_frame(c, Y, axis);
             // own side: x (nearest vertex) and ex (an edge out of it)
-            const int st = kd_descend(off, splitv, PT, B0, B1, D, d + 1, 2 * j + side, p, axis, side);
+            const int st = kd_descend(off, splitv, PT, B0, B1, newid, D, d + 1, 2 * j + side, p, axis, side);
             int ex = -1;
             int x = st >= 0 ? nn_walk(q, st, p, 512, ex) : -1;
             int xo = __shfl_xor(x, 1), exo = __shfl_xor(ex, 1);
@@ -2295,6 +2323,7 @@ struct Engine::Bufs
     uint32_t *vA, *vB, *dist, *ID0, *rep, *B0, *B1, *Bt, *nodeA, *nodeB, *Z, *tris, *face, *queue;
     uint32_t *partials, *hist, *ghist, *vrank, *rin, *one_set;
     double2* P0;
+    double2* P1; // the other order's coordinates (vertex ids renumbered for the cells)
     uint8_t *bits, *flagp;
     int *NX, *OR;
     unsigned long long* counters;
@@ -2514,7 +2543,7 @@ Status Engine::ensure(size_t n)
         { (void**)&b.hist, 4 * 256 * (size_t)nb_sort },
         { (void**)&b.ghist, 4 * 8 * 256 },
         { (void**)&b.vrank, 4 * N },    { (void**)&b.rin, 4 * N },
-        { (void**)&b.P0, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.flagp, N },         { (void**)&b.NX, 4 * 4 * R },
+        { (void**)&b.P0, 16 * N },      { (void**)&b.P1, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.flagp, N },         { (void**)&b.NX, 4 * 4 * R },
         { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
         { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },
         { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },
@@ -3012,6 +3041,21 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                 k_roots<<<cdiv((size_t)(rb - ra) + 1, kBlock), kBlock, 0, s>>>(b.off, d, ra, rb, vstart);
         }
 
+    // vertex ids in the cells' order; newid lives in the dedupe's x keys
+    // (free since the dedupe) until the merges end, the old coordinates and
+    // original indices become the spare buffers of the next run
+    const uint32_t* newid = nullptr;
+    if (use_cells)
+    {
+        uint32_t* nid = reinterpret_cast<uint32_t*>(b.kx);
+        k_renum<<<cdiv(m, kBlock), kBlock, 0, s>>>(B0, m, b.P0, b.ID0, nid, b.P1, b.nodeB);
+        std::swap(b.P0, b.P1);
+        std::swap(b.ID0, b.nodeB);
+        k_renum_ids<<<1024, kBlock, 0, s>>>(nid, m, b.splitv, (size_t)P.nodes);
+        k_renum_ids<<<1024, kBlock, 0, s>>>(nid, m, b.qv, (size_t)kSegMaxK * b.qv_nodes);
+        newid = nid;
+    }
+
     // the levels the nodes asked to cut into pieces, and their anchor counts
     {
         std::vector<int> lk(kMaxLevels, 0);
@@ -3045,10 +3089,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             int cap = 1;
             for (int d = dc; d <= D; ++d) cap = std::max(cap, (int)P.smax[d]);
             const size_t lds = cell_lds_bytes(cap);
-            k_cell<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, dc, D, cap, b.res, b.redo, b.err,
-                                          b.counters);
-            k_cell_exact<<<std::min<uint32_t>(ncell, 1024), 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, b.off, dc,
-                                                                         D, cap, b.res, b.redo, b.err, b.counters);
+            k_cell<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, newid, b.off, dc, D, cap, b.res, b.redo,
+                                          b.err, b.counters);
+            k_cell_exact<<<std::min<uint32_t>(ncell, 1024), 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, newid,
+                                                                         b.off, dc, D, cap, b.res, b.redo, b.err,
+                                                                         b.counters);
         }
     }
     else
@@ -3074,7 +3119,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         {
             const int total = (int)(cnt * (uint32_t)K);
             k_seg_anchor<<<cdiv(2 * (size_t)total + cnt, kMergeBlock), kMergeBlock, 0, s>>>(
-                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, b.kn, Dk, d, K, m, lim_e, b.arec, b.acoord, b.akey,
+                b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, newid, b.kn, Dk, d, K, m, lim_e, b.arec, b.acoord,
+                b.akey,
                 b.astate, b.res, b.nstate, b.err, b.counters);
             k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
                                                                           b.spare, spare_base, spare_cap, b.res,
