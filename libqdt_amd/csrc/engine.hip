@@ -453,32 +453,39 @@ __global__ __launch_bounds__(kBlock) void k_root_nodes(const int* off, int d1, i
     (void)d1;
 }
 
-// positional split of the array sorted in the cut's frame
-__global__ __launch_bounds__(kBlock) void k_split(const uint32_t* Bf, const uint32_t* node, const int* off, int d,
-                                                  uint32_t m, uint32_t* node_next, uint8_t* bits)
+// Each kd array carries, beside its vertex ids, their frame-1 ranks (R):
+// an id is the point's frame-0 rank, so a point's key in either frame is at
+// hand without a lookup.  Within a node both arrays are sorted by their own
+// frame's key, so the positional cut of the frame-f array ("the first
+// left_size(s) go left") is "key below the key at position o + left_size(s)"
+// (medk, per node), which decides the side of every entry of the other array
+// from that entry alone.
+__global__ __launch_bounds__(kBlock) void k_kd_init(uint32_t* B0, const uint32_t* B1, uint32_t m, uint32_t* R0,
+                                                    uint32_t* R1)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    const int* od = off + off_base(off, d);
-    if (p >= (uint32_t)od[lcount(off, d)]) return; // points of sets rooted deeper
-    const uint32_t j = node[p];
-    const int o = od[j];
-    const int s = od[j + 1] - o;
-    const uint32_t c = ((int)p - o) >= left_size(s) ? 1u : 0u;
-    node_next[p] = 2 * j + c;
-    bits[Bf[p]] = (uint8_t)c;
+    if (p >= m) return;
+    B0[p] = p;
+    R1[p] = p;
+    R0[B1[p]] = p;
 }
 
-// "goes left" of position p of the other frame's array: gathered once by
-// the scan's first pass and kept by position (F), which the second pass and
-// the partition read without the gather
-struct ZeroFlag
+// "goes left" of position p of the other frame's array (K: that array's keys
+// in the cut's frame), computed by the scan's first pass and kept by
+// position (F) for the second pass and the partition
+struct SideFlag
 {
-    const uint32_t* B;
-    const uint8_t* bits;
+    const uint32_t* K;
+    const uint32_t* node;
+    const uint32_t* medk;
+    const int* off;
+    int d;
     uint8_t* F;
     __device__ uint32_t operator()(uint32_t p) const
     {
-        const uint32_t f = bits[B[p]] ? 0u : 1u;
+        const int* od = off + off_base(off, d);
+        uint32_t f = 0;
+        if (p < (uint32_t)od[lcount(off, d)]) f = K[p] < medk[node[p]] ? 1u : 0u; // else: a set rooted deeper
         F[p] = (uint8_t)f;
         return f;
     }
@@ -491,22 +498,21 @@ struct LoadU8
 };
 
 // Stable two-way partition, within each node, of the array sorted in the
-// other frame (F[p]: position p goes left).  It also does the next level's
-// split: the node of every position one level down (node_next) and, as v
-// lands in its child, v's side of the child's cut (bits[v]; each v belongs
-// to one thread).  The next level's cut array is Bout, split by position, so
-// that side is whether v lands in the child's upper part.
-__global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const uint32_t* node, const int* off,
-                                                      int d, const uint8_t* F, uint8_t* bits, const uint32_t* Z,
-                                                      uint32_t m, uint32_t* Bout, uint32_t* node_next)
+// other frame (F[p]: position p goes left), ids and frame-1 ranks together;
+// it also gives every position its node one level down (node_next).
+__global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const uint32_t* Ro, const uint32_t* node,
+                                                      const int* off, int d, const uint8_t* F, const uint32_t* Z,
+                                                      uint32_t m, uint32_t* Bout, uint32_t* Rout,
+                                                      uint32_t* node_next)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= m) return;
     const int* od = off + off_base(off, d);
-    const uint32_t v = Bo[p];
+    const uint32_t v = Bo[p], r = Ro[p];
     if (p >= (uint32_t)od[lcount(off, d)])
     {
         Bout[p] = v; // sets rooted deeper keep their order
+        Rout[p] = r;
         return;
     }
     const uint32_t j = node[p];
@@ -514,12 +520,10 @@ __global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const 
     const int s = od[j + 1] - o;
     const int ls = left_size(s);
     const int z = (int)(Z[p] - Z[o]);
-    const int b = F[p] ? 0 : 1;
-    const int pos = b ? o + ls + ((int)p - o - z) : o + z;
+    const int pos = F[p] ? o + z : o + ls + ((int)p - o - z);
     Bout[pos] = v;
+    Rout[pos] = r;
     node_next[p] = 2 * j + (((int)p - o) >= ls ? 1u : 0u);
-    const int oc = b ? o + ls : o, sc = b ? s - ls : ls;
-    bits[v] = (uint8_t)((pos - oc) >= left_size(sc) ? 1 : 0);
 }
 
 // Vertex ids renumbered into the order of the frame-0 array at the cell
@@ -1079,15 +1083,16 @@ __host__ __device__ inline int seam_anchors_max(double s)
 }
 
 // split vertex of every internal node; anchor heights of segmented levels
-__global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const uint32_t* Bo, const int* off, int d,
-                                                      int K, const double2* PT, int* splitv, int* qv, int* kn,
-                                                      int* lvlK)
+__global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const uint32_t* Bo, const uint32_t* Kf,
+                                                      const int* off, int d, int K, const double2* PT, int* splitv,
+                                                      int* qv, int* kn, int* lvlK, uint32_t* medk)
 {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= lcount(off, d)) return;
     const int* od = off + off_base(off, d);
     const int o = od[j];
     const int s = od[j + 1] - o;
+    medk[j] = s < 4 ? 0xffffffffu : Kf[o + left_size(s)]; // a node of < 4 points passes through
     if (s < 4) return;
     const size_t ni = node_index(off, d, j);
     splitv[ni] = (int)Bf[o + left_size(s)];
@@ -2324,7 +2329,7 @@ struct Engine::Bufs
     uint32_t *partials, *hist, *ghist, *vrank, *rin, *one_set;
     double2* P0;
     double2* P1; // the other order's coordinates (vertex ids renumbered for the cells)
-    uint8_t *bits, *flagp;
+    uint8_t* flagp;
     int *NX, *OR;
     unsigned long long* counters;
     int* err;
@@ -2543,7 +2548,7 @@ Status Engine::ensure(size_t n)
         { (void**)&b.hist, 4 * 256 * (size_t)nb_sort },
         { (void**)&b.ghist, 4 * 8 * 256 },
         { (void**)&b.vrank, 4 * N },    { (void**)&b.rin, 4 * N },
-        { (void**)&b.P0, 16 * N },      { (void**)&b.P1, 16 * N },      { (void**)&b.bits, N },         { (void**)&b.flagp, N },         { (void**)&b.NX, 4 * 4 * R },
+        { (void**)&b.P0, 16 * N },      { (void**)&b.P1, 16 * N },      { (void**)&b.flagp, N },         { (void**)&b.NX, 4 * 4 * R },
         { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
         { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },
         { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },
@@ -2973,7 +2978,15 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     }
     uint32_t* B1 = vA;
     uint32_t* B0 = b.B0;
-    k_iota<<<cdiv(m, kBlock), kBlock, 0, s>>>(B0, m);
+    // frame-1 ranks carried by the two kd arrays (and the partition's
+    // output), in the sort keys' buffers
+    uint32_t* R0 = reinterpret_cast<uint32_t*>(b.kA);
+    uint32_t* R1 = reinterpret_cast<uint32_t*>(b.kB);
+    uint32_t* Rt = reinterpret_cast<uint32_t*>(b.ky);
+    // the per-node keys of each cut: in the sort's value buffer that does
+    // not hold the frame-1 order (the kd arrays rotate among B0, B1, Bt)
+    uint32_t* const medk = B1 == b.vA ? b.vB : b.vA;
+    k_kd_init<<<cdiv(m, kBlock), kBlock, 0, s>>>(B0, B1, m, R0, R1);
 
     mark(4);
     // 5. kd levels
@@ -3014,21 +3027,23 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         const uint32_t cnt = (uint32_t)P.count[d];
         uint32_t*& Bf = (d & 1) ? B1 : B0;
         uint32_t*& Bo = (d & 1) ? B0 : B1;
+        uint32_t* Rf = (d & 1) ? R1 : R0;
+        uint32_t*& Ro = (d & 1) ? R0 : R1;
         if (cnt)
         {
+            // keys in the cut's frame: ids for frame 0, frame-1 ranks for frame 1
+            const uint32_t* Kf = (d & 1) ? Rf : Bf;
+            const uint32_t* Ko = (d & 1) ? Ro : Bo;
             k_children<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(b.off, d);
-            k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, b.off, d, P.K[d], b.P0, b.splitv, b.qv,
-                                                              b.kn, b.lvlK);
+            k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, Kf, b.off, d, P.K[d], b.P0, b.splitv, b.qv,
+                                                              b.kn, b.lvlK, medk);
             if (P.K[d])
                 k_node_qv<<<cdiv((size_t)cnt * P.K[d], kBlock), kBlock, 0, s>>>(Bo, b.off, d, P.K[d], b.kn, b.qv);
-            // the sides of this level's cut: from the level above's partition,
-            // except on levels where point sets are rooted
-            if (d == 0 || P.ra[d] < P.ra[d + 1])
-                k_split<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bf, nodeA, b.off, d, m, nodeB, b.bits);
-            scan_u32(ZeroFlag{ Bo, b.bits, b.flagp }, LoadU8{ b.flagp }, m, StoreU32{ b.Z }, b.partials, s);
-            k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, nodeA, b.off, d, b.flagp, b.bits, b.Z, m, Bt,
-                                                           nodeB);
+            scan_u32(SideFlag{ Ko, nodeA, medk, b.off, d, b.flagp }, LoadU8{ b.flagp }, m, StoreU32{ b.Z },
+                     b.partials, s);
+            k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, Ro, nodeA, b.off, d, b.flagp, b.Z, m, Bt, Rt, nodeB);
             std::swap(Bo, Bt);
+            std::swap(Ro, Rt);
         }
         roots_at(d + 1, nodeB);
         std::swap(nodeA, nodeB);
