@@ -131,23 +131,34 @@ __global__ __launch_bounds__(kBlock) void k_gather_u64(const uint32_t* idx, cons
     if (i < n) dst[i] = src[idx[i]];
 }
 
-// Sorted position p starts a new distinct point.  One set: the sort's last
-// keys are x keys.  Several sets: the last keys are set ranks and x keys are
-// read by input index.
-struct DedupeFlag
+// Sorted position p starts a new distinct point when its order keys (x, y)
+// or, with several sets, its set rank differ from position p-1's (F[p]).
+// The point is gathered here once, in sorted order, for the write that
+// follows (Pg).
+__global__ __launch_bounds__(kBlock) void k_dedupe_flags(const double2* pts, const uint32_t* idx,
+                                                         const uint64_t* rank, uint32_t n, uint8_t* F, double2* Pg)
 {
-    const uint64_t* k_sorted;
-    const uint32_t* idx;
-    const uint64_t* ky;
-    const uint64_t* kx; // by input index, when k_sorted holds ranks (else null)
-    __device__ uint32_t operator()(uint32_t p) const
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t q = p < n ? p : n - 1; // every lane takes part in the shuffles
+    const double2 a = pts[idx[q]];
+    // position p-1's point: the lane below's, or loaded by lane 0
+    const int lane = (int)(threadIdx.x & 63);
+    double2 b;
+    b.x = __shfl_up(a.x, 1);
+    b.y = __shfl_up(a.y, 1);
+    if (lane == 0 && q > 0) b = pts[idx[q - 1]];
+    if (p >= n) return;
+    Pg[p] = a;
+    uint32_t f = 1;
+    if (p > 0)
     {
-        if (p == 0) return 1;
-        const uint32_t i = idx[p], h = idx[p - 1];
-        if (k_sorted[p] != k_sorted[p - 1] || ky[i] != ky[h]) return 1;
-        return (kx && kx[i] != kx[h]) ? 1 : 0;
+        f = (order_key(a.x) != order_key(b.x) || order_key(a.y) != order_key(b.y) ||
+             (rank && rank[p] != rank[p - 1]))
+                ? 1u
+                : 0u;
     }
-};
+    F[p] = (uint8_t)f;
+}
 
 struct LoadU32
 {
@@ -161,19 +172,19 @@ struct StoreU32
     __device__ void operator()(uint32_t i, uint32_t v) const { out[i] = v; }
 };
 
-__global__ __launch_bounds__(kBlock) void k_dedupe_write(const double2* pts, const uint32_t* idx, DedupeFlag f,
-                                                         const uint32_t* dist, uint32_t n, double2* P0,
-                                                         uint32_t* ID0, uint32_t* vrank, uint32_t* didx)
+__global__ __launch_bounds__(kBlock) void k_dedupe_write(const double2* Pg, const uint32_t* idx, const uint8_t* F,
+                                                         const uint64_t* rank, const uint32_t* dist, uint32_t n,
+                                                         double2* P0, uint32_t* ID0, uint32_t* vrank, uint32_t* didx)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
-    if (f(p))
+    if (F[p])
     {
         const uint32_t d = dist[p];
         const uint32_t i = idx[p];
-        P0[d] = pts[i];
+        P0[d] = Pg[p];
         ID0[d] = i;
-        if (f.kx) vrank[d] = (uint32_t)f.k_sorted[p];
+        if (rank) vrank[d] = (uint32_t)rank[p];
         if (didx) didx[i] = d;
     }
 }
@@ -357,12 +368,12 @@ __global__ __launch_bounds__(kBlock) void k_rep_local(uint32_t* rep, uint32_t n,
     if (i < n) rep[i] -= soff[set_of(soff, k, i)];
 }
 
-__global__ __launch_bounds__(kBlock) void k_rep(const uint32_t* idx, DedupeFlag f, const uint32_t* dist,
+__global__ __launch_bounds__(kBlock) void k_rep(const uint32_t* idx, const uint8_t* F, const uint32_t* dist,
                                                 uint32_t n, const uint32_t* ID0, uint32_t* rep)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
-    const uint32_t d = f(p) ? dist[p] : dist[p] - 1;
+    const uint32_t d = F[p] ? dist[p] : dist[p] - 1;
     rep[idx[p]] = ID0[d];
 }
 
@@ -2840,8 +2851,6 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     k_keys<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, n, b.kx, b.kA, b.vA, b.err);
     uint64_t *kA = b.kA, *kB = b.kB;
     uint32_t *vA = b.vA, *vB = b.vB;
-    // y keys by input index are needed again for the equality test
-    QDT_HIP(hipMemcpyAsync(b.ky, b.kA, 8 * (size_t)n, hipMemcpyDeviceToDevice, s));
     Status st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
     if (st != Status::ok) return st;
     // one set: keep the y order for frame 1 (counters[14]: a tie in y)
@@ -2868,13 +2877,15 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
 
     mark(2);
     // 3. dedupe
-    DedupeFlag df{ kA, vA, b.ky, multi ? b.kx : nullptr };
-    scan_u32(df, n, StoreU32{ b.dist }, b.partials, s);
+    // (several sets: the last sort's keys are the set ranks)
+    const uint64_t* srank = multi ? kA : nullptr;
+    k_dedupe_flags<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, vA, srank, n, b.flagp, b.P1);
+    scan_u32(LoadU8{ b.flagp }, n, StoreU32{ b.dist }, b.partials, s);
     const uint32_t nbs = cdiv(n, kScanTile);
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbs, 4, hipMemcpyDeviceToHost, s));
-    k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, vA, df, b.dist, n, b.P0, b.ID0, b.vrank,
+    k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.P1, vA, b.flagp, srank, b.dist, n, b.P0, b.ID0, b.vrank,
                                                       multi ? nullptr : b.Z);
-    k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, df, b.dist, n, b.ID0, b.rep);
+    k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.flagp, b.dist, n, b.ID0, b.rep);
     std::vector<uint32_t> vstart_h;
     if (multi)
     {
