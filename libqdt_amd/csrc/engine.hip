@@ -286,6 +286,17 @@ __global__ __launch_bounds__(kBlock) void k_gather_rank(const uint32_t* idx, con
     if (i < n) key[i] = rank[idx[i]];
 }
 
+// (set rank, input index) of every position of an order of input indices
+__global__ __launch_bounds__(kBlock) void k_rank_pairs(const uint32_t* idx, const uint32_t* rank, uint32_t n,
+                                                       uint64_t* key, uint32_t* val)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = idx[i];
+    key[i] = rank[v];
+    val[i] = v;
+}
+
 // first distinct point of every rank; rpos[r] = first sorted position of rank r
 __global__ __launch_bounds__(kBlock) void k_vstart(const uint32_t* dist, const uint32_t* rpos, int nr, uint32_t n,
                                                    const uint32_t* m_total, uint32_t* vstart)
@@ -2853,8 +2864,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     uint32_t *vA = b.vA, *vB = b.vB;
     Status st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
     if (st != Status::ok) return st;
-    // one set: keep the y order for frame 1 (counters[14]: a tie in y)
-    if (!multi) k_ytie<<<cdiv(n, kBlock), kBlock, 0, s>>>(kA, vA, b.kx, n, b.Bt, &b.counters[14]);
+    // keep the y order for frame 1 (counters[14]: a tie in y)
+    k_ytie<<<cdiv(n, kBlock), kBlock, 0, s>>>(kA, vA, b.kx, n, b.Bt, &b.counters[14]);
     if (!multi)
     {
         // non-finite input is reported before any further work
@@ -2884,7 +2895,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     const uint32_t nbs = cdiv(n, kScanTile);
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbs, 4, hipMemcpyDeviceToHost, s));
     k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.P1, vA, b.flagp, srank, b.dist, n, b.P0, b.ID0, b.vrank,
-                                                      multi ? nullptr : b.Z);
+                                                      b.Z);
     k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.flagp, b.dist, n, b.ID0, b.rep);
     std::vector<uint32_t> vstart_h;
     if (multi)
@@ -2893,10 +2904,10 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         QDT_HIP(hipMemcpyAsync(b.host_sets, b.vstart, 4 * ((size_t)k + 1), hipMemcpyDeviceToHost, s));
     }
     unsigned long long ytie = 1;
-    if (!multi) QDT_HIP(hipMemcpyAsync(hs + 4, &b.counters[14], 8, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipMemcpyAsync(hs + 4, &b.counters[14], 8, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
     const uint32_t m = hs[0];
-    if (!multi) memcpy(&ytie, hs + 4, 8);
+    memcpy(&ytie, hs + 4, 8);
     if (multi) vstart_h.assign(b.host_sets, b.host_sets + k + 1);
     res_.n = n;
     res_.rep = b.rep;
@@ -2960,20 +2971,34 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     const int lim_e = (int)(4 * b.rec_cap);
     const int spare_base = (int)(3 * (size_t)m);
     const int spare_cap = (int)(b.rec_cap - 3 * (size_t)m);
-    // 4. frame-1 order: (y asc, x desc).  One set with no two distinct
-    // points at the same y: the representatives of the first sort's y order.
-    // Otherwise a stable y-sort of the reversed frame-0 order.
+    // 4. frame-1 order: (y asc, x desc).  No two distinct points at the
+    // same y: the representatives of the first sort's y order (several sets:
+    // that order grouped by set, stably).  Otherwise a stable y-sort of the
+    // reversed frame-0 order (then by set).
     kA = b.kA;
     kB = b.kB;
     vA = b.vA;
     vB = b.vB;
-    if (!multi && !ytie)
+    if (!ytie)
     {
-        // kB and vB are free here (they serve the sorts of the other branch)
+        // kA, kB, vA and vB are free here (they serve the sorts of the other branch)
+        const uint32_t* perm = b.Bt;
         uint32_t* w = b.vB;
         uint32_t* g = reinterpret_cast<uint32_t*>(b.kB);
+        uint32_t* out = b.vA;
+        if (multi)
+        {
+            k_rank_pairs<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.Bt, b.rin, n, kA, vA);
+            st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
+            if (st != Status::ok) return st;
+            perm = vA;
+            w = reinterpret_cast<uint32_t*>(kA);
+            g = reinterpret_cast<uint32_t*>(kB);
+            out = vB;
+        }
         k_rep_id<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.rep, b.Z, (uint32_t)n, w);
-        scan_u32(RepFlag{ b.Bt, w, g }, RepLoad{ g }, n, RepWrite{ g, b.vA }, b.partials, s);
+        scan_u32(RepFlag{ perm, w, g }, RepLoad{ g }, n, RepWrite{ g, out }, b.partials, s);
+        vA = out;
     }
     else
     {

@@ -120,3 +120,19 @@ def test_uneven_batch_alone_and_in_the_mix(gpu):
     allt2 = b2.all_triangles()
     for i in range(len(sets)):
         assert np.array_equal(O.canonical(b2.triangles(i, allt2)), O.canonical(b.triangles(i, allt))), i
+
+
+def test_copies_without_y_ties(gpu):
+    """Sets of random points where some points are repeated (exact copies,
+    the same point in two sets) but no two distinct points share a y value:
+    the frame-1 order then comes from the y sort grouped by set, and the
+    duplicate rule (lowest input index represents) must still hold."""
+    rng = np.random.default_rng(11)
+    base = rng.random((3000, 2))
+    sets = []
+    for k in range(12):
+        n = int(rng.integers(4, 400))
+        p = base[rng.choice(len(base), size=n, replace=True)]  # copies inside a set
+        sets.append(p)
+    sets.append(sets[0].copy())  # the same points in another set
+    check_sets("copies, no y ties", sets)
