@@ -1082,6 +1082,14 @@ constexpr int kSegCap = kSegMaxK - 2;  // anchors per node at most: a phase's pi
 constexpr int kSegSlots = kSegMaxK + 1; // pieces per node at most
 constexpr int kSegThreads = 256;       // threads of the per-node compaction and joining kernels
 constexpr int kSegMin = 1024;          // smaller nodes merge in one piece
+// A level merges one node per lane (k_merge) instead of in seam pieces when
+// its longest seam asks for fewer than kSegLevelAnchors anchors (the pieces'
+// per-level kernels cost more than the chains they shorten), or when it has
+// a merge for every lane of k_merge's layout on the GPU (256 CUs x 4 SIMDs x
+// 16 lanes) and no node of kSegLevelBig points or more.
+constexpr int kSegLevelAnchors = 8;
+constexpr int kSegLevelNodes = 16384;
+constexpr int kSegLevelBig = 4096;
 constexpr int kPhases = 2;             // pieces k, k+2, ... run together
 constexpr int kSeamErr = 64;           // err bit: redo the run with unsplit merges
 
@@ -3116,6 +3124,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         for (int d = 0; d < D; ++d)
         {
             P.K[d] = d < Dk ? std::min(P.K[d], lk[d]) : 0;
+            if (P.K[d] < kSegLevelAnchors || (P.count[d] >= kSegLevelNodes && P.smax[d] < kSegLevelBig)) P.K[d] = 0;
             if (!P.K[d]) continue;
             seg_nodes = std::max(seg_nodes, (size_t)P.count[d]);
             anchor_cap = std::max(anchor_cap, (size_t)P.count[d] * P.K[d]);
