@@ -493,24 +493,16 @@ __global__ __launch_bounds__(kBlock) void k_kd_init(uint32_t* B0, const uint32_t
 }
 
 // "goes left" of position p of the other frame's array (K: that array's keys
-// in the cut's frame), computed by the scan's first pass and kept by
-// position (F) for the second pass and the partition
+// in the cut's frame; positions from `end` on belong to sets rooted deeper).
+// It only loads, so the scan's passes issue the loads of all their items at
+// once; both passes and the partition evaluate it.
 struct SideFlag
 {
     const uint32_t* K;
     const uint32_t* node;
     const uint32_t* medk;
-    const int* off;
-    int d;
-    uint8_t* F;
-    __device__ uint32_t operator()(uint32_t p) const
-    {
-        const int* od = off + off_base(off, d);
-        uint32_t f = 0;
-        if (p < (uint32_t)od[lcount(off, d)]) f = K[p] < medk[node[p]] ? 1u : 0u; // else: a set rooted deeper
-        F[p] = (uint8_t)f;
-        return f;
-    }
+    uint32_t end;
+    __device__ uint32_t operator()(uint32_t p) const { return p < end && K[p] < medk[node[p]] ? 1u : 0u; }
 };
 
 struct LoadU8
@@ -520,10 +512,12 @@ struct LoadU8
 };
 
 // Stable two-way partition, within each node, of the array sorted in the
-// other frame (F[p]: position p goes left), ids and frame-1 ranks together;
-// it also gives every position its node one level down (node_next).
+// other frame (position p goes left when its key, the id on levels cut in
+// frame 0 and the frame-1 rank on the others, is below its node's medk), ids
+// and frame-1 ranks together; it also gives every position its node one
+// level down (node_next).  Z: exclusive prefix counts of "goes left".
 __global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const uint32_t* Ro, const uint32_t* node,
-                                                      const int* off, int d, const uint8_t* F, const uint32_t* Z,
+                                                      const int* off, int d, const uint32_t* medk, const uint32_t* Z,
                                                       uint32_t m, uint32_t* Bout, uint32_t* Rout,
                                                       uint32_t* node_next)
 {
@@ -542,7 +536,8 @@ __global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const 
     const int s = od[j + 1] - o;
     const int ls = left_size(s);
     const int z = (int)(Z[p] - Z[o]);
-    const int pos = F[p] ? o + z : o + ls + ((int)p - o - z);
+    const bool left = ((d & 1) ? r : v) < medk[j];
+    const int pos = left ? o + z : o + ls + ((int)p - o - z);
     Bout[pos] = v;
     Rout[pos] = r;
     node_next[p] = 2 * j + (((int)p - o) >= ls ? 1u : 0u);
@@ -3083,9 +3078,10 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                                                               b.kn, b.lvlK, medk);
             if (P.K[d])
                 k_node_qv<<<cdiv((size_t)cnt * P.K[d], kBlock), kBlock, 0, s>>>(Bo, b.off, d, P.K[d], b.kn, b.qv);
-            scan_u32(SideFlag{ Ko, nodeA, medk, b.off, d, b.flagp }, LoadU8{ b.flagp }, m, StoreU32{ b.Z },
-                     b.partials, s);
-            k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, Ro, nodeA, b.off, d, b.flagp, b.Z, m, Bt, Rt, nodeB);
+            // positions of the sets rooted at this level or above
+            const uint32_t end = multi ? vstart_h[P.ra[d + 1]] : m;
+            scan_u32(SideFlag{ Ko, nodeA, medk, end }, m, StoreU32{ b.Z }, b.partials, s);
+            k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, Ro, nodeA, b.off, d, medk, b.Z, m, Bt, Rt, nodeB);
             std::swap(Bo, Bt);
             std::swap(Ro, Rt);
         }
