@@ -213,20 +213,13 @@ __global__ __launch_bounds__(kBlock) void k_rep_id(const uint32_t* rep, const ui
     if (i < n) w[i] = rep[i] == i ? didx[i] : ~0u;
 }
 
-// the scan's first pass gathers w in y order once and keeps it by position
-// (g), which the second pass reads back without the gather
-struct RepFlag
+// w in y order, gathered once by position (g) for both passes of the scan
+__global__ __launch_bounds__(kBlock) void k_rep_gather(const uint32_t* perm, const uint32_t* w, uint32_t n,
+                                                       uint32_t* g)
 {
-    const uint32_t* perm;
-    const uint32_t* w;
-    uint32_t* g;
-    __device__ uint32_t operator()(uint32_t p) const
-    {
-        const uint32_t v = w[perm[p]];
-        g[p] = v;
-        return v != ~0u ? 1u : 0u;
-    }
-};
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < n) g[p] = w[perm[p]];
+}
 
 struct RepLoad
 {
@@ -238,9 +231,9 @@ struct RepWrite
 {
     const uint32_t* g;
     uint32_t* out;
-    __device__ void operator()(uint32_t p, uint32_t pre) const
+    __device__ uint32_t load(uint32_t p) const { return g[p]; }
+    __device__ void store(uint32_t, uint32_t pre, uint32_t v) const
     {
-        const uint32_t v = g[p];
         if (v != ~0u) out[pre] = v;
     }
 };
@@ -2110,7 +2103,8 @@ struct FaceCount
 struct FacePrefix
 {
     uint32_t* face;
-    __device__ void operator()(uint32_t k, uint32_t pre) const { face[k] = (pre << 2) | (face[k] & 3u); }
+    __device__ uint32_t load(uint32_t k) const { return face[k]; }
+    __device__ void store(uint32_t k, uint32_t pre, uint32_t f) const { face[k] = (pre << 2) | (f & 3u); }
 };
 
 // one thread per record: writes its faces at their prefix (as original,
@@ -3000,7 +2994,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             out = vB;
         }
         k_rep_id<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.rep, b.Z, (uint32_t)n, w);
-        scan_u32(RepFlag{ perm, w, g }, RepLoad{ g }, n, RepWrite{ g, out }, b.partials, s);
+        k_rep_gather<<<cdiv(n, kBlock), kBlock, 0, s>>>(perm, w, n, g);
+        scan_u32(RepLoad{ g }, n, RepWrite{ g, out }, b.partials, s);
         vA = out;
     }
     else
@@ -3445,10 +3440,11 @@ struct FirstWrite
     FirstFlag f;
     int2* vert;
     int* o2l;
-    __device__ void operator()(uint32_t h, uint32_t pre) const
+    // h's origin when h is its first appearance, else ~0u
+    __device__ uint32_t load(uint32_t h) const { return f(h) ? f.tris[h] : ~0u; }
+    __device__ void store(uint32_t h, uint32_t pre, uint32_t u) const
     {
-        if (!f(h)) return;
-        const uint32_t u = f.tris[h];
+        if (u == ~0u) return;
         vert[pre] = make_int2((int)u, (int)h);
         o2l[u] = (int)pre;
     }

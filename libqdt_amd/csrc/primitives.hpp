@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
+#include <utility>
 
 namespace qdt
 {
@@ -114,6 +116,18 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* partials, uint
 // (thread t reads 16 consecutive items) free of bank conflicts
 __device__ __forceinline__ uint32_t scan_slot(uint32_t p) { return p + (p >> 5); }
 
+// An output functor that splits into load(i) (loads only) and
+// store(i, prefix, loaded) lets the scan issue all of a thread's loads before
+// its stores.
+template <class T, class = void>
+struct SplitOut : std::false_type
+{
+};
+template <class T>
+struct SplitOut<T, std::void_t<decltype(std::declval<const T&>().load(0u))>> : std::true_type
+{
+};
+
 template <class In, class Out>
 __global__ __launch_bounds__(kBlock) void k_scan_down(In in, uint32_t n, const uint32_t* partials, Out out)
 {
@@ -148,12 +162,34 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(In in, uint32_t n, const u
         pre += v[k];
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
+    if constexpr (SplitOut<Out>::value)
     {
-        const uint32_t p = k * kBlock + threadIdx.x;
-        const uint32_t i = base + p;
-        if (i < n) out(i, tile[scan_slot(p)]);
+        // every item's loads first, then every item's stores: a store that
+        // may alias a later item's load would otherwise hold that load back
+        decltype(out.load(0u)) st[kScanItems];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            const uint32_t i = base + k * kBlock + threadIdx.x;
+            if (i < n) st[k] = out.load(i);
+        }
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            const uint32_t p = k * kBlock + threadIdx.x;
+            const uint32_t i = base + p;
+            if (i < n) out.store(i, tile[scan_slot(p)], st[k]);
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            const uint32_t p = k * kBlock + threadIdx.x;
+            const uint32_t i = base + p;
+            if (i < n) out(i, tile[scan_slot(p)]);
+        }
     }
 }
 
