@@ -2028,6 +2028,47 @@ __device__ __forceinline__ bool face_of(const int* NX, const int* OR, int e, int
     return (unsigned)a < (unsigned)lim_v && (unsigned)b < (unsigned)lim_v && (unsigned)c < (unsigned)lim_v;
 }
 
+// The left faces of edges 4k and 4k+2 of record k (its two primal edges),
+// walked side by side so that each step's loads for both faces are in flight
+// together: ok[i] when the lnext cycle of edge 4k+2i closes after three
+// edges within bounds (and, with `smallest`, when no edge of it is below the
+// starting one, so each face is seen from one edge only); v[i] its origins.
+__device__ __forceinline__ void record_faces(const int* NX, const int* OR, uint32_t k, int lim_e, int lim_v,
+                                             bool smallest, uint32_t want, bool ok[2], int v[2][3])
+{
+    const int4 R = reinterpret_cast<const int4*>(NX)[k]; // onext of 4k .. 4k+3
+    const int e[2] = { 4 * (int)k, 4 * (int)k + 2 };
+    int f[2], g[2];
+    f[0] = e_rot(R.w); // lnext(4k) = rot(onext(invrot(4k))) = rot(onext(4k+3))
+    f[1] = e_rot(R.y);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        ok[i] = (want >> i & 1u) && (unsigned)f[i] < (unsigned)lim_e && (!smallest || f[i] >= e[i]);
+    int t[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) t[i] = ok[i] ? NX[e_invrot(f[i])] : 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+    {
+        g[i] = e_rot(t[i]);
+        ok[i] = ok[i] && (unsigned)g[i] < (unsigned)lim_e && (!smallest || g[i] >= e[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) t[i] = ok[i] ? NX[e_invrot(g[i])] : 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+    {
+        ok[i] = ok[i] && e_rot(t[i]) == e[i];
+        v[i][0] = ok[i] ? OR[e[i] >> 1] : 0;
+        v[i][1] = ok[i] ? OR[f[i] >> 1] : 0;
+        v[i][2] = ok[i] ? OR[g[i] >> 1] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        ok[i] = ok[i] && (unsigned)v[i][0] < (unsigned)lim_v && (unsigned)v[i][1] < (unsigned)lim_v &&
+                (unsigned)v[i][2] < (unsigned)lim_v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_extract_test(const int* NX, const int* OR, const double2* PT,
                                                          uint32_t nrec, uint32_t m, int lim_e, const unsigned* spare,
                                                          uint32_t spare_cap, uint32_t* face, uint32_t* queue,
@@ -2039,20 +2080,26 @@ __global__ __launch_bounds__(kBlock) void k_extract_test(const int* NX, const in
     uint32_t fl = 0;
     if (k < used && OR[2 * k] >= 0)
     {
+        // the face is reported by its smallest edge: the walk stops at the
+        // first smaller edge of the cycle before reading further
+        bool ok[2];
+        int v[2][3];
+        record_faces(NX, OR, k, lim_e, (int)m, true, 3u, ok, v);
+        double2 PA[2], PB[2], PC[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            if (ok[i])
+            {
+                PA[i] = PT[v[i][0]];
+                PB[i] = PT[v[i][1]];
+                PC[i] = PT[v[i][2]];
+            }
 #pragma unroll
         for (int r = 0; r < 4; r += 2)
         {
+            if (!ok[r >> 1]) continue;
             const int e = 4 * (int)k + r;
-            // the face is reported by its smallest edge: stop at the first
-            // smaller edge of the cycle before reading further
-            const int f = e_rot(NX[e_invrot(e)]);
-            if (f < e || f >= lim_e) continue;
-            const int g = e_rot(NX[e_invrot(f)]);
-            if (g < e || g >= lim_e) continue;
-            if (e_rot(NX[e_invrot(g)]) != e) continue;
-            const int a = OR[e >> 1], b = OR[f >> 1], c = OR[g >> 1];
-            if ((unsigned)a >= m || (unsigned)b >= m || (unsigned)c >= m) continue;
-            const double2 A = PT[a], B = PT[b], C = PT[c];
+            const double2 A = PA[r >> 1], B = PB[r >> 1], C = PC[r >> 1];
             const double detleft = (A.x - C.x) * (B.y - C.y);
             const double detright = (A.y - C.y) * (B.x - C.x);
             const double det = detleft - detright;
@@ -2120,15 +2167,22 @@ __global__ __launch_bounds__(kBlock) void k_face_write(const uint32_t* face, con
     if (k == split) *tmain = pre; // triangles of the records below the spare region
     const uint32_t fl = f & 3u;
     if (!fl) return;
-    for (int r = 0; r < 4; r += 2)
+    // both faces' loads before any store
+    bool ok[2];
+    int v[2][3];
+    record_faces(NX, OR, k, lim_e, (int)m, false, fl, ok, v);
+    uint32_t id[2][3];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) id[i][j] = (fl >> i & 1u) ? ID0[ok[i] ? v[i][j] : 0] : 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
     {
-        if (!(fl & (1u << (r >> 1)))) continue;
-        const int e = 4 * (int)k + r;
-        int g, h, a, b, c;
-        if (!face_of(NX, OR, e, lim_e, (int)m, g, h, a, b, c)) a = b = c = 0;
-        tris[3 * pre] = ID0[a];
-        tris[3 * pre + 1] = ID0[b];
-        tris[3 * pre + 2] = ID0[c];
+        if (!(fl >> i & 1u)) continue;
+        tris[3 * pre] = id[i][0];
+        tris[3 * pre + 1] = id[i][1];
+        tris[3 * pre + 2] = id[i][2];
         ++pre;
     }
 }
