@@ -189,17 +189,45 @@ __global__ __launch_bounds__(kBlock) void k_dedupe_write(const double2* Pg, cons
     }
 }
 
-// After the sort by y (ties by input index): keeps that order, and flags a
-// y value held by two distinct points (the frame-1 order then needs the
-// x-descending tie rule, which this order does not give).
-__global__ __launch_bounds__(kBlock) void k_ytie(const uint64_t* ysorted, const uint32_t* idx, const uint64_t* kx,
-                                                 uint32_t n, uint32_t* perm, unsigned long long* tie)
+// After the sort by y (ties by input index): keeps that order for frame 1.
+__global__ __launch_bounds__(kBlock) void k_ytie(const uint32_t* idx, uint32_t n, uint32_t* perm)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t i = idx[p];
-    perm[p] = i;
-    if (p + 1 < n && ysorted[p] == ysorted[p + 1] && kx[i] != kx[idx[p + 1]]) *tie = 1;
+    if (p < n) perm[p] = idx[p];
+}
+
+// Frame 1 orders points of equal y by x descending, which the y sort's
+// index order does not give: the first thread of every run of equal y keys
+// re-sorts its run in perm by x descending (insertion sort, stable, so
+// copies keep their index order).  A run longer than kYRunMax is left alone
+// and flagged (tie): frame 1 then comes from a full sort.
+constexpr uint32_t kYRunMax = 64;
+__global__ __launch_bounds__(kBlock) void k_yfix(const uint64_t* ysorted, const uint64_t* kx, uint32_t n,
+                                                 uint32_t* perm, unsigned long long* tie)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p + 1 >= n) return;
+    const uint64_t y = ysorted[p];
+    if (ysorted[p + 1] != y || (p > 0 && ysorted[p - 1] == y)) return; // not the first of a run
+    uint32_t q = p + 2;
+    while (q < n && q - p <= kYRunMax && ysorted[q] == y) ++q;
+    if (q - p > kYRunMax)
+    {
+        *tie = 1;
+        return;
+    }
+    for (uint32_t a = p + 1; a < q; ++a)
+    {
+        const uint32_t v = perm[a];
+        const uint64_t xv = kx[v];
+        uint32_t b = a;
+        while (b > p && kx[perm[b - 1]] < xv)
+        {
+            perm[b] = perm[b - 1];
+            --b;
+        }
+        perm[b] = v;
+    }
 }
 
 // frame-1 order without a sort: the y order's representatives (a point's
@@ -2915,8 +2943,10 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     uint32_t *vA = b.vA, *vB = b.vB;
     Status st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
     if (st != Status::ok) return st;
-    // keep the y order for frame 1 (counters[14]: a tie in y)
-    k_ytie<<<cdiv(n, kBlock), kBlock, 0, s>>>(kA, vA, b.kx, n, b.Bt, &b.counters[14]);
+    // keep the y order for frame 1, runs of equal y put in x-descending
+    // order (counters[14]: a run too long for that)
+    k_ytie<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, n, b.Bt);
+    k_yfix<<<cdiv(n, kBlock), kBlock, 0, s>>>(kA, b.kx, n, b.Bt, &b.counters[14]);
     if (!multi)
     {
         // non-finite input is reported before any further work
@@ -3022,10 +3052,10 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     const int lim_e = (int)(4 * b.rec_cap);
     const int spare_base = (int)(3 * (size_t)m);
     const int spare_cap = (int)(b.rec_cap - 3 * (size_t)m);
-    // 4. frame-1 order: (y asc, x desc).  No two distinct points at the
-    // same y: the representatives of the first sort's y order (several sets:
-    // that order grouped by set, stably).  Otherwise a stable y-sort of the
-    // reversed frame-0 order (then by set).
+    // 4. frame-1 order: (y asc, x desc).  No run of equal y longer than
+    // kYRunMax: the representatives of the first sort's y order with its runs
+    // re-sorted (several sets: that order grouped by set, stably).
+    // Otherwise a stable y-sort of the reversed frame-0 order (then by set).
     kA = b.kA;
     kB = b.kB;
     vA = b.vA;

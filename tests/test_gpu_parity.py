@@ -72,6 +72,8 @@ def test_structured(gpu):
         "circle 2^27": [(round((1 << 27) * math.cos(2 * math.pi * i / 200)),
                          round((1 << 27) * math.sin(2 * math.pi * i / 200))) for i in range(200)],
         "two rows": [(i, r) for i in range(100) for r in (0, 1)],
+        "rows of 64 and 65": [(i, r) for r in range(3) for i in range(64 + (r & 1))],
+        "rows with copies": [(i % 7, r) for r in range(5) for i in range(20)],
         "triangle dups": [(0, 0), (1, 0), (0, 1)] * 20,
         "three points": [(0, 0), (5, 0), (0, 5)],
         "2^28": [(-(1 << 28), -(1 << 28)), (1 << 28, -(1 << 28)), (0, 1 << 28), (1, 1), (-1, 2)],
