@@ -167,3 +167,15 @@ def test_c3_degenerate_matches_reference_live(gpu):
     assert len(got) == len(want)
     assert O.canon_hash(got) == O.canon_hash(want)
     assert st["incircle_exact"] > 0 and st["orient_exact"] > 0
+
+
+def test_shared_y_values(gpu):
+    """Points whose y values repeat (random x): runs of equal y up to the
+    length that is re-sorted in place (x descending) and runs longer than
+    that, which take the full frame-1 sort; with and without exact copies."""
+    rng = np.random.default_rng(21)
+    for rows, n in ((400, 6000), (40, 6000), (3, 500)):
+        p = np.stack([rng.random(n), rng.integers(0, rows, n) / rows], axis=1)
+        check_same("y from %d values, %d points" % (rows, n), p)
+        q = np.concatenate([p, p[rng.choice(n, n // 10)]])
+        check_same("y from %d values, %d points, copies" % (rows, n), q[rng.permutation(len(q))])
