@@ -18,9 +18,9 @@ namespace qdt
 
 constexpr int kBlock = 256;          // threads per block for streaming kernels
 constexpr int kWaves = kBlock / 64;  // waves per block
-constexpr int kScanItems = 16;       // items per thread in scans
+constexpr int kScanItems = 8;        // items per thread in scans
 constexpr int kScanTile = kBlock * kScanItems;
-constexpr int kSortItems = 16;       // keys per lane in the radix sort
+constexpr int kSortItems = 8;        // keys per lane in the radix sort
 constexpr int kSortTile = kBlock * kSortItems;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
