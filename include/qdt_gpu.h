@@ -159,6 +159,15 @@ typedef struct qdt_gpu_mesh_info
 QDT_API qdt_status qdt_gpu_build_mesh(const uint32_t* d_tris, size_t T, size_t n, int32_t* d_he, int32_t* d_vert,
                                       int32_t* d_orig_to_local, qdt_gpu_mesh_info* out);
 
+/* For tests of the fallback paths (no counterpart in the reference).
+ * device_bytes: the most device memory the context's workspace may hold; a
+ * batch call that needs more fails with QDT_ERR_OUT_OF_MEMORY and empties
+ * the context, as a failed allocation does (0 = no limit).  spare_records:
+ * the most quad-edge records the seam pieces may take beyond each node's own
+ * range (0 = the default region); a run that needs more is repeated without
+ * seam pieces (qdt_gpu_stats.seam_retries), with the same result. */
+QDT_API void qdt_gpu_set_test_limits(qdt_t* ptr, size_t device_bytes, size_t spare_records);
+
 /* Number of HIP devices visible to the library (0 without a GPU). */
 QDT_API int qdt_gpu_device_count(void);
 
@@ -248,6 +257,12 @@ QDT_API size_t qdt_gpu_server_unique_points(qdt_gpu_server* s, uint64_t id);
 QDT_API size_t qdt_gpu_server_copy_triangles(qdt_gpu_server* s, uint64_t id, uint32_t* out);
 /* Forgets a done request. */
 QDT_API void qdt_gpu_server_release(qdt_gpu_server* s, uint64_t id);
+/* Device bytes the workers hold now (each worker's engine workspace and
+ * input buffer, kept between batches; a running batch is charged the larger
+ * of that and its estimate) and the largest such sum so far, counting both
+ * the estimates and the memory really held.  Admission keeps the charges
+ * within hbm_budget_bytes, releasing idle workers' memory when needed. */
+QDT_API int qdt_gpu_server_memory(qdt_gpu_server* s, size_t* held, size_t* peak);
 /* Batches run so far and the requests they held. */
 QDT_API int qdt_gpu_server_counters(qdt_gpu_server* s, uint64_t* batches, uint64_t* requests);
 

@@ -37,7 +37,8 @@ EXPORTS = [
 ]
 GPU_EXPORTS = [
     "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
-    "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_set_cells", "qdt_gpu_device_count",
+    "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_set_cells", "qdt_gpu_set_test_limits",
+    "qdt_gpu_device_count",
     "qdt_gpu_release_workspace", "qdt_gpu_verify", "qdt_gpu_verify_triangles", "qdt_gpu_build_mesh",
     "qdt_gpu_batch_create", "qdt_gpu_batch_destroy", "qdt_gpu_batch_execute", "qdt_gpu_batch_execute_device",
     "qdt_gpu_batch_sets", "qdt_gpu_batch_set_status", "qdt_gpu_batch_triangles_size",
@@ -47,7 +48,7 @@ GPU_EXPORTS = [
     "qdt_gpu_server_create", "qdt_gpu_server_destroy", "qdt_gpu_server_submit", "qdt_gpu_server_poll",
     "qdt_gpu_server_wait", "qdt_gpu_server_times", "qdt_gpu_server_triangles_size",
     "qdt_gpu_server_unique_points", "qdt_gpu_server_copy_triangles", "qdt_gpu_server_release",
-    "qdt_gpu_server_counters",
+    "qdt_gpu_server_counters", "qdt_gpu_server_memory",
 ]
 
 
@@ -162,6 +163,8 @@ def lib():
         L.qdt_gpu_set_seams.argtypes = [vp, ctypes.c_int]
         L.qdt_gpu_set_cells.restype = None
         L.qdt_gpu_set_cells.argtypes = [vp, ctypes.c_int]
+        L.qdt_gpu_set_test_limits.restype = None
+        L.qdt_gpu_set_test_limits.argtypes = [vp, sz, sz]
         L.qdt_gpu_verify.restype = st
         L.qdt_gpu_verify.argtypes = [vp, ctypes.POINTER(VerifyReport)]
         L.qdt_gpu_verify_triangles.restype = st
@@ -224,6 +227,8 @@ def lib():
         L.qdt_gpu_server_release.argtypes = [vp, u64]
         L.qdt_gpu_server_counters.restype = ctypes.c_int
         L.qdt_gpu_server_counters.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.qdt_gpu_server_memory.restype = ctypes.c_int
+        L.qdt_gpu_server_memory.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         _lib = L
     return _lib
 
@@ -368,6 +373,9 @@ class Triangulation:
 
     def set_cells(self, on=True):
         lib().qdt_gpu_set_cells(self._h, 1 if on else 0)
+
+    def set_test_limits(self, device_bytes=0, spare_records=0):
+        lib().qdt_gpu_set_test_limits(self._h, device_bytes, spare_records)
 
     def release_workspace(self):
         lib().qdt_gpu_release_workspace(self._h)
@@ -542,6 +550,12 @@ class Server:
         b, r = ctypes.c_uint64(), ctypes.c_uint64()
         lib().qdt_gpu_server_counters(self._h, ctypes.byref(b), ctypes.byref(r))
         return b.value, r.value
+
+    def memory(self):
+        """(device bytes the workers hold now, largest sum so far)."""
+        h, p = ctypes.c_size_t(), ctypes.c_size_t()
+        lib().qdt_gpu_server_memory(self._h, ctypes.byref(h), ctypes.byref(p))
+        return h.value, p.value
 
 
 def verify_triangles(d_pts, n, d_tris, T, d_rep=None):

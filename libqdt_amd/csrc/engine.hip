@@ -2139,12 +2139,16 @@ __global__ __launch_bounds__(kBlock) void k_extract_test(const int* NX, const in
             }
             else
             {
-                fl |= 4u << (r >> 1);
+                // queued for k_extract_exact; past the queue's end the face
+                // is marked for k_extract_overflow instead
                 const unsigned long long slot = atomicAdd(&counters[4], 1ull);
                 if (slot < qcap)
+                {
+                    fl |= 4u << (r >> 1);
                     queue[slot] = (uint32_t)e;
+                }
                 else
-                    atomicOr(err, 4);
+                    fl |= 16u << (r >> 1);
             }
         }
     }
@@ -2164,6 +2168,35 @@ __global__ __launch_bounds__(kBlock) void k_extract_exact(const int* NX, const i
         if (!face_of(NX, OR, e, lim_e, (int)m, f, g, a, b, c)) continue;
         const double2 A = PT[a], B = PT[b], C = PT[c];
         if (orient2d_exact(A.x, A.y, B.x, B.y, C.x, C.y) > 0) atomicOr(&face[e >> 2], 1u << ((e & 2) >> 1));
+    }
+}
+
+// Faces the queue had no room for (inputs whose every face escapes the
+// filter, e.g. all coordinates below ~1e-126 or above ~1e154, where the
+// filter's products fall under its floor or overflow): decided exactly here,
+// by a grid-stride pass over the records that ends at once when the queue
+// did not overflow.
+__global__ __launch_bounds__(kBlock) void k_extract_overflow(const int* NX, const int* OR, const double2* PT,
+                                                             uint32_t nrec, uint32_t m, int lim_e,
+                                                             const unsigned long long* counters, uint32_t qcap,
+                                                             uint32_t* face)
+{
+    if (counters[4] <= qcap) return;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < nrec; k += gridDim.x * kBlock)
+    {
+        const uint32_t fl = face[k];
+        if (!(fl & 48u)) continue;
+        uint32_t add = 0;
+        for (int i = 0; i < 2; ++i)
+        {
+            if (!(fl >> (4 + i) & 1u)) continue;
+            const int e = 4 * (int)k + 2 * i;
+            int f, g, a, b, c;
+            if (!face_of(NX, OR, e, lim_e, (int)m, f, g, a, b, c)) continue;
+            const double2 A = PT[a], B = PT[b], C = PT[c];
+            if (orient2d_exact(A.x, A.y, B.x, B.y, C.x, C.y) > 0) add |= 1u << i;
+        }
+        face[k] = (fl & ~48u) | add;
     }
 }
 
@@ -2532,10 +2565,11 @@ struct Item
 constexpr size_t kHostSetArrays = 8;
 
 template <size_t N>
-Status carve(const Item (&items)[N], char*& region, size_t& bytes)
+Status carve(const Item (&items)[N], char*& region, size_t& bytes, size_t budget)
 {
     size_t total = 0;
     for (const Item& it : items) total += (it.bytes + 255) & ~(size_t)255;
+    if (total > budget) return Status::out_of_memory;
     void* ws = nullptr;
     const hipError_t e = hipMalloc(&ws, total);
     if (e != hipSuccess) return e == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal;
@@ -2606,6 +2640,15 @@ void Engine::release()
 
 size_t Engine::workspace_bytes() const { return ws_bytes_ + ws_nodes_bytes_ + ws_anchors_bytes_ + ws_sets_bytes_; }
 
+// bytes a new region may take under the test limit (regions being replaced
+// are freed before they are carved again)
+size_t Engine::alloc_budget() const
+{
+    if (!mem_cap_) return SIZE_MAX;
+    const size_t held = workspace_bytes();
+    return mem_cap_ > held ? mem_cap_ - held : 0;
+}
+
 Status Engine::ensure(size_t n)
 {
     if (!stream_) return Status::internal;
@@ -2654,7 +2697,7 @@ Status Engine::ensure(size_t n)
         { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },
         { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },
     };
-    const Status st = carve(items, ws_, ws_bytes_);
+    const Status st = carve(items, ws_, ws_bytes_, alloc_budget());
     if (st != Status::ok) return st;
     b.qcap = (uint32_t)qcap;
     b.rec_cap = R;
@@ -2678,7 +2721,7 @@ Status Engine::ensure_nodes(const Plan& P)
         { (void**)&b.kn, 4 * nodes }, { (void**)&b.lvlK, 4 * kMaxLevels }, { (void**)&b.redo, 4 * (nodes + 1) },
         { (void**)&b.qv, 4 * (size_t)kSegMaxK * qv_nodes },
     };
-    const Status st = carve(items, ws_nodes_, ws_nodes_bytes_);
+    const Status st = carve(items, ws_nodes_, ws_nodes_bytes_, alloc_budget());
     if (st != Status::ok) return st;
     b.nodes_cap = nodes;
     b.offs_cap = offs;
@@ -2708,7 +2751,7 @@ Status Engine::ensure_anchors(size_t seg_nodes_need, size_t anchor_need)
         { (void**)&b.nsel, 4 * seg_nodes },
         { (void**)&b.alist, 4 * (2 + 4 * (size_t)kCross) * anchor_cap }, { (void**)&b.alist_n, 64 },
     };
-    const Status st = carve(items, ws_anchors_, ws_anchors_bytes_);
+    const Status st = carve(items, ws_anchors_, ws_anchors_bytes_, alloc_budget());
     if (st != Status::ok) return st;
     b.seg_nodes = seg_nodes;
     b.anchor_cap = anchor_cap;
@@ -2741,7 +2784,7 @@ Status Engine::ensure_sets(size_t k, size_t n)
         { (void**)&b.mstart, 4 * K }, { (void**)&b.sstart, 4 * K }, { (void**)&b.toff, 4 * K },
         { (void**)&b.trank, 4 * T },  { (void**)&b.btris, 12 * T }, { (void**)&b.tmain, 64 },
     };
-    const Status st = carve(items, ws_sets_, ws_sets_bytes_);
+    const Status st = carve(items, ws_sets_, ws_sets_bytes_, alloc_budget());
     if (st != Status::ok) return st;
     b.sets_cap = K - 1;
     b.tris_cap = T;
@@ -3051,7 +3094,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         QDT_HIP(hipMemsetAsync(b.OR + 6 * (size_t)m_forest, 0xff, 4 * 6 * (size_t)(m - m_forest), s));
     const int lim_e = (int)(4 * b.rec_cap);
     const int spare_base = (int)(3 * (size_t)m);
-    const int spare_cap = (int)(b.rec_cap - 3 * (size_t)m);
+    const int spare_cap = (int)(spare_limit_ ? std::min<size_t>(spare_limit_, b.rec_cap - 3 * (size_t)m)
+                                             : b.rec_cap - 3 * (size_t)m);
     // 4. frame-1 order: (y asc, x desc).  No run of equal y longer than
     // kYRunMax: the representatives of the first sort's y order with its runs
     // re-sorted (several sets: that order grouped by set, stably).
@@ -3313,6 +3357,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                                                           (uint32_t)spare_cap, b.face, b.queue, b.qcap, b.counters,
                                                           b.err);
     k_extract_exact<<<256, kBlock, 0, s>>>(b.NX, b.OR, b.P0, m, lim_e, b.queue, b.counters, b.qcap, b.face);
+    k_extract_overflow<<<1024, kBlock, 0, s>>>(b.NX, b.OR, b.P0, nrec, m, lim_e, b.counters, b.qcap, b.face);
     scan_u32(FaceCount{ b.face }, nrec, FacePrefix{ b.face }, b.partials, s);
     k_face_write<<<cdiv(nrec, kBlock), kBlock, 0, s>>>(b.face, b.NX, b.OR, b.ID0, nrec, m, lim_e,
                                                         multi ? 3 * m : 0xffffffffu, multi ? b.tmain : b.partials,

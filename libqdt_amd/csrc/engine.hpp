@@ -154,6 +154,15 @@ public:
     void set_cells(bool on) { cells_ = on; }
     // Device bytes currently held.
     size_t workspace_bytes() const;
+    // Tests of the fallback paths: the device bytes the workspace may hold
+    // (a larger request fails as out of memory; 0 = no limit) and the spare
+    // quad-edge records seam pieces may draw (0 = the default region; too
+    // few make a run repeat without seam pieces).
+    void set_limits(size_t mem_bytes, size_t spare_records)
+    {
+        mem_cap_ = mem_bytes;
+        spare_limit_ = spare_records;
+    }
 
     struct Plan;
 
@@ -177,6 +186,8 @@ private:
     bool profiling_ = false;
     bool seams_ = true;
     bool cells_ = true;
+    size_t mem_cap_ = 0, spare_limit_ = 0;
+    size_t alloc_budget() const;
     std::vector<hipEvent_t> evs_; // profiling events, created on demand
     hipEvent_t prof_event(size_t i);
     DeviceResult res_;

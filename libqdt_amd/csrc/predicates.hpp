@@ -12,7 +12,8 @@
 // evaluated in 128-bit (orient2d) or 256-bit (incircle) two's-complement
 // integers.  Otherwise the expanded monomials are summed into a wide
 // fixed-point accumulator.
-// Both are exact for every pair of finite doubles, so the sign equals the
+// Both are exact for all finite doubles (a difference that overflows is kept
+// as its two terms, see exact_diff), so the sign equals the
 // sign of the true determinant, which is what the reference's exact path
 // (robust.hxx:297-396) computes.
 #pragma once
@@ -47,6 +48,22 @@ QDT_HD inline void two_sum(double a, double b, double& x, double& y)
     const double bv = x - a;
     const double av = x - bv;
     y = (a - av) + (b - bv);
+}
+
+// a - b as two dyadic parts whose exact sum is the difference: the
+// error-free (hi, lo) of two_sum, or (a, -b) itself when the rounded
+// difference or one of two_sum's intermediate differences overflows
+// (coordinates of magnitude near DBL_MAX; hi or lo is then infinite or NaN).
+// The integer stages below only need hi + lo to be exact, not the parts to
+// be non-overlapping.
+QDT_HD inline void exact_diff(double a, double b, double& hi, double& lo)
+{
+    two_sum(a, -b, hi, lo);
+    if (!(fabs(hi) <= 1.7976931348623157e308 && fabs(lo) <= 1.7976931348623157e308))
+    {
+        hi = a;
+        lo = -b;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -360,10 +377,10 @@ QDT_HD_NOINLINE int incircle_general(const double (&hi)[6], const double (&lo)[6
 QDT_HD_NOINLINE int orient2d_exact(double ax, double ay, double bx, double by, double cx, double cy)
 {
     double hi[4], lo[4];
-    two_sum(ax, -cx, hi[0], lo[0]);
-    two_sum(ay, -cy, hi[1], lo[1]);
-    two_sum(bx, -cx, hi[2], lo[2]);
-    two_sum(by, -cy, hi[3], lo[3]);
+    exact_diff(ax, cx, hi[0], lo[0]);
+    exact_diff(ay, cy, hi[1], lo[1]);
+    exact_diff(bx, cx, hi[2], lo[2]);
+    exact_diff(by, cy, hi[3], lo[3]);
     int64_t d[4];
     if (integerize<4>(hi, lo, d))
     {
@@ -377,12 +394,12 @@ QDT_HD_NOINLINE int incircle_exact(double ax, double ay, double bx, double by, d
                                            double dx, double dy)
 {
     double hi[6], lo[6];
-    two_sum(ax, -dx, hi[0], lo[0]);
-    two_sum(ay, -dy, hi[1], lo[1]);
-    two_sum(bx, -dx, hi[2], lo[2]);
-    two_sum(by, -dy, hi[3], lo[3]);
-    two_sum(cx, -dx, hi[4], lo[4]);
-    two_sum(cy, -dy, hi[5], lo[5]);
+    exact_diff(ax, dx, hi[0], lo[0]);
+    exact_diff(ay, dy, hi[1], lo[1]);
+    exact_diff(bx, dx, hi[2], lo[2]);
+    exact_diff(by, dy, hi[3], lo[3]);
+    exact_diff(cx, dx, hi[4], lo[4]);
+    exact_diff(cy, dy, hi[5], lo[5]);
     int64_t v[6];
     if (integerize<6>(hi, lo, v))
     {

@@ -88,6 +88,7 @@ struct qdt_t
     bool profiling = false;
     bool seams = true;
     bool cells = true;
+    size_t mem_cap = 0, spare_limit = 0; // test limits (qdt_gpu_set_test_limits)
     // constraints accepted so far (pairs of representative indices)
     std::set<std::pair<size_t, size_t>> cons;
 
@@ -186,6 +187,7 @@ qdt_status ensure_engine(qdt_t* Q)
     Q->eng->set_profiling(Q->profiling);
     Q->eng->set_seams(Q->seams);
     Q->eng->set_cells(Q->cells);
+    Q->eng->set_limits(Q->mem_cap, Q->spare_limit);
     if (!Q->eng->stream())
     {
         delete Q->eng;
@@ -670,6 +672,14 @@ void qdt_gpu_set_cells(qdt_t* ptr, int enable)
     if (!ptr) return;
     ptr->cells = enable != 0;
     if (ptr->eng) ptr->eng->set_cells(ptr->cells);
+}
+
+void qdt_gpu_set_test_limits(qdt_t* ptr, size_t device_bytes, size_t spare_records)
+{
+    if (!ptr) return;
+    ptr->mem_cap = device_bytes;
+    ptr->spare_limit = spare_records;
+    if (ptr->eng) ptr->eng->set_limits(device_bytes, spare_records);
 }
 
 int qdt_gpu_device_count(void)

@@ -30,6 +30,7 @@ namespace
 
 constexpr size_t kMaxPoints = 100000000;  // per request, libqdt/wrapper.hxx:15
 constexpr size_t kBytesPerPoint = 320;    // device workspace + input per point (engine buffers, rounded up)
+constexpr size_t kBytesPerBatch = 4u << 20; // fixed part: spare quad-edge records, node, anchor and per-set buffers
 constexpr size_t kMaxBatchPoints = 64u << 20;
 
 double now_ms()
@@ -67,6 +68,12 @@ struct Worker
     void* h_in = nullptr; // pinned staging of a batch's points
     size_t h_in_cap = 0;
     std::thread th;
+    // device bytes charged to this worker against the budget (guarded by the
+    // server's mutex): while a batch runs, the larger of what it holds and
+    // the batch's estimate; while idle, what its engine and input buffer
+    // still hold (kept for the next batch)
+    size_t held = 0;
+    bool idle = true;
 };
 
 bool grow_pinned(void*& p, size_t& cap, size_t bytes)
@@ -97,8 +104,7 @@ bool grow_device(void*& p, size_t& cap, size_t bytes)
 
 struct qdt_gpu_server
 {
-    size_t budget = 0;      // device bytes the in-flight batches may use
-    size_t in_flight = 0;   // device bytes of the batches running now
+    size_t budget = 0;      // device bytes all workers together may hold
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
@@ -108,8 +114,17 @@ struct qdt_gpu_server
     std::vector<Worker*> workers;
     std::vector<ResultBuf*> pool; // free result buffers
     uint64_t batches = 0, batched_requests = 0;
+    size_t peak_held = 0; // largest sum of the workers' charges so far
+    void note_peak()      // with mu held
+    {
+        size_t t = 0;
+        for (const Worker* v : workers) t += std::max(v->held, v->eng->workspace_bytes() + v->d_in_cap);
+        peak_held = std::max(peak_held, t);
+    }
 
     void work(Worker* w);
+    size_t held_by_others(const Worker* w) const; // with mu held
+    bool fits(Worker* w, size_t need);            // with mu held
     ResultBuf* get_buf(size_t bytes);   // with mu held
     void put_ref(ResultBuf* b);         // with mu held
     ~qdt_gpu_server();
@@ -158,6 +173,44 @@ void qdt_gpu_server::put_ref(ResultBuf* b)
     }
 }
 
+size_t qdt_gpu_server::held_by_others(const Worker* w) const
+{
+    size_t t = 0;
+    for (const Worker* v : workers)
+        if (v != w) t += v->held;
+    return t;
+}
+
+// Whether worker w may run a batch estimated at `need` bytes: what the other
+// workers hold plus the larger of w's own retained memory and the estimate
+// stays within the budget.  Idle workers give their retained memory back
+// when that is what stands in the way (their engines are not in use while
+// they wait for work, and the mutex is held).
+bool qdt_gpu_server::fits(Worker* w, size_t need)
+{
+    if (held_by_others(w) + std::max(w->held, need) <= budget) return true;
+    for (Worker* v : workers)
+    {
+        if (v == w || !v->idle || !v->held) continue;
+        v->eng->release();
+        if (v->d_in) (void)hipFree(v->d_in);
+        v->d_in = nullptr;
+        v->d_in_cap = 0;
+        v->held = 0;
+        if (held_by_others(w) + std::max(w->held, need) <= budget) return true;
+    }
+    if (w->held > need)
+    {
+        // its own retained workspace is larger than this batch needs
+        w->eng->release();
+        if (w->d_in) (void)hipFree(w->d_in);
+        w->d_in = nullptr;
+        w->d_in_cap = 0;
+        w->held = 0;
+    }
+    return held_by_others(w) + need <= budget;
+}
+
 void qdt_gpu_server::work(Worker* w)
 {
     std::vector<Request*> take;
@@ -167,6 +220,7 @@ void qdt_gpu_server::work(Worker* w)
         size_t need = 0;
         {
             std::unique_lock<std::mutex> lk(mu);
+            w->idle = true;
             cv_work.wait(lk, [&] { return stop || !queue.empty(); });
             if (stop && queue.empty()) return;
             // admission: the queue's head requests while they fit the budget
@@ -175,11 +229,11 @@ void qdt_gpu_server::work(Worker* w)
             while (!queue.empty())
             {
                 Request* r = queue.front();
-                const size_t b = kBytesPerPoint * std::max<size_t>(r->pts.size(), 64);
-                if (!take.empty() && (pts + r->pts.size() > kMaxBatchPoints || in_flight + need + b > budget)) break;
-                if (take.empty() && in_flight + b > budget)
+                const size_t b = kBytesPerPoint * std::max<size_t>(r->pts.size(), 64) + kBytesPerBatch;
+                if (!take.empty() && (pts + r->pts.size() > kMaxBatchPoints || !fits(w, need + b))) break;
+                if (take.empty() && !fits(w, b))
                 {
-                    if (in_flight == 0)
+                    if (b > budget)
                     {
                         // alone it does not fit: reject it
                         queue.pop_front();
@@ -200,15 +254,17 @@ void qdt_gpu_server::work(Worker* w)
             if (take.empty())
             {
                 // the head does not fit next to the running batches: wait
-                // until enough of them finish
+                // until a batch ends
                 cv_done.wait(lk, [&] {
                     if (stop || queue.empty()) return true;
-                    const size_t hb = kBytesPerPoint * std::max<size_t>(queue.front()->pts.size(), 64);
-                    return in_flight + hb <= budget || in_flight == 0;
+                    const size_t hb = kBytesPerPoint * std::max<size_t>(queue.front()->pts.size(), 64) + kBytesPerBatch;
+                    return fits(w, hb);
                 });
                 continue;
             }
-            in_flight += need;
+            w->held = std::max(w->held, need);
+            w->idle = false;
+            note_peak();
             ++batches;
             batched_requests += take.size();
         }
@@ -276,7 +332,9 @@ void qdt_gpu_server::work(Worker* w)
                 r->state = 2;
             }
             if (buf && buf->refs == 0) pool.push_back(buf);
-            in_flight -= need;
+            // from now on the worker is charged what it really keeps
+            note_peak();
+            w->held = w->eng->workspace_bytes() + w->d_in_cap;
         }
         cv_done.notify_all();
         cv_work.notify_all();
@@ -401,9 +459,15 @@ qdt_status qdt_gpu_server_wait(qdt_gpu_server* S, uint64_t id, double* latency_m
 {
     if (!S) return QDT_ERR_NULL_CONTEXT;
     std::unique_lock<std::mutex> lk(S->mu);
-    Request* r = find(S, id);
+    if (!find(S, id)) return QDT_ERR_INVALID_INDEX;
+    // the request is looked up again on every wake-up: another thread may
+    // release it (and free it) once it is done
+    S->cv_done.wait(lk, [&] {
+        const Request* q = find(S, id);
+        return !q || q->state == 2;
+    });
+    const Request* r = find(S, id);
     if (!r) return QDT_ERR_INVALID_INDEX;
-    S->cv_done.wait(lk, [&] { return r->state == 2; });
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
     return r->status;
 }
@@ -455,6 +519,18 @@ void qdt_gpu_server_release(qdt_gpu_server* S, uint64_t id)
     S->reqs.erase(id);
     S->put_ref(r->buf);
     delete r;
+    S->cv_done.notify_all();
+}
+
+int qdt_gpu_server_memory(qdt_gpu_server* S, size_t* held, size_t* peak)
+{
+    if (!S) return 0;
+    std::lock_guard<std::mutex> lk(S->mu);
+    size_t t = 0;
+    for (const Worker* v : S->workers) t += v->held;
+    if (held) *held = t;
+    if (peak) *peak = S->peak_held;
+    return 1;
 }
 
 int qdt_gpu_server_counters(qdt_gpu_server* S, uint64_t* batches, uint64_t* requests)

@@ -8,6 +8,7 @@
  */
 #include "oracle.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -997,8 +998,35 @@ static int area_add(double* acc, int nacc, const double* xy, uint32_t a, uint32_
         goto done;                                       \
     } while (0)
 
+static int check_impl(const double* xy, size_t n, const uint32_t* tris, size_t ntri,
+                      char* msg, size_t msglen, int64_t info[4]);
+
+/* The checker's area sum multiplies coordinate differences without the
+ * predicates' normalisation, so a point set whose largest coordinate is far
+ * from 1 is first scaled by a power of two (as robust.hxx:297-314 scales the
+ * predicates' operands): exact unless a nonzero coordinate drops below the
+ * normal range, which the sets checked here (spread < 2^600) never do. */
 int orc_check(const double* xy, size_t n, const uint32_t* tris, size_t ntri,
               char* msg, size_t msglen, int64_t info[4])
+{
+    int emax = INT_MIN;
+    for (size_t i = 0; i < 2 * n; ++i)
+        if (xy[i] != 0.0)
+        {
+            const int e = ilogb(xy[i]);
+            if (e > emax) emax = e;
+        }
+    if (emax == INT_MIN || (emax >= -400 && emax <= 400)) return check_impl(xy, n, tris, ntri, msg, msglen, info);
+    double* sc = (double*)malloc(sizeof(double) * 2 * n);
+    if (!sc) return 99;
+    for (size_t i = 0; i < 2 * n; ++i) sc[i] = ldexp(xy[i], -emax);
+    const int rc = check_impl(sc, n, tris, ntri, msg, msglen, info);
+    free(sc);
+    return rc;
+}
+
+static int check_impl(const double* xy, size_t n, const uint32_t* tris, size_t ntri,
+                      char* msg, size_t msglen, int64_t info[4])
 {
     int rc = 0;
     uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));

@@ -227,6 +227,24 @@ def ref_batch_time(pts, offsets, threads):
     return dt, tot.value
 
 
+def ref_batch_hashes(pts, offsets, threads=8):
+    """Per set [offsets[i], offsets[i+1]): the reference's triangle count and
+    the FNV-1a-64 of its canonical triangle list (computed in the shim, a
+    second implementation beside canon_hash)."""
+    R = ref()
+    a = _xy(pts)
+    o = np.ascontiguousarray(offsets, dtype=np.uint64)
+    k = len(o) - 1
+    cnt = np.zeros(max(k, 1), np.uint64)
+    hs = np.zeros(max(k, 1), np.uint64)
+    R.ref_batch_hashes.restype = ctypes.c_int
+    R.ref_batch_hashes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+    if R.ref_batch_hashes(a.ctypes.data, o.ctypes.data, k, int(threads), cnt.ctypes.data, hs.ctypes.data) != 0:
+        raise RuntimeError("reference failed on a set")
+    return cnt[:k], hs[:k]
+
+
 def ref_mesh_time(pts, parallel=True):
     """Wall time (s) of the reference's half-edge mesh build (ensure_mesh_ /
     ensure_mesh_parallel_) after an untimed batch call on pts, and 3 T."""

@@ -93,3 +93,61 @@ extern "C" double ref_mesh_seconds(const double* xy, size_t n, int parallel, uin
     qdt_destroy(q);
     return dt;
 }
+
+// Per-set results of the reference for many independent sets (the batched
+// and served workloads' parity fixtures): set i = points [offs[i],
+// offs[i+1]) through qdt_execute on its own context, on `threads` host
+// threads.  counts[i] = triangles, hashes[i] = FNV-1a-64 of the canonical
+// triangle list (each triangle rotated so that its smallest index comes
+// first, the list sorted, every index written as 8 little-endian bytes).
+// Returns 0, or -1 when a set fails.
+#include <algorithm>
+#include <array>
+
+static uint64_t canon_fnv(const qdt_t* q)
+{
+    const wrapper::Triangulation* T = reinterpret_cast<const wrapper::Triangulation*>(q);
+    std::vector<std::array<uint64_t, 3>> c;
+    c.reserve(T->triangles.size());
+    for (const auto& t : T->triangles)
+    {
+        int r = 0;
+        if (t[1] < t[r]) r = 1;
+        if (t[2] < t[r]) r = 2;
+        c.push_back({ (uint64_t)t[r], (uint64_t)t[(r + 1) % 3], (uint64_t)t[(r + 2) % 3] });
+    }
+    std::sort(c.begin(), c.end());
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (const auto& t : c)
+        for (uint64_t v : t)
+            for (int b = 0; b < 8; ++b)
+            {
+                h ^= (v >> (8 * b)) & 0xff;
+                h *= 0x100000001b3ull;
+            }
+    return h;
+}
+
+extern "C" int ref_batch_hashes(const double* xy, const uint64_t* offs, size_t k, int threads, uint64_t* counts,
+                                uint64_t* hashes)
+{
+    std::atomic<size_t> next{ 0 };
+    std::atomic<int> bad{ 0 };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            qdt_t* q = qdt_create();
+            for (;;)
+            {
+                const size_t i = next.fetch_add(1);
+                if (i >= k) break;
+                const qdt_vec2d* p = reinterpret_cast<const qdt_vec2d*>(xy + 2 * offs[i]);
+                if (qdt_execute(q, p, (size_t)(offs[i + 1] - offs[i])) != QDT_OK) bad.fetch_add(1);
+                counts[i] = qdt_triangles_size(q);
+                hashes[i] = canon_fnv(q);
+            }
+            qdt_destroy(q);
+        });
+    for (std::thread& th : pool) th.join();
+    return bad.load() ? -1 : 0;
+}

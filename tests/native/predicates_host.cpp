@@ -19,22 +19,22 @@ int ph_incircle_exact(double ax, double ay, double bx, double by, double cx, dou
 int ph_orient2d_general(double ax, double ay, double bx, double by, double cx, double cy)
 {
     double hi[4], lo[4];
-    qdt::two_sum(ax, -cx, hi[0], lo[0]);
-    qdt::two_sum(ay, -cy, hi[1], lo[1]);
-    qdt::two_sum(bx, -cx, hi[2], lo[2]);
-    qdt::two_sum(by, -cy, hi[3], lo[3]);
+    qdt::exact_diff(ax, cx, hi[0], lo[0]);
+    qdt::exact_diff(ay, cy, hi[1], lo[1]);
+    qdt::exact_diff(bx, cx, hi[2], lo[2]);
+    qdt::exact_diff(by, cy, hi[3], lo[3]);
     return qdt::orient_general(hi, lo);
 }
 
 int ph_incircle_general(double ax, double ay, double bx, double by, double cx, double cy, double dx, double dy)
 {
     double hi[6], lo[6];
-    qdt::two_sum(ax, -dx, hi[0], lo[0]);
-    qdt::two_sum(ay, -dy, hi[1], lo[1]);
-    qdt::two_sum(bx, -dx, hi[2], lo[2]);
-    qdt::two_sum(by, -dy, hi[3], lo[3]);
-    qdt::two_sum(cx, -dx, hi[4], lo[4]);
-    qdt::two_sum(cy, -dy, hi[5], lo[5]);
+    qdt::exact_diff(ax, dx, hi[0], lo[0]);
+    qdt::exact_diff(ay, dy, hi[1], lo[1]);
+    qdt::exact_diff(bx, dx, hi[2], lo[2]);
+    qdt::exact_diff(by, dy, hi[3], lo[3]);
+    qdt::exact_diff(cx, dx, hi[4], lo[4]);
+    qdt::exact_diff(cy, dy, hi[5], lo[5]);
     return qdt::incircle_general(hi, lo);
 }
 

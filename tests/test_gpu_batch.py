@@ -136,3 +136,32 @@ def test_copies_without_y_ties(gpu):
         sets.append(p)
     sets.append(sets[0].copy())  # the same points in another set
     check_sets("copies, no y ties", sets)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not present")
+def test_c4_full_batch_matches_reference(gpu):
+    """The batched workload at its own size: 4,096 sets, sizes log-uniform in
+    [64, 65536] (seed 3), uniform / Gaussian / 32-cluster mix, in one
+    qdt_gpu_batch_execute_device call; every set's triangle count and
+    canonical FNV-1a-64 equal the compiled reference's qdt_execute on that
+    set alone (one context per host thread)."""
+    import torch
+
+    sets = W.batch_sets(count=4096, seed=3)
+    pts, offs = concat(sets)
+    print("case: C4 4096 sets,", len(pts), "points", flush=True)
+    d = torch.from_numpy(pts).to("cuda:0")
+    b = Q.Batch()
+    assert b.execute_device(d.data_ptr(), offs) == Q.QDT_OK
+    assert b.sets() == 4096
+    allt = b.all_triangles()
+    print("case: C4 reference hashes", flush=True)
+    cnt, hs = O.ref_batch_hashes(pts, offs, threads=16)
+    bad = []
+    for i in range(len(sets)):
+        got = b.triangles(i, allt)
+        if len(got) != int(cnt[i]) or O.canon_hash(got) != int(hs[i]):
+            bad.append(i)
+    assert not bad, "sets differing from the reference: %s" % bad[:20]
+    assert int(allt.shape[0]) == int(cnt.sum())
+    b.close()

@@ -96,6 +96,26 @@ def test_scaled(gpu):
             check_same("scale 2^%d #%d" % (k, rep), p)
 
 
+def test_scaled_large(gpu):
+    """Tens of thousands of points at magnitudes where the filters decide
+    nothing (products below their floor, or overflowing): every extraction
+    test goes exact, more than the extraction queue holds (a quarter of the
+    points), so the overflow pass decides the rest.  Near +-DBL_MAX the
+    coordinate differences themselves overflow."""
+    rng = np.random.default_rng(4)
+    big = 1.7976931348623157e308
+    cases = {
+        "uniform 30k x 2^-600": rng.random((30000, 2)) * 2.0 ** -600,
+        "uniform 30k x 2^600": rng.random((30000, 2)) * 2.0 ** 600,
+        "uniform 30k in +-DBL_MAX": (rng.random((30000, 2)) * 2 - 1) * big,
+        "integers 20k x 2^-1000": rng.integers(-1000, 1000, (20000, 2)).astype(np.float64) * 2.0 ** -1000,
+        "integers 20k x 2^990": rng.integers(-1000, 1000, (20000, 2)).astype(np.float64) * 2.0 ** 990,
+    }
+    for name, p in cases.items():
+        st = check_same(name, p)
+        assert st["extract_exact"] > len(p) // 4, (name, st["extract_exact"])
+
+
 def test_random_distributions(gpu):
     check_same("uniform 100k seed 0", W.uniform(100000, 0))
     check_same("gaussian 100k", W.gaussian(100000, 1))

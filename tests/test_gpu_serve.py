@@ -54,9 +54,73 @@ def test_requests_alone_and_in_the_mix(gpu):
 
 def test_budget_rejects_what_cannot_fit(gpu):
     print("case: budget", flush=True)
-    with Q.Server(hbm_budget_bytes=1 << 20, workers=1) as srv:
+    # a request is charged ~320 B per point plus 4 MB per batch: 1,000 points
+    # fit in 16 MB, 100,000 do not
+    with Q.Server(hbm_budget_bytes=16 << 20, workers=1) as srv:
         small = srv.submit(W.uniform(1000, 1))
         big = srv.submit(W.uniform(100000, 2))
         assert srv.wait(small)[0] == Q.QDT_OK
         assert srv.wait(big)[0] == Q.QDT_ERR_OUT_OF_MEMORY
         assert srv.poll(12345) == -1
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not present")
+def test_served_workload_matches_reference(gpu):
+    """The served workload's requests at their own sizes: 32 requests of
+    log-uniform size in [1,000, 2,000,000] (seed 5, the uniform / Gaussian /
+    32-cluster mix), submitted with Poisson arrivals at 500 requests/s into a
+    server with a 200 GB budget and 4 workers; each result equals the
+    compiled reference's qdt_execute of that request alone."""
+    nreq = 32
+    sizes = W.log_uniform_sizes(nreq, 1000, 2_000_000, 5)
+    reqs = [W.mixed_set(int(sz), 5_000_011 + i)[0] for i, sz in enumerate(sizes)]
+    offs = np.zeros(nreq + 1, np.uint64)
+    for i, r in enumerate(reqs):
+        offs[i + 1] = offs[i] + len(r)
+    print("case: served 32 requests,", int(offs[-1]), "points", flush=True)
+    cnt, hs = O.ref_batch_hashes(np.concatenate(reqs), offs, threads=16)
+    gaps = np.random.default_rng(17).exponential(1.0 / 500, size=nreq)
+    with Q.Server(hbm_budget_bytes=200 << 30, workers=4) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        at = 0.0
+        for i, r in enumerate(reqs):
+            at += gaps[i]
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(r))
+        for i, rid in enumerate(ids):
+            st, lat = srv.wait(rid)
+            assert st == Q.QDT_OK, (i, Q.status_string(st))
+            got = srv.triangles(rid)
+            assert len(got) == int(cnt[i]), i
+            assert O.canon_hash(got) == int(hs[i]), i
+            srv.release(rid)
+        batches, served = srv.counters()
+        assert served == nreq
+        held, peak = srv.memory()
+        print("served: %d batches, device bytes held %d, peak %d" % (batches, held, peak), flush=True)
+        assert 0 < peak <= 200 << 30
+
+
+def test_budget_bounds_the_workers_memory(gpu):
+    """A budget below what four workers would keep: the workers' charges
+    (retained workspaces included) never exceed it, idle workers give memory
+    back, and every request still completes with the oracle's result."""
+    budget = 1 << 30
+    sizes = [int(x) for x in W.log_uniform_sizes(24, 50000, 1_200_000, 11)]
+    reqs = [W.uniform(n, 7000 + i) for i, n in enumerate(sizes)]
+    print("case: budget 1 GiB, 24 requests,", sum(sizes), "points", flush=True)
+    with Q.Server(hbm_budget_bytes=budget, workers=4) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        for i, rid in enumerate(ids):
+            st, _ = srv.wait(rid)
+            assert st == Q.QDT_OK, (i, Q.status_string(st))
+            if i % 6 == 0:
+                want = O.triangulate(reqs[i])
+                assert np.array_equal(O.canonical(srv.triangles(rid)), O.canonical(want.tris)), i
+            srv.release(rid)
+        held, peak = srv.memory()
+        print("held %d peak %d" % (held, peak), flush=True)
+        assert peak <= budget and held <= budget

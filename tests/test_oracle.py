@@ -78,6 +78,46 @@ def test_readme_kats():
     assert FIX["cases"]["duplicates 5"]["used"] == [1, 1, 0, 1, 0]
 
 
+def _fnv1a64_py(rows):
+    """Textbook FNV-1a-64 (offset basis 0xcbf29ce484222325, prime
+    0x100000001b3) over every index as 8 little-endian bytes, in pure
+    Python: an implementation independent of the oracle's."""
+    h = 0xcbf29ce484222325
+    for row in rows:
+        for v in row:
+            for b in int(v).to_bytes(8, "little"):
+                h ^= b
+                h = (h * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_canonical_form_and_hash_independently():
+    """orc_canonicalize / orc_canon_hash against hand-written canonical lists
+    and a pure-Python FNV-1a-64: the rotation puts the smallest index first
+    (keeping the cyclic order, so orientation is kept), then rows sort
+    lexicographically."""
+    tris = np.array([[3, 0, 1], [1, 2, 3], [9, 4, 7], [7, 9, 4], [2, 8, 5]], np.uint32)
+    want = [[0, 1, 3], [1, 2, 3], [2, 8, 5], [4, 7, 9], [4, 7, 9]]
+    assert O.canonical(tris).tolist() == want
+    assert O.canon_hash(tris) == _fnv1a64_py(want)
+    assert O.canon_hash(np.zeros((0, 3), np.uint32)) == 0xcbf29ce484222325
+    # the fixtures' small canonical lists, written by the compiled reference
+    n = 0
+    for name, case in FIX["cases"].items():
+        if "canonical" in case and case["canonical"]:
+            assert "%016x" % _fnv1a64_py(case["canonical"]) == case["fnv"], name
+            n += 1
+    assert n >= 5
+    # a larger random list: canonicalise in numpy by hand, hash in Python
+    rng = np.random.default_rng(5)
+    t = rng.integers(0, 1000, size=(500, 3)).astype(np.uint32)
+    r = np.argmin(t, axis=1)
+    rot = np.stack([t[np.arange(len(t)), (r + k) % 3] for k in range(3)], axis=1)
+    rot = rot[np.lexsort((rot[:, 2], rot[:, 1], rot[:, 0]))]
+    assert O.canonical(t).tolist() == rot.tolist()
+    assert O.canon_hash(t) == _fnv1a64_py(rot.tolist())
+
+
 def test_dedupe_rule_closed_form():
     rng = np.random.default_rng(3)
     pts = rng.integers(0, 20, size=(3000, 2)).astype(np.float64)

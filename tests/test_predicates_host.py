@@ -75,6 +75,39 @@ def test_exact_paths_against_rationals():
         assert L.ph_incircle_general(*a, *b, *c, *d) == t
 
 
+def test_coordinates_near_dbl_max():
+    """Coordinates up to +-1.7e308 with opposite signs: the rounded
+    differences overflow, the exact paths must still give the true sign."""
+    L = lib()
+    rng = random.Random(3)
+    big = 1.7976931348623157e308
+    for it in range(4000):
+        p = []
+        for _ in range(4):
+            mode = rng.randint(0, 3)
+            if mode == 0:
+                v = (rng.choice([-1, 1]) * big * rng.uniform(0.5, 1.0), rng.choice([-1, 1]) * big * rng.uniform(0.5, 1.0))
+            elif mode == 1:
+                v = (rng.choice([-1.0, 1.0]) * big, rng.uniform(-1, 1) * 2.0 ** rng.randint(900, 1023))
+            elif mode == 2:
+                v = (rng.randint(-3, 3) * 2.0 ** 1022, rng.randint(-3, 3) * 2.0 ** 1022)
+            else:
+                v = (rng.uniform(-1, 1) * 2.0 ** rng.randint(-1074, 1023), rng.uniform(-1, 1))
+            p.append(v)
+        if it % 5 == 0:  # collinear: the midpoint of two representable points
+            p[2] = (p[0][0] / 2 + p[1][0] / 2, p[0][1] / 2 + p[1][1] / 2)
+        a, b, c, d = p
+        cnt = ctypes.c_uint(0)
+        t = orient_true(a, b, c)
+        assert L.ph_orient2d_exact(*a, *b, *c) == t
+        assert L.ph_orient2d_general(*a, *b, *c) == t
+        assert sgn(L.ph_orient2d(*a, *b, *c, ctypes.byref(cnt))) == t
+        t = incircle_true(a, b, c, d)
+        assert L.ph_incircle_exact(*a, *b, *c, *d) == t
+        assert L.ph_incircle_general(*a, *b, *c, *d) == t
+        assert sgn(L.ph_incircle(*a, *b, *c, *d, ctypes.byref(cnt))) == t
+
+
 def test_filtered_matches_exact_and_counts():
     L = lib()
     rng = random.Random(2)
