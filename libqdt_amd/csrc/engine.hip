@@ -2640,6 +2640,47 @@ void Engine::release()
 
 size_t Engine::workspace_bytes() const { return ws_bytes_ + ws_nodes_bytes_ + ws_anchors_bytes_ + ws_sets_bytes_; }
 
+namespace
+{
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+} // namespace
+
+// Device bytes the workspace will hold after a run over n points in k sets:
+// every region keeps its size or grows to what the run needs (ensure,
+// ensure_sets: the same sizes as there; the node and seam regions depend on
+// the distinct points' tree, bounded here by 96 B per point and 8 MB).
+size_t Engine::bytes_after(size_t n, size_t k) const
+{
+    const size_t N = std::max<size_t>(n, 64);
+    size_t ws = ws_bytes_;
+    if (n > cap_n_ || !ws_)
+    {
+        const size_t nb_sort = cdiv(N, kSortTile);
+        const size_t R = 3 * N + N / 2 + 65536;
+        const size_t nb_scan = cdiv(R, kScanTile) + 1;
+        const size_t qcap = std::max<size_t>(4096, N / 4);
+        const size_t per[] = { 8 * N, 8 * N, 8 * N, 8 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N,
+                               4 * N, 4 * N, 4 * N, 4 * N, 4 * 6 * N, 4 * R, 4 * qcap,
+                               4 * (nb_scan + 1 + 256 * nb_sort), 4 * 256 * nb_sort, 4 * 8 * 256, 4 * N, 4 * N,
+                               16 * N, 16 * N, N, 4 * 4 * R, 4 * 2 * R, 8 * 16, 64, 4 * N, 4 * N, 64, 64 };
+        ws = 0;
+        for (size_t b : per) ws += round256(b);
+    }
+    size_t sets = ws_sets_bytes_;
+    if (k)
+    {
+        const size_t tris = 2 * N;
+        if (!ws_sets_ || k > b_->sets_cap || tris > b_->tris_cap)
+        {
+            const size_t K = std::max(k, b_->sets_cap) + 1;
+            const size_t T = std::max(tris, b_->tris_cap);
+            sets = 9 * round256(4 * K) + round256(4 * T) + round256(12 * T) + round256(64);
+        }
+    }
+    const size_t tree = std::max(ws_nodes_bytes_ + ws_anchors_bytes_, 96 * N + (8u << 20));
+    return std::max(ws, ws_bytes_) + std::max(sets, ws_sets_bytes_) + tree;
+}
+
 // bytes a new region may take under the test limit (regions being replaced
 // are freed before they are carved again)
 size_t Engine::alloc_budget() const

@@ -154,6 +154,9 @@ public:
     void set_cells(bool on) { cells_ = on; }
     // Device bytes currently held.
     size_t workspace_bytes() const;
+    // Device bytes held after a run over n points in k sets (k = 0: a
+    // single set), for admission by memory budget.
+    size_t bytes_after(size_t n, size_t k) const;
     // Tests of the fallback paths: the device bytes the workspace may hold
     // (a larger request fails as out of memory; 0 = no limit) and the spare
     // quad-edge records seam pieces may draw (0 = the default region; too

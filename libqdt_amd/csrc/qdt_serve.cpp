@@ -29,8 +29,6 @@ namespace
 {
 
 constexpr size_t kMaxPoints = 100000000;  // per request, libqdt/wrapper.hxx:15
-constexpr size_t kBytesPerPoint = 320;    // device workspace + input per point (engine buffers, rounded up)
-constexpr size_t kBytesPerBatch = 4u << 20; // fixed part: spare quad-edge records, node, anchor and per-set buffers
 constexpr size_t kMaxBatchPoints = 64u << 20;
 
 double now_ms()
@@ -124,7 +122,7 @@ struct qdt_gpu_server
 
     void work(Worker* w);
     size_t held_by_others(const Worker* w) const; // with mu held
-    bool fits(Worker* w, size_t need);            // with mu held
+    bool fits(Worker* w, size_t pts, size_t k);   // with mu held
     ResultBuf* get_buf(size_t bytes);   // with mu held
     void put_ref(ResultBuf* b);         // with mu held
     ~qdt_gpu_server();
@@ -181,34 +179,39 @@ size_t qdt_gpu_server::held_by_others(const Worker* w) const
     return t;
 }
 
-// Whether worker w may run a batch estimated at `need` bytes: what the other
-// workers hold plus the larger of w's own retained memory and the estimate
-// stays within the budget.  Idle workers give their retained memory back
-// when that is what stands in the way (their engines are not in use while
-// they wait for work, and the mutex is held).
-bool qdt_gpu_server::fits(Worker* w, size_t need)
+// Device bytes worker w holds after running a batch of `pts` points in `k`
+// requests: its engine's regions at the larger of their current size and
+// what the batch needs, and the input buffer.
+size_t charge(const Worker* w, size_t pts, size_t k)
 {
-    if (held_by_others(w) + std::max(w->held, need) <= budget) return true;
+    return w->eng->bytes_after(pts, k) + std::max(w->d_in_cap, 16 * std::max<size_t>(pts, 1));
+}
+
+void drop_memory(Worker* v)
+{
+    v->eng->release();
+    if (v->d_in) (void)hipFree(v->d_in);
+    v->d_in = nullptr;
+    v->d_in_cap = 0;
+    v->held = 0;
+}
+
+// Whether worker w may run a batch of `pts` points in `k` requests within
+// the budget, next to what the other workers hold.  Idle workers give their
+// retained memory back when that is what stands in the way (their engines
+// are not in use while they wait for work, and the mutex is held), and so
+// does w itself when its retained regions are larger than the batch needs.
+bool qdt_gpu_server::fits(Worker* w, size_t pts, size_t k)
+{
+    if (held_by_others(w) + charge(w, pts, k) <= budget) return true;
     for (Worker* v : workers)
     {
         if (v == w || !v->idle || !v->held) continue;
-        v->eng->release();
-        if (v->d_in) (void)hipFree(v->d_in);
-        v->d_in = nullptr;
-        v->d_in_cap = 0;
-        v->held = 0;
-        if (held_by_others(w) + std::max(w->held, need) <= budget) return true;
+        drop_memory(v);
+        if (held_by_others(w) + charge(w, pts, k) <= budget) return true;
     }
-    if (w->held > need)
-    {
-        // its own retained workspace is larger than this batch needs
-        w->eng->release();
-        if (w->d_in) (void)hipFree(w->d_in);
-        w->d_in = nullptr;
-        w->d_in_cap = 0;
-        w->held = 0;
-    }
-    return held_by_others(w) + need <= budget;
+    if (w->held) drop_memory(w);
+    return held_by_others(w) + charge(w, pts, k) <= budget;
 }
 
 void qdt_gpu_server::work(Worker* w)
@@ -229,11 +232,11 @@ void qdt_gpu_server::work(Worker* w)
             while (!queue.empty())
             {
                 Request* r = queue.front();
-                const size_t b = kBytesPerPoint * std::max<size_t>(r->pts.size(), 64) + kBytesPerBatch;
-                if (!take.empty() && (pts + r->pts.size() > kMaxBatchPoints || !fits(w, need + b))) break;
-                if (take.empty() && !fits(w, b))
+                const size_t np = pts + r->pts.size();
+                if (!take.empty() && (np > kMaxBatchPoints || !fits(w, np, take.size() + 1))) break;
+                if (take.empty() && !fits(w, np, 1))
                 {
-                    if (b > budget)
+                    if (held_by_others(w) == 0 && w->held == 0)
                     {
                         // alone it does not fit: reject it
                         queue.pop_front();
@@ -248,20 +251,16 @@ void qdt_gpu_server::work(Worker* w)
                 queue.pop_front();
                 r->state = 1;
                 take.push_back(r);
-                need += b;
-                pts += r->pts.size();
+                pts = np;
             }
             if (take.empty())
             {
                 // the head does not fit next to the running batches: wait
                 // until a batch ends
-                cv_done.wait(lk, [&] {
-                    if (stop || queue.empty()) return true;
-                    const size_t hb = kBytesPerPoint * std::max<size_t>(queue.front()->pts.size(), 64) + kBytesPerBatch;
-                    return fits(w, hb);
-                });
+                cv_done.wait(lk, [&] { return stop || queue.empty() || fits(w, queue.front()->pts.size(), 1); });
                 continue;
             }
+            need = charge(w, pts, take.size());
             w->held = std::max(w->held, need);
             w->idle = false;
             note_peak();
