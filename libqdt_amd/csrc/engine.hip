@@ -3661,6 +3661,48 @@ __global__ __launch_bounds__(kBlock) void k_mesh_twins(const uint64_t* keys,
     atomicAdd(&cnt[kmUnmatched], (unsigned long long)g);
 }
 
+// Twins only, in the triangle-major half-edge numbering (3t+i from corner
+// i to corner i+1): the start of an insertion or constraint on the host.
+__global__ __launch_bounds__(kBlock) void k_twin_keys(const uint32_t* tris, uint32_t T, uint64_t* keys,
+                                                      uint32_t* vals, int* twin)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t v[3] = { tris[3 * (size_t)t], tris[3 * (size_t)t + 1], tris[3 * (size_t)t + 2] };
+    for (int i = 0; i < 3; ++i)
+    {
+        const uint32_t h = 3 * t + i;
+        const uint32_t a = v[i], b = v[(i + 1) % 3];
+        keys[h] = ((uint64_t)min(a, b) << 32) | max(a, b);
+        vals[h] = h | (a > b ? 0x80000000u : 0u);
+        twin[h] = -1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_twin_pairs(const uint64_t* keys, const uint32_t* vals, uint32_t E,
+                                                       int* twin, unsigned long long* bad)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= E) return;
+    const uint64_t key = keys[p];
+    if (p > 0 && keys[p - 1] == key) return;
+    uint32_t g = 1;
+    while (p + g < E && keys[p + g] == key) ++g;
+    if (g == 1) return;
+    if (g == 2)
+    {
+        const uint32_t a = vals[p], b = vals[p + 1];
+        if ((a ^ b) >> 31)
+        {
+            const uint32_t ha = a & 0x7fffffffu, hb = b & 0x7fffffffu;
+            twin[ha] = (int)hb;
+            twin[hb] = (int)ha;
+            return;
+        }
+    }
+    atomicAdd(bad, (unsigned long long)g);
+}
+
 namespace
 {
 
@@ -3841,6 +3883,37 @@ Status build_mesh(const uint32_t* tris, uint32_t T, uint32_t n, int4* he, int2* 
     out.boundary = c[kmBoundary];
     out.unmatched = c[kmUnmatched];
     out.bad_index = c[kmBadIndex];
+    return Status::ok;
+}
+
+Status mesh_twins(const uint32_t* tris, uint32_t T, int* twin, hipStream_t s, uint64_t& unmatched)
+{
+    unmatched = 0;
+    const size_t E = 3 * (size_t)T;
+    if (E >= 0x80000000ull) return Status::internal;
+    if (!T) return Status::ok;
+    const uint32_t nb = cdiv(E, kSortTile);
+    TempBuffers tmp;
+    uint64_t *kA = nullptr, *kB = nullptr;
+    uint32_t *vA = nullptr, *vB = nullptr, *hist = nullptr, *partials = nullptr, *ghist = nullptr;
+    unsigned long long* cnt = nullptr;
+    Status st;
+    if ((st = tmp.get(kA, E)) != Status::ok || (st = tmp.get(kB, E)) != Status::ok ||
+        (st = tmp.get(vA, E)) != Status::ok || (st = tmp.get(vB, E)) != Status::ok ||
+        (st = tmp.get(hist, 256 * (size_t)nb)) != Status::ok ||
+        (st = tmp.get(partials, cdiv(256 * (size_t)nb, kScanTile) + 2 + 256 * (size_t)nb)) != Status::ok ||
+        (st = tmp.get(ghist, 8 * 256)) != Status::ok || (st = tmp.get(cnt, 1)) != Status::ok)
+        return st;
+    std::vector<uint32_t> host_ghist(8 * 256);
+    QDT_HIP(hipMemsetAsync(cnt, 0, 8, s));
+    k_twin_keys<<<cdiv(T, kBlock), kBlock, 0, s>>>(tris, T, kA, vA, twin);
+    st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, hist, partials, ghist, host_ghist.data(), s);
+    if (st != Status::ok) return st;
+    k_twin_pairs<<<cdiv(E, kBlock), kBlock, 0, s>>>(kA, vA, (uint32_t)E, twin, cnt);
+    unsigned long long c = 0;
+    QDT_HIP(hipMemcpyAsync(&c, cnt, 8, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    unmatched = c;
     return Status::ok;
 }
 

@@ -125,6 +125,13 @@ struct MeshReport
 Status build_mesh(const uint32_t* tris, uint32_t T, uint32_t n, int4* he, int2* vert, int* orig_to_local,
                   hipStream_t s, MeshReport& out);
 
+// The twin of every half-edge of T triangles (3 T input indices, device
+// memory), half-edge 3t+i running from corner i to corner (i+1)%3: twin
+// (device, 3 T ints) gets the opposite half-edge, -1 on the boundary.
+// unmatched counts half-edges of edges that are in more than two triangles
+// or twice in the same direction.  Temporary device memory is allocated.
+Status mesh_twins(const uint32_t* tris, uint32_t T, int* twin, hipStream_t s, uint64_t& unmatched);
+
 class Engine
 {
 public:

@@ -25,7 +25,7 @@
 // Everything here is __host__ __device__ so that the exact arithmetic can
 // also be checked on the CPU.
 #define QDT_HD __host__ __device__
-#define QDT_HD_NOINLINE __host__ __device__ __attribute__((noinline))
+#define QDT_HD_NOINLINE __host__ __device__ inline __attribute__((noinline))
 
 namespace qdt
 {

@@ -9,6 +9,7 @@
 #include "../../include/qdt.h"
 #include "../../include/qdt_gpu.h"
 #include "engine.hpp"
+#include "mesh_edit.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -89,11 +90,14 @@ struct qdt_t
     bool seams = true;
     bool cells = true;
     size_t mem_cap = 0, spare_limit = 0; // test limits (qdt_gpu_set_test_limits)
-    // constraints accepted so far (pairs of representative indices)
-    std::set<std::pair<size_t, size_t>> cons;
+    // Insertions and constraints after a batch call work on a host mesh
+    // that takes over the points, triangles and representatives (null until
+    // the first such call on a triangulation with triangles).
+    qdt::MeshEdit* ed = nullptr;
 
     ~qdt_t()
     {
+        delete ed;
         delete eng;
         if (d_in) (void)hipFree(d_in);
     }
@@ -110,7 +114,8 @@ void reset_context(qdt_t* Q) noexcept
     Q->tris_on_device_only = false;
     Q->pts_on_device_only = false;
     Q->stats = qdt_gpu_stats{};
-    Q->cons.clear();
+    delete Q->ed;
+    Q->ed = nullptr;
 }
 
 bool finite_pt(const qdt_vec2d& p) { return std::isfinite(p.x) && std::isfinite(p.y); }
@@ -335,22 +340,62 @@ qdt_status execute_host(qdt_t* Q, const qdt_vec2d* pts, size_t n)
     }
 }
 
-// Rebuild after appending points: re-runs the batch path over all points.
-// Constraints accepted earlier must survive as edges, else the call fails.
-bool has_edge(const qdt_t* Q, size_t a, size_t b)
+// Above this many triangles the half-edge twins for the host mesh are paired
+// on the device (a sort of edge keys) instead of by the host.
+constexpr size_t kDeviceTwinsMin = 1 << 16;
+
+// The host mesh for insertions and constraints, built from the batch result
+// on the first such call (GuibasStolfiDwyer_Wrapper.cpp:99-202 builds the
+// reference's).  Needs triangles.
+qdt_status ensure_editor(qdt_t* Q)
 {
-    for (size_t t = 0; t < Q->ntris; ++t)
+    if (Q->ed) return QDT_OK;
+    qdt_status st = materialize_points(Q);
+    if (st == QDT_OK) st = materialize(Q);
+    if (st != QDT_OK) return st;
+    std::vector<int32_t> twins;
+    const int32_t* tw = nullptr;
+    if (Q->ntris >= kDeviceTwinsMin && Q->eng && Q->eng->result().tris &&
+        Q->eng->result().triangles == Q->ntris)
     {
-        const uint32_t* T = Q->tris.p + 3 * t;
-        for (int k = 0; k < 3; ++k)
+        void* d = nullptr;
+        if (hipMalloc(&d, 12 * Q->ntris) == hipSuccess)
         {
-            const size_t u = T[k], v = T[(k + 1) % 3];
-            if ((u == a && v == b) || (u == b && v == a)) return true;
+            uint64_t bad = 0;
+            const qdt::Status s = qdt::mesh_twins(Q->eng->result().tris, (uint32_t)Q->ntris, (int*)d,
+                                                  Q->eng->stream(), bad);
+            if (s == qdt::Status::ok && bad == 0)
+            {
+                twins.resize(3 * Q->ntris);
+                if (hipMemcpy(twins.data(), d, 12 * Q->ntris, hipMemcpyDeviceToHost) == hipSuccess) tw = twins.data();
+            }
+            (void)hipFree(d);
         }
     }
-    return false;
+    qdt::MeshEdit* ed = new qdt::MeshEdit();
+    if (!ed->build(Q->pts.p, Q->npts, Q->rep.p, Q->tris.p, Q->ntris, tw))
+    {
+        delete ed;
+        return QDT_ERR_INTERNAL;
+    }
+    Q->ed = ed;
+    // the mesh holds its own copies from now on
+    Q->tris.reset();
+    Q->rep.reset();
+    Q->pts.reset();
+    Q->ntris = Q->nrep = 0;
+    return QDT_OK;
 }
 
+size_t cur_points(const qdt_t* Q) { return Q->ed ? Q->ed->points() : Q->npts; }
+size_t cur_triangles(const qdt_t* Q)
+{
+    if (Q->ed) return Q->ed->triangles();
+    if (Q->tris_on_device_only) return Q->eng->result().triangles;
+    return Q->ntris;
+}
+
+// qdt_insert_point / qdt_insert_point_parallel (libqdt/qdt.cpp:115-140)
 qdt_status insert_impl(qdt_t* Q, const qdt_vec2d& pt, size_t* index)
 {
     if (!Q) return QDT_ERR_NULL_CONTEXT;
@@ -364,11 +409,27 @@ qdt_status insert_impl(qdt_t* Q, const qdt_vec2d& pt, size_t* index)
             reset_context(Q);
             return st;
         }
-        if (Q->npts >= kMaxPoints) return QDT_ERR_TOO_MANY_POINTS;
+        if (cur_points(Q) >= kMaxPoints) return QDT_ERR_TOO_MANY_POINTS;
+        if (cur_triangles(Q))
+        {
+            // local update on the host mesh
+            st = ensure_editor(Q);
+            size_t idx = 0;
+            if (st == QDT_OK) st = Q->ed->insert(pt, idx);
+            if (st != QDT_OK)
+            {
+                reset_context(Q);
+                return st;
+            }
+            if (index) *index = idx;
+            return QDT_OK;
+        }
+        // No triangles yet (fewer than three distinct points, or all of them
+        // collinear): the batch path over all points, as the reference
+        // rebuilds then (GuibasStolfiDwyer_Wrapper.cpp:496-502).
         const size_t n = Q->npts + 1;
         if (!Q->pts.grow_keep(n, Q->npts)) throw std::bad_alloc();
         Q->pts.p[n - 1] = pt;
-        auto cons = Q->cons;
         st = ensure_engine(Q);
         if (st == QDT_OK) st = ensure_device_input(Q, n);
         if (st == QDT_OK &&
@@ -377,16 +438,12 @@ qdt_status insert_impl(qdt_t* Q, const qdt_vec2d& pt, size_t* index)
             st = QDT_ERR_INTERNAL;
         if (st == QDT_OK) st = run_device(Q, n);
         if (st == QDT_OK) st = materialize(Q);
-        if (st == QDT_OK)
-            for (const auto& c : cons)
-                if (!has_edge(Q, c.first, c.second)) st = QDT_ERR_INTERNAL;
         if (st != QDT_OK)
         {
             reset_context(Q);
             return st;
         }
         Q->npts = n;
-        Q->cons = cons;
         if (index) *index = n - 1;
         return QDT_OK;
     }
@@ -402,63 +459,23 @@ qdt_status insert_impl(qdt_t* Q, const qdt_vec2d& pt, size_t* index)
     }
 }
 
-// Exact segment tests for the constraint checks (integers not assumed).
-int orient_sign(const qdt_vec2d& a, const qdt_vec2d& b, const qdt_vec2d& c)
-{
-    // long double has a 64-bit mantissa on x86: not exact in general, so use
-    // the error-bounded double filter and fall back to a wider evaluation
-    const double l = (a.x - c.x) * (b.y - c.y), r = (a.y - c.y) * (b.x - c.x);
-    const double det = l - r, bound = 3.3306690738754716e-16 * (std::fabs(l) + std::fabs(r));
-    if (det > bound) return 1;
-    if (-det > bound) return -1;
-    const long double L = ((long double)a.x - c.x) * ((long double)b.y - c.y);
-    const long double R = ((long double)a.y - c.y) * ((long double)b.x - c.x);
-    return L > R ? 1 : (L < R ? -1 : 0);
-}
-
+// Triangulation::insert_constraint (GuibasStolfiDwyer_Wrapper.cpp:1246-1262):
+// checks in the order INVALID_INDEX, SAME_POINT, NO_TRIANGLES, then the
+// mesh's own (CROSSES_CONSTRAINT, THROUGH_POINT).
 qdt_status constraint_impl(qdt_t* Q, size_t v0, size_t v1)
 {
     qdt_status st = materialize_points(Q);
     if (st == QDT_OK) st = materialize(Q);
     if (st != QDT_OK) return st;
-    if (v0 >= Q->npts || v1 >= Q->npts) return QDT_ERR_INVALID_INDEX;
-    const size_t a = v0 < Q->nrep ? Q->rep.p[v0] : v0;
-    const size_t b = v1 < Q->nrep ? Q->rep.p[v1] : v1;
-    if (a == b) return QDT_ERR_SAME_POINT;
-    const qdt_vec2d& pa = Q->pts.p[a];
-    const qdt_vec2d& pb = Q->pts.p[b];
-    if (pa.x == pb.x && pa.y == pb.y) return QDT_ERR_SAME_POINT;
-    if (Q->ntris == 0) return QDT_ERR_NO_TRIANGLES;
-    const auto key = std::make_pair(std::min(a, b), std::max(a, b));
-    if (Q->cons.count(key)) return QDT_OK;
-    // crossing an accepted constraint
-    for (const auto& c : Q->cons)
-    {
-        const qdt_vec2d& p = Q->pts.p[c.first];
-        const qdt_vec2d& q = Q->pts.p[c.second];
-        if (c.first == a || c.first == b || c.second == a || c.second == b) continue;
-        const int o1 = orient_sign(pa, pb, p), o2 = orient_sign(pa, pb, q);
-        const int o3 = orient_sign(p, q, pa), o4 = orient_sign(p, q, pb);
-        if (o1 * o2 < 0 && o3 * o4 < 0) return QDT_ERR_CROSSES_CONSTRAINT;
-    }
-    if (has_edge(Q, a, b))
-    {
-        Q->cons.insert(key);
-        return QDT_OK;
-    }
-    // the segment through a third point
-    for (size_t i = 0; i < Q->npts; ++i)
-    {
-        if (i == a || i == b || (i < Q->nrep && Q->rep.p[i] != i)) continue;
-        const qdt_vec2d& p = Q->pts.p[i];
-        if (orient_sign(pa, pb, p) != 0) continue;
-        const double t = (p.x - pa.x) * (pb.x - pa.x) + (p.y - pa.y) * (pb.y - pa.y);
-        const double L = (pb.x - pa.x) * (pb.x - pa.x) + (pb.y - pa.y) * (pb.y - pa.y);
-        if (t > 0 && t < L) return QDT_ERR_THROUGH_POINT;
-    }
-    // Forcing a non-Delaunay edge (constrained triangulation) is not part of
-    // the GPU batch path of this library.
-    return QDT_ERR_INTERNAL;
+    const size_t n = cur_points(Q);
+    if (v0 >= n || v1 >= n) return QDT_ERR_INVALID_INDEX;
+    const uint32_t* rep = Q->ed ? Q->ed->rep() : Q->rep.p;
+    if (!Q->ed && Q->nrep < n) return QDT_ERR_INTERNAL;
+    if (rep[v0] == rep[v1]) return QDT_ERR_SAME_POINT;
+    if (!cur_triangles(Q)) return QDT_ERR_NO_TRIANGLES;
+    st = ensure_editor(Q);
+    if (st != QDT_OK) return st;
+    return Q->ed->constrain(v0, v1);
 }
 
 } // namespace
@@ -495,6 +512,9 @@ qdt_status qdt_insert_point(qdt_t* ptr, qdt_vec2d pt, size_t* index) { return in
 
 qdt_status qdt_insert_point_parallel(qdt_t* ptr, qdt_vec2d pt, size_t* index) { return insert_impl(ptr, pt, index); }
 
+// libqdt/qdt.cpp:193-210: an internal failure or running out of memory
+// empties the context; bad_alloc maps to QDT_ERR_OUT_OF_MEMORY, any other
+// exception to QDT_ERR_INTERNAL (qdt.cpp:63-77).
 qdt_status qdt_insert_constraint(qdt_t* ptr, size_t v0, size_t v1)
 {
     if (!ptr) return QDT_ERR_NULL_CONTEXT;
@@ -504,10 +524,15 @@ qdt_status qdt_insert_constraint(qdt_t* ptr, size_t v0, size_t v1)
         if (st == QDT_ERR_INTERNAL || st == QDT_ERR_OUT_OF_MEMORY) reset_context(ptr);
         return st;
     }
-    catch (...)
+    catch (const std::bad_alloc&)
     {
         reset_context(ptr);
         return QDT_ERR_OUT_OF_MEMORY;
+    }
+    catch (...)
+    {
+        reset_context(ptr);
+        return QDT_ERR_INTERNAL;
     }
 }
 
@@ -545,8 +570,7 @@ qdt_status qdt_insert_constraints_batch(qdt_t* ptr, const qdt_edge* edges, size_
 size_t qdt_triangles_size(const qdt_t* ptr)
 {
     if (!ptr) return 0;
-    if (ptr->tris_on_device_only) return ptr->eng->result().triangles;
-    return ptr->ntris;
+    return cur_triangles(ptr);
 }
 
 size_t qdt_get_triangle(const qdt_t* ptr, size_t T[3], size_t id)
@@ -554,30 +578,37 @@ size_t qdt_get_triangle(const qdt_t* ptr, size_t T[3], size_t id)
     if (!ptr || !T) return 0;
     qdt_t* Q = const_cast<qdt_t*>(ptr);
     if (materialize(Q) != QDT_OK) return 0;
-    if (id >= Q->ntris) return 0;
-    const uint32_t* t = Q->tris.p + 3 * id;
+    if (id >= cur_triangles(Q)) return 0;
+    const uint32_t* t = (Q->ed ? Q->ed->tris() : Q->tris.p) + 3 * id;
     T[0] = t[0];
     T[1] = t[1];
     T[2] = t[2];
     return 1;
 }
 
-size_t qdt_points_size(const qdt_t* ptr) { return ptr ? ptr->npts : 0; }
+size_t qdt_points_size(const qdt_t* ptr) { return ptr ? cur_points(ptr) : 0; }
 
 size_t qdt_get_point(const qdt_t* ptr, qdt_vec2d* pt, size_t id)
 {
     if (!ptr || !pt) return 0;
     qdt_t* Q = const_cast<qdt_t*>(ptr);
-    if (id >= Q->npts) return 0;
+    if (id >= cur_points(Q)) return 0;
+    if (Q->ed)
+    {
+        *pt = Q->ed->pts()[id];
+        return 1;
+    }
     if (materialize_points(Q) != QDT_OK) return 0;
     *pt = Q->pts.p[id];
     return 1;
 }
 
+// libqdt/qdt.cpp:285-298: no triangles -> 0; else the representatives
 size_t qdt_point_is_used(const qdt_t* ptr, size_t id)
 {
     if (!ptr) return 0;
     qdt_t* Q = const_cast<qdt_t*>(ptr);
+    if (Q->ed) return id < Q->ed->points() && Q->ed->triangles() && Q->ed->rep()[id] == id ? 1 : 0;
     if (materialize(Q) != QDT_OK) return 0;
     if (id >= Q->npts || id >= Q->nrep) return 0;
     if (Q->ntris == 0) return 0;
@@ -621,7 +652,7 @@ qdt_status qdt_gpu_execute_device(qdt_t* ptr, const qdt_vec2d* d_pts, size_t n)
 
 size_t qdt_gpu_device_triangles(const qdt_t* ptr, const uint32_t** d_tris)
 {
-    if (!ptr || !ptr->eng || (!ptr->tris_on_device_only && ptr->ntris == 0))
+    if (!ptr || !ptr->eng || ptr->ed || (!ptr->tris_on_device_only && ptr->ntris == 0))
     {
         if (d_tris) *d_tris = nullptr;
         return 0;
@@ -635,6 +666,11 @@ size_t qdt_gpu_copy_triangles(const qdt_t* ptr, uint32_t* out)
 {
     if (!ptr || !out) return 0;
     qdt_t* Q = const_cast<qdt_t*>(ptr);
+    if (Q->ed)
+    {
+        std::memcpy(out, Q->ed->tris(), 12 * Q->ed->triangles());
+        return Q->ed->triangles();
+    }
     if (Q->tris_on_device_only)
     {
         const qdt::DeviceResult& r = Q->eng->result();
@@ -769,7 +805,7 @@ qdt_status qdt_gpu_verify(qdt_t* ptr, qdt_gpu_verify_report* out)
     try
     {
         qdt::VerifyReport r;
-        if (Q->npts == 0 || !Q->eng)
+        if (cur_points(Q) == 0 || !Q->eng)
         {
             r.ok = r.euler_ok = true;
             copy_report(r, out);
@@ -785,9 +821,13 @@ qdt_status qdt_gpu_verify(qdt_t* ptr, qdt_gpu_verify_report* out)
         }
         else
         {
-            // the host store (the device workspace may have been released)
+            // the host store (the device workspace may have been released),
+            // or the host mesh after insertions and constraints
             qdt_status hs = materialize_points(Q);
             if (hs != QDT_OK) return hs;
+            const qdt_vec2d* hp = Q->ed ? Q->ed->pts() : Q->pts.p;
+            const uint32_t* ht = Q->ed ? Q->ed->tris() : Q->tris.p;
+            const uint32_t* hr = Q->ed ? Q->ed->rep() : Q->rep.p;
             void *dp = nullptr, *dt = nullptr, *dr = nullptr;
             struct Free
             {
@@ -798,13 +838,13 @@ qdt_status qdt_gpu_verify(qdt_t* ptr, qdt_gpu_verify_report* out)
                         if (*q) (void)hipFree(*q);
                 }
             } fr{ { &dp, &dt, &dr } };
-            const size_t np = Q->npts, nt = Q->ntris;
+            const size_t np = cur_points(Q), nt = cur_triangles(Q);
             if (hipMalloc(&dp, 16 * np) != hipSuccess || hipMalloc(&dt, 12 * std::max<size_t>(nt, 1)) != hipSuccess ||
                 hipMalloc(&dr, 4 * np) != hipSuccess)
                 return QDT_ERR_OUT_OF_MEMORY;
-            if (hipMemcpy(dp, Q->pts.p, 16 * np, hipMemcpyHostToDevice) != hipSuccess ||
-                (nt && hipMemcpy(dt, Q->tris.p, 12 * nt, hipMemcpyHostToDevice) != hipSuccess) ||
-                hipMemcpy(dr, Q->rep.p, 4 * np, hipMemcpyHostToDevice) != hipSuccess)
+            if (hipMemcpy(dp, hp, 16 * np, hipMemcpyHostToDevice) != hipSuccess ||
+                (nt && hipMemcpy(dt, ht, 12 * nt, hipMemcpyHostToDevice) != hipSuccess) ||
+                hipMemcpy(dr, hr, 4 * np, hipMemcpyHostToDevice) != hipSuccess)
                 return QDT_ERR_INTERNAL;
             s = qdt::verify_triangulation((const double*)dp, (uint32_t)np, (const uint32_t*)dt, (uint32_t)nt,
                                           (const uint32_t*)dr, Q->eng->stream(), r);

@@ -183,6 +183,10 @@ def ref():
         R.ref_incircle.argtypes = [d] * 8
         R.ref_copy_triangles.restype = sz
         R.ref_copy_triangles.argtypes = [vp, vp]
+        R.qdt_insert_point.restype = ctypes.c_int
+        R.qdt_insert_point.argtypes = [vp, _Vec2, ctypes.POINTER(sz)]
+        R.qdt_insert_constraint.restype = ctypes.c_int
+        R.qdt_insert_constraint.argtypes = [vp, sz, sz]
         _ref = R
     return _ref
 
@@ -208,6 +212,49 @@ def ref_triangulate(pts, parallel=False, timed=False):
         return (out, used, dt) if timed else (out, used)
     finally:
         R.qdt_destroy(q)
+
+
+class RefContext:
+    """A context of the compiled reference, for the insertion and
+    constraint parity tests: execute / insert / constrain / triangles."""
+
+    def __init__(self):
+        self.R = ref()
+        self.q = self.R.qdt_create()
+
+    def close(self):
+        if self.q:
+            self.R.qdt_destroy(self.q)
+            self.q = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def execute(self, pts):
+        a = _xy(pts)
+        return self.R.qdt_execute(self.q, a.ctypes.data if len(a) else None, len(a))
+
+    def insert(self, x, y):
+        idx = ctypes.c_size_t(0)
+        st = self.R.qdt_insert_point(self.q, _Vec2(float(x), float(y)), ctypes.byref(idx))
+        return st, idx.value
+
+    def constrain(self, a, b):
+        return self.R.qdt_insert_constraint(self.q, int(a), int(b))
+
+    def triangles(self):
+        nt = self.R.qdt_triangles_size(self.q)
+        out = np.zeros((nt, 3), np.uint32)
+        if nt:
+            self.R.ref_copy_triangles(self.q, out.ctypes.data)
+        return out
+
+    def used(self):
+        n = self.R.qdt_points_size(self.q)
+        return np.array([self.R.qdt_point_is_used(self.q, i) for i in range(n)], np.uint8)
 
 
 def ref_batch_time(pts, offsets, threads):
