@@ -2556,24 +2556,37 @@ void make_plan(Engine::Plan& P, const std::vector<uint32_t>& sizes, const std::v
     }
 }
 
+constexpr size_t kHostSetArrays = 8;
+
+} // namespace
+
 struct Item
 {
     void** p;
     size_t bytes;
 };
 
-constexpr size_t kHostSetArrays = 8;
-
+// One device region holding the items (each 256-byte aligned): a fresh
+// allocation, or the next part of the external workspace.
 template <size_t N>
-Status carve(const Item (&items)[N], char*& region, size_t& bytes, size_t budget)
+Status Engine::carve_region(const Item (&items)[N], char*& region, size_t& bytes)
 {
     size_t total = 0;
     for (const Item& it : items) total += (it.bytes + 255) & ~(size_t)255;
-    if (total > budget) return Status::out_of_memory;
-    void* ws = nullptr;
-    const hipError_t e = hipMalloc(&ws, total);
-    if (e != hipSuccess) return e == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal;
-    region = (char*)ws;
+    if (ext_)
+    {
+        if (total > ext_bytes_ - ext_used_) return Status::out_of_memory;
+        region = ext_ + ext_used_;
+        ext_used_ += total;
+    }
+    else
+    {
+        if (total > alloc_budget()) return Status::out_of_memory;
+        void* ws = nullptr;
+        const hipError_t e = hipMalloc(&ws, total);
+        if (e != hipSuccess) return e == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal;
+        region = (char*)ws;
+    }
     bytes = total;
     size_t at = 0;
     for (const Item& it : items)
@@ -2584,7 +2597,21 @@ Status carve(const Item (&items)[N], char*& region, size_t& bytes, size_t budget
     return Status::ok;
 }
 
-} // namespace
+void Engine::free_region(char*& region, size_t& bytes)
+{
+    if (region && !ext_) (void)hipFree(region);
+    region = nullptr;
+    bytes = 0;
+}
+
+Engine::Engine(hipStream_t stream, void* ext, size_t ext_bytes)
+    : stream_(stream), own_stream_(false), ext_((char*)ext), ext_bytes_(ext_bytes)
+{
+    b_ = new Bufs();
+    std::memset(b_, 0, sizeof(Bufs));
+    b_->host_scalars = ext_scalars_;
+    b_->host_table = ext_table_;
+}
 
 Engine::Engine(hipStream_t stream) : stream_(stream), own_stream_(stream == nullptr)
 {
@@ -2611,7 +2638,7 @@ Engine::~Engine()
 {
     for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
     release();
-    if (b_)
+    if (b_ && !ext_)
     {
         if (b_->host_scalars) (void)hipHostFree(b_->host_scalars);
         if (b_->host_table) (void)hipHostFree(b_->host_table);
@@ -2625,12 +2652,11 @@ Engine::~Engine()
 
 void Engine::release()
 {
-    for (char** r : { &ws_, &ws_nodes_, &ws_anchors_, &ws_sets_ })
-    {
-        if (*r) (void)hipFree(*r);
-        *r = nullptr;
-    }
-    ws_bytes_ = ws_nodes_bytes_ = ws_anchors_bytes_ = ws_sets_bytes_ = 0;
+    free_region(ws_, ws_bytes_);
+    free_region(ws_nodes_, ws_nodes_bytes_);
+    free_region(ws_anchors_, ws_anchors_bytes_);
+    free_region(ws_sets_, ws_sets_bytes_);
+    ext_used_ = ext_ws_end_ = 0;
     cap_n_ = 0;
     b_->nodes_cap = b_->offs_cap = b_->seg_nodes = b_->anchor_cap = b_->qv_nodes = 0;
     b_->sets_cap = b_->tris_cap = 0;
@@ -2692,7 +2718,15 @@ size_t Engine::alloc_budget() const
 
 Status Engine::ensure(size_t n)
 {
-    if (!stream_) return Status::internal;
+    if (!stream_ && !ext_) return Status::internal;
+    if (ext_)
+    {
+        // every run carves its regions afresh from the start of the workspace
+        release();
+        const Status st = ensure_ws(n);
+        ext_ws_end_ = ext_used_;
+        return st;
+    }
     if (!b_->host_scalars)
     {
         void* h = nullptr;
@@ -2707,9 +2741,12 @@ Status Engine::ensure(size_t n)
         QDT_HIP(hipEventCreate(&ev1_));
     }
     if (n <= cap_n_ && ws_) return Status::ok;
-    if (ws_) (void)hipFree(ws_);
-    ws_ = nullptr;
-    ws_bytes_ = 0;
+    return ensure_ws(n);
+}
+
+Status Engine::ensure_ws(size_t n)
+{
+    free_region(ws_, ws_bytes_);
     cap_n_ = 0;
     const size_t N = std::max<size_t>(n, 64);
     const uint32_t nb_sort = cdiv(N, kSortTile);
@@ -2738,7 +2775,7 @@ Status Engine::ensure(size_t n)
         { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },
         { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },
     };
-    const Status st = carve(items, ws_, ws_bytes_, alloc_budget());
+    const Status st = carve_region(items, ws_, ws_bytes_);
     if (st != Status::ok) return st;
     b.qcap = (uint32_t)qcap;
     b.rec_cap = R;
@@ -2750,9 +2787,7 @@ Status Engine::ensure_nodes(const Plan& P)
 {
     Bufs& b = *b_;
     if (ws_nodes_ && P.nodes <= b.nodes_cap && P.offs <= b.offs_cap && P.qv_nodes <= b.qv_nodes) return Status::ok;
-    if (ws_nodes_) (void)hipFree(ws_nodes_);
-    ws_nodes_ = nullptr;
-    ws_nodes_bytes_ = 0;
+    free_region(ws_nodes_, ws_nodes_bytes_);
     // grow with headroom so that similar batches reuse the allocation
     const size_t nodes = std::max(P.nodes, b.nodes_cap) + 64;
     const size_t offs = std::max(P.offs, b.offs_cap) + 64;
@@ -2762,7 +2797,7 @@ Status Engine::ensure_nodes(const Plan& P)
         { (void**)&b.kn, 4 * nodes }, { (void**)&b.lvlK, 4 * kMaxLevels }, { (void**)&b.redo, 4 * (nodes + 1) },
         { (void**)&b.qv, 4 * (size_t)kSegMaxK * qv_nodes },
     };
-    const Status st = carve(items, ws_nodes_, ws_nodes_bytes_, alloc_budget());
+    const Status st = carve_region(items, ws_nodes_, ws_nodes_bytes_);
     if (st != Status::ok) return st;
     b.nodes_cap = nodes;
     b.offs_cap = offs;
@@ -2775,9 +2810,7 @@ Status Engine::ensure_anchors(size_t seg_nodes_need, size_t anchor_need)
 {
     Bufs& b = *b_;
     if (ws_anchors_ && seg_nodes_need <= b.seg_nodes && anchor_need <= b.anchor_cap) return Status::ok;
-    if (ws_anchors_) (void)hipFree(ws_anchors_);
-    ws_anchors_ = nullptr;
-    ws_anchors_bytes_ = 0;
+    free_region(ws_anchors_, ws_anchors_bytes_);
     const size_t seg_nodes = std::max(seg_nodes_need, b.seg_nodes) + 1;
     const size_t anchor_cap = std::max(anchor_need, b.anchor_cap) + 1;
     const Item items[] = {
@@ -2792,7 +2825,7 @@ Status Engine::ensure_anchors(size_t seg_nodes_need, size_t anchor_need)
         { (void**)&b.nsel, 4 * seg_nodes },
         { (void**)&b.alist, 4 * (2 + 4 * (size_t)kCross) * anchor_cap }, { (void**)&b.alist_n, 64 },
     };
-    const Status st = carve(items, ws_anchors_, ws_anchors_bytes_, alloc_budget());
+    const Status st = carve_region(items, ws_anchors_, ws_anchors_bytes_);
     if (st != Status::ok) return st;
     b.seg_nodes = seg_nodes;
     b.anchor_cap = anchor_cap;
@@ -2814,9 +2847,7 @@ Status Engine::ensure_sets(size_t k, size_t n)
         b.host_sets_cap = k + 1;
     }
     if (ws_sets_ && k <= b.sets_cap && tris <= b.tris_cap) return Status::ok;
-    if (ws_sets_) (void)hipFree(ws_sets_);
-    ws_sets_ = nullptr;
-    ws_sets_bytes_ = 0;
+    free_region(ws_sets_, ws_sets_bytes_);
     const size_t K = std::max(k, b.sets_cap) + 1;
     const size_t T = std::max(tris, b.tris_cap);
     const Item items[] = {
@@ -2825,7 +2856,7 @@ Status Engine::ensure_sets(size_t k, size_t n)
         { (void**)&b.mstart, 4 * K }, { (void**)&b.sstart, 4 * K }, { (void**)&b.toff, 4 * K },
         { (void**)&b.trank, 4 * T },  { (void**)&b.btris, 12 * T }, { (void**)&b.tmain, 64 },
     };
-    const Status st = carve(items, ws_sets_, ws_sets_bytes_, alloc_budget());
+    const Status st = carve_region(items, ws_sets_, ws_sets_bytes_);
     if (st != Status::ok) return st;
     b.sets_cap = K - 1;
     b.tris_cap = T;
@@ -2940,7 +2971,15 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             if (e) (void)hipEventRecord(e, s);
         }
     };
-    QDT_HIP(hipEventRecord(ev0_, s));
+    if (ev0_) QDT_HIP(hipEventRecord(ev0_, s));
+    if (ext_)
+    {
+        // the node and seam regions are carved again after the points' region
+        free_region(ws_nodes_, ws_nodes_bytes_);
+        free_region(ws_anchors_, ws_anchors_bytes_);
+        b.nodes_cap = b.offs_cap = b.seg_nodes = b.anchor_cap = b.qv_nodes = 0;
+        ext_used_ = ext_ws_end_;
+    }
     mark(0);
     QDT_HIP(hipMemsetAsync(b.counters, 0, 8 * 16, s));
     QDT_HIP(hipMemsetAsync(b.err, 0, 64, s));
@@ -3081,10 +3120,15 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     const uint32_t m_forest = multi ? vstart_h[kg] : m; // distinct points in rooted sets
     if (m_forest < 3 && !multi)
     {
-        QDT_HIP(hipEventRecord(ev1_, s));
-        QDT_HIP(hipEventSynchronize(ev1_));
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+        if (ev1_)
+        {
+            QDT_HIP(hipEventRecord(ev1_, s));
+            QDT_HIP(hipEventSynchronize(ev1_));
+            (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+        }
+        else
+            QDT_HIP(hipStreamSynchronize(s));
         st_.device_ms = ms;
         return Status::ok;
     }
@@ -3404,7 +3448,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                                                         multi ? 3 * m : 0xffffffffu, multi ? b.tmain : b.partials,
                                                         b.tris);
     mark(8);
-    QDT_HIP(hipEventRecord(ev1_, s));
+    if (ev1_) QDT_HIP(hipEventRecord(ev1_, s));
     const uint32_t nbx = cdiv(nrec, kScanTile);
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbx, 4, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipMemcpyAsync(hs + 1, b.err, 4, hipMemcpyDeviceToHost, s));
@@ -3428,7 +3472,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         return Status::internal;
     }
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+    if (ev0_) (void)hipEventElapsedTime(&ms, ev0_, ev1_);
     st_.device_ms = ms;
     st_.orient = cnts[0];
     st_.orient_exact = cnts[1];

@@ -136,6 +136,12 @@ class Engine
 {
 public:
     explicit Engine(hipStream_t stream = nullptr);
+    // External workspace: every buffer is carved from [ext, ext + ext_bytes)
+    // (device memory the caller owns), host staging lives in the object, no
+    // events are recorded (device_ms stays 0) and work runs on `stream`
+    // (null: the null stream), so that a run allocates and frees no memory;
+    // a run that needs more than ext_bytes fails as out of memory.
+    Engine(hipStream_t stream, void* ext, size_t ext_bytes);
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -178,6 +184,7 @@ public:
 
 private:
     Status ensure(size_t n);
+    Status ensure_ws(size_t n);
     Status ensure_nodes(const Plan& P);
     Status ensure_anchors(size_t seg_nodes, size_t anchor_cap);
     Status ensure_sets(size_t k, size_t n);
@@ -198,6 +205,15 @@ private:
     bool cells_ = true;
     size_t mem_cap_ = 0, spare_limit_ = 0;
     size_t alloc_budget() const;
+    // external workspace (see the constructor): bump allocation, reset at
+    // the start of every run
+    char* ext_ = nullptr;
+    size_t ext_bytes_ = 0, ext_used_ = 0, ext_ws_end_ = 0;
+    uint32_t ext_scalars_[16] = {};
+    int ext_table_[4 * 64] = {};
+    template <size_t N>
+    Status carve_region(const struct Item (&items)[N], char*& region, size_t& bytes);
+    void free_region(char*& region, size_t& bytes);
     std::vector<hipEvent_t> evs_; // profiling events, created on demand
     hipEvent_t prof_event(size_t i);
     DeviceResult res_;

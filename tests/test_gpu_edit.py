@@ -92,8 +92,9 @@ def test_rejected_constraint_keeps_the_context(gpu):
         assert t.insert_constraint(0, 63) == Q.QDT_ERR_THROUGH_POINT
         assert t.insert_constraint(3, 3) == Q.QDT_ERR_SAME_POINT
         assert t.insert_constraint(0, 64) == Q.QDT_ERR_INVALID_INDEX
-        assert t.insert_constraint(1, 8) == rc.constrain(1, 8) == Q.QDT_OK
-        assert t.insert_constraint(0, 9) == rc.constrain(0, 9) == Q.QDT_ERR_CROSSES_CONSTRAINT
+        # (0, 9), a cell diagonal, was accepted above: the other diagonal crosses it
+        assert t.insert_constraint(0, 9) == rc.constrain(0, 9) == Q.QDT_OK
+        assert t.insert_constraint(1, 8) == rc.constrain(1, 8) == Q.QDT_ERR_CROSSES_CONSTRAINT
         assert t.triangles_size() == len(before)
         assert t.insert_point(3.5, 3.25) == rc.insert(3.5, 3.25)
         assert t.insert_constraint(10, 64) == rc.constrain(10, 64)
