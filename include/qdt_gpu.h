@@ -168,6 +168,27 @@ QDT_API qdt_status qdt_gpu_build_mesh(const uint32_t* d_tris, size_t T, size_t n
  * seam pieces (qdt_gpu_stats.seam_retries), with the same result. */
 QDT_API void qdt_gpu_set_test_limits(qdt_t* ptr, size_t device_bytes, size_t spare_records);
 
+/* ---- No-allocation form ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same triangulation as qdt_gpu_execute_device of n
+ * device-resident points, with every buffer of the pipeline taken from a
+ * fixed per-GPU array inside the library (112 MiB, held by each GPU the
+ * library runs on until the process ends) instead of allocated.  The call
+ * allocates and frees no device or pinned memory; it runs on the null
+ * stream and returns when the results are in place.
+ *   d_tris: room for tri_capacity >= 2 n triangles (3 uint32 each); receives
+ *           the triangles as input indices, counter-clockwise;
+ *   d_rep:  n uint32, the lowest index with the same coordinates per point;
+ *   *triangles: the triangle count.
+ * At most 262,144 points (QDT_ERR_TOO_MANY_POINTS beyond); NULL pointers:
+ * QDT_ERR_NULL_ARGUMENT; tri_capacity below 2 n: QDT_ERR_OUT_OF_MEMORY; a
+ * NaN or infinite coordinate: QDT_ERR_NOT_FINITE.  Calls on one GPU share
+ * its array: they are serialised by a lock inside the library, and two
+ * processes never share one (each loads its own copy). */
+QDT_API qdt_status qdt_gpu_execute_noalloc(const qdt_vec2d* d_pts, size_t n, uint32_t* d_tris, size_t tri_capacity,
+                                           uint32_t* d_rep, size_t* triangles);
+
 /* Number of HIP devices visible to the library (0 without a GPU). */
 QDT_API int qdt_gpu_device_count(void);
 

@@ -38,6 +38,7 @@ EXPORTS = [
 GPU_EXPORTS = [
     "qdt_gpu_execute_device", "qdt_gpu_device_triangles", "qdt_gpu_copy_triangles", "qdt_gpu_get_stats",
     "qdt_gpu_set_profiling", "qdt_gpu_set_seams", "qdt_gpu_set_cells", "qdt_gpu_set_test_limits",
+    "qdt_gpu_execute_noalloc",
     "qdt_gpu_device_count",
     "qdt_gpu_release_workspace", "qdt_gpu_verify", "qdt_gpu_verify_triangles", "qdt_gpu_build_mesh",
     "qdt_gpu_batch_create", "qdt_gpu_batch_destroy", "qdt_gpu_batch_execute", "qdt_gpu_batch_execute_device",
@@ -165,6 +166,8 @@ def lib():
         L.qdt_gpu_set_cells.argtypes = [vp, ctypes.c_int]
         L.qdt_gpu_set_test_limits.restype = None
         L.qdt_gpu_set_test_limits.argtypes = [vp, sz, sz]
+        L.qdt_gpu_execute_noalloc.restype = st
+        L.qdt_gpu_execute_noalloc.argtypes = [vp, sz, vp, sz, vp, ctypes.POINTER(sz)]
         L.qdt_gpu_verify.restype = st
         L.qdt_gpu_verify.argtypes = [vp, ctypes.POINTER(VerifyReport)]
         L.qdt_gpu_verify_triangles.restype = st
@@ -556,6 +559,15 @@ class Server:
         h, p = ctypes.c_size_t(), ctypes.c_size_t()
         lib().qdt_gpu_server_memory(self._h, ctypes.byref(h), ctypes.byref(p))
         return h.value, p.value
+
+
+def execute_noalloc(d_pts, n, d_tris, tri_capacity, d_rep):
+    """qdt_gpu_execute_noalloc on device pointers (integers, e.g. torch
+    data_ptr()): (status, triangle count)."""
+    t = ctypes.c_size_t(0)
+    st = int(lib().qdt_gpu_execute_noalloc(ctypes.c_void_p(int(d_pts)), int(n), ctypes.c_void_p(int(d_tris)),
+                                            int(tri_capacity), ctypes.c_void_p(int(d_rep)), ctypes.byref(t)))
+    return st, t.value
 
 
 def verify_triangles(d_pts, n, d_tris, T, d_rep=None):
