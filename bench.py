@@ -33,6 +33,20 @@ Extra fields:
                 12 B per triangle read, 16 B per half-edge, 8 B per local
                 vertex and 4 B per input index written.
 
+  c2_gaussian   the second 10M input of the headline configuration (Gaussian,
+                mean 0.5, sigma 0.1): GPU ms per call (median of the steps),
+                filter-failure rates, and the reference's threaded time.
+  c3            the degenerate-heavy 5.2M set (2000 x 2000 lattice step 1e-3,
+                200,000 copies, 1,000,000 jittered circle points): GPU ms,
+                filter-failure rates, cells rebuilt exactly, reference time.
+  c1            100,000 uniform points (seed 0): the reference's serial
+                qdt_execute and the GPU drop-in (host input), medians of 5.
+  readme_sizes  the reference's own benchmark program (test/qdt_bench.cpp,
+                scenario batch: 100 .. 2,624,409 uniform points in
+                [0,1000)^2, median of 5, 3 above 2M) built twice, against the
+                reference's library and against this one (oracle/_ref/
+                qdt_bench_ref, qdt_bench_amd): serial and parallel call
+                latencies side by side.
   batched       the batched form: 4,096 sets, sizes log-uniform in [64, 65536]
                 (seed 3), each uniform / Gaussian (sigma 0.1) / 32-cluster
                 mixture; one qdt_gpu_batch_execute_device call per step
@@ -45,7 +59,11 @@ Extra fields:
                 budget): p50/p99 submit-to-done latency and points/s, beside
                 the plain per-request path on the same trace (one request at
                 a time in arrival order; each request's own qdt_execute time,
-                queueing included in its latency).
+                queueing included in its latency; the plain path is one FIFO
+                server, so its latencies follow from each request's measured
+                service time and the arrival times).  CPU baseline: the
+                reference, one context per host thread, on the first
+                requests of the trace.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
 """
@@ -76,6 +94,98 @@ def traffic_per_launch():
     return None
 
 
+def host_info():
+    """CPU model, usable cores and the reference build's compiler and flags."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        import subprocess
+
+        gxx = subprocess.run(["g++", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    except Exception:
+        gxx = None
+    return {"cpu": model, "cores_usable": len(os.sched_getaffinity(0)), "compiler": gxx,
+            "flags": "-O2 -std=c++14 -DNDEBUG -pthread -ffp-contract=off -fno-fast-math (oracle/Makefile)"}
+
+
+def readme_sizes(args):
+    """The reference's own benchmark program, scenario batch, against both
+    libraries; returns rows per size."""
+    import csv
+    import subprocess
+
+    out = {}
+    for tag, exe in (("reference", "qdt_bench_ref"), ("gpu_dropin", "qdt_bench_amd")):
+        path = os.path.join(ROOT, "oracle", "_ref", exe)
+        if not os.path.exists(path):
+            return {"error": "%s not built" % exe}
+        csvp = os.path.join("/tmp", "qdt_bench_%s_%d.csv" % (tag, os.getpid()))
+        r = subprocess.run([path, "--scenario", "batch", "--repeat", "5", "--csv", csvp, "--output", os.devnull],
+                           capture_output=True, text=True, timeout=900)
+        print("readme sizes (%s): rc %d" % (tag, r.returncode), file=sys.stderr, flush=True)
+        if r.returncode != 0:
+            return {"error": "%s exited %d: %s" % (exe, r.returncode, r.stderr[-500:])}
+        for row in csv.DictReader(open(csvp)):
+            n = int(row["points"])
+            d = out.setdefault(n, {"points": n})
+            m = row["metric"]
+            if m == "triangles":
+                d.setdefault("triangles", {})[tag] = int(row["value"])
+            elif m in ("serial_ms_median", "parallel_ms_median"):
+                d["%s_%s" % (tag, m.replace("_median", ""))] = round(float(row["value"]), 4)
+        os.unlink(csvp)
+    return {"program": "test/qdt_bench.cpp --scenario batch --repeat 5 (medians; 3 above 2M points), uniform "
+                       "points in [0,1000)^2, built against each library",
+            "rows": [out[k] for k in sorted(out)]}
+
+
+def gpu_section(Q, torch, pts, steps, stats_keys=True):
+    """Median GPU ms of qdt_gpu_execute_device on device-resident pts, and
+    the last run's counters."""
+    dev = torch.from_numpy(pts).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx = Q.Triangulation()
+    assert ctx.execute_device(dev.data_ptr(), len(pts)) == Q.QDT_OK
+    ms = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        assert ctx.execute_device(dev.data_ptr(), len(pts)) == Q.QDT_OK
+        ms.append((time.perf_counter() - t0) * 1e3)
+    st = ctx.stats()
+    ver = ctx.verify()
+    ctx.close()
+    del dev
+    return {
+        "points": len(pts), "distinct": int(st["unique_points"]), "triangles": int(st["triangles"]),
+        "ms_median": round(statistics.median(ms), 3), "mpts_per_s": round(len(pts) / statistics.median(ms) / 1e3, 2),
+        "filter_failure_rate": {"orient2d": st["orient_exact"] / max(1, st["orient_calls"]),
+                                "incircle": st["incircle_exact"] / max(1, st["incircle_calls"]),
+                                "extract_exact": int(st["extract_exact"])},
+        "cells_exact": int(st["cells_exact"]), "cells": int(st["cells"]),
+        "verify_ok": bool(ver["ok"]),
+    }
+
+
+def ref_threaded(pts, reps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    if not pyoracle.ref_available():
+        return None
+    times, tri = [], None
+    for _ in range(reps):
+        dt, tri = pyoracle.ref_time(pts, parallel=True)
+        times.append(dt)
+    return {"ms_median": round(1e3 * statistics.median(times), 1), "runs": reps, "triangles": int(tri),
+            "kind": "reference", "call": "qdt_execute_parallel", "cores": min(os.cpu_count() or 1, 64)}
+
+
 def concat(sets):
     import numpy as np
 
@@ -102,7 +212,12 @@ def bench_batched(args, Q, W, np, torch):
         ts.append(time.perf_counter() - t0)
     t_batch = statistics.median(ts)
     tri_total = int(b.all_triangles().shape[0])
+    bst = b.stats()
     b.close()
+    m_b = int(bst["unique_points"])
+    # per set b_i = 2 m_i - 2 - T_i summed: E = 3 m - 3 k - b, written via T
+    E_b = 3 * m_b - 3 * len(sets) - (2 * m_b - 2 * len(sets) - tri_total)
+    alg_b = 36 * n + 40 * m_b + 48 * E_b + 12 * tri_total
     # one call per set, device-resident input, same sets
     t = Q.Triangulation()
     for i in range(8):
@@ -123,6 +238,9 @@ def bench_batched(args, Q, W, np, torch):
         "one_call_per_set": {"sets_per_s": round(len(sets) / t_single, 1),
                              "mpts_per_s": round(n / t_single / 1e6, 3), "ms": round(t_single * 1e3, 1)},
         "speedup_vs_one_call_per_set": round(t_single / t_batch, 2),
+        "device_ms": round(bst["device_ms"], 3),
+        "pipeline_alg_bytes": alg_b,
+        "pipeline_frac": alg_b / (bst["device_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS if bst["device_ms"] else None,
     }
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -146,24 +264,30 @@ def bench_batched(args, Q, W, np, torch):
 def bench_serve(args, Q, W, np):
     """The serving form against the plain per-request path (module docstring)."""
     nreq = args.serve_requests
-    sizes = W.log_uniform_sizes(nreq, 1000, 2_000_000, 5)
-    reqs = [W.mixed_set(int(sz), 5_000_011 + i)[0] for i, sz in enumerate(sizes)]
+    nuniq = min(nreq, 250)
+    sizes = W.log_uniform_sizes(nuniq, 1000, 2_000_000, 5)
+    uniq = [W.mixed_set(int(sz), 5_000_011 + i)[0] for i, sz in enumerate(sizes)]
+    rng = np.random.default_rng(17)
+    # the trace: nreq arrivals, each one of the distinct point sets
+    pick = np.concatenate([np.arange(nuniq), rng.integers(0, nuniq, size=max(0, nreq - nuniq))])
+    reqs = [uniq[i] for i in pick]
     npts = int(sum(len(r) for r in reqs))
-    # plain path: each request alone through qdt_execute (host input, host results)
+    # plain path: each distinct request alone through qdt_execute (host input,
+    # host results), its service time measured once
     t = Q.Triangulation()
-    t.execute(reqs[0])
-    dur = []
-    for r in reqs:
+    t.execute(uniq[0])
+    dur_u = []
+    for r in uniq:
         t0 = time.perf_counter()
         assert t.execute(r) == Q.QDT_OK
-        dur.append(time.perf_counter() - t0)
+        dur_u.append(time.perf_counter() - t0)
     t.close()
-    rng = np.random.default_rng(17)
+    dur = [dur_u[i] for i in pick]
     gaps = rng.exponential(1.0, size=nreq)
     rows = []
     with Q.Server(hbm_budget_bytes=200 << 30, workers=args.serve_workers) as srv:
         # warm the workers' workspaces
-        for rid in [srv.submit(reqs[i]) for i in range(min(4, nreq))]:
+        for rid in [srv.submit(reqs[i]) for i in range(min(8, nreq))]:
             srv.wait(rid)
             srv.release(rid)
         for qps in args.serve_qps:
@@ -193,24 +317,50 @@ def bench_serve(args, Q, W, np):
                 start = max(arr[i], end)
                 end = start + dur[i]
                 slat.append((end - arr[i]) * 1e3)
+            span = (last - first) * 1e-3
             rows.append({
                 "qps": qps,
+                "requests": nreq,
+                "span_s": round(span, 2),
                 "p50_ms": round(float(np.percentile(lat, 50)), 2),
                 "p99_ms": round(float(np.percentile(lat, 99)), 2),
-                "mpts_per_s": round(npts / ((last - first) * 1e-3) / 1e6, 2),
+                "mpts_per_s": round(npts / span / 1e6, 2),
                 "batches": b1[0] - b0[0],
                 "plain_p50_ms": round(float(np.percentile(slat, 50)), 2),
                 "plain_p99_ms": round(float(np.percentile(slat, 99)), 2),
                 "plain_mpts_per_s": round(float(npts / (end - arr[0]) / 1e6), 2),
+                "plain_saturated": bool(sum(dur) > arr[-1]),
             })
+            rows[-1]["vs_plain"] = round(rows[-1]["mpts_per_s"] / rows[-1]["plain_mpts_per_s"], 2)
             print("serve qps %s: %s" % (qps, rows[-1]), file=sys.stderr, flush=True)
-    return {
-        "workload": "%d requests, sizes log-uniform [1000, 2000000] (seed 5), uniform / Gaussian / 32-cluster "
-                    "mix, %d points in all (mean %d), Poisson arrivals, host input and results, 200 GB budget, "
-                    "%d workers" % (nreq, npts, npts // nreq, args.serve_workers),
+        held, peak = srv.memory()
+    out = {
+        "workload": "%d requests (%d distinct point sets, sizes log-uniform [1000, 2000000], seed 5), uniform / "
+                    "Gaussian / 32-cluster mix, %d points in all (mean %d), Poisson arrivals, host input and "
+                    "results, 200 GB budget, %d workers" % (nreq, nuniq, npts, npts // nreq, args.serve_workers),
         "plain_path_ms_per_request_mean": round(1e3 * statistics.mean(dur), 2),
+        "device_bytes_peak": int(peak),
         "by_qps": rows,
     }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        try:
+            import pyoracle
+
+            if pyoracle.ref_available():
+                k = min(args.serve_cpu_requests, nreq)
+                sample = reqs[:k]
+                pts_s, offs = concat(sample)
+                thr = min(len(os.sched_getaffinity(0)), 64)
+                dt, _ = pyoracle.ref_batch_time(pts_s, offs, thr)
+                out["cpu_baseline"] = {"mpts_per_s": round(len(pts_s) / dt / 1e6, 3), "cores": thr,
+                                       "kind": "reference",
+                                       "sample": "the first %d requests of the trace (%d points), reference "
+                                                 "qdt_execute per request, one context per host thread, %.2f s, "
+                                                 "no arrival gaps (its best case)" % (k, len(pts_s), dt)}
+        except Exception as e:
+            out["cpu_baseline"] = {"error": str(e)}
+    return out
 
 
 def main():
@@ -220,15 +370,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--dist", default="uniform", choices=["uniform", "gaussian"])
-    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-sections", action="store_true", help="skip c1 / c2_gaussian / c3")
+    ap.add_argument("--no-readme-sizes", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--batch-sets", type=int, default=4096)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=64)
-    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-cpu-requests", type=int, default=100)
     ap.add_argument("--serve-qps", type=lambda v: [float(x) for x in v.split(",")],
                     default=[100, 250, 500, 1000, 2500, 5000])
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -346,6 +499,7 @@ def main():
             "seam_levels_ms": round(stats["seam_level_ms_sum"], 3),
             "pipeline_device_ms": round(stats["device_ms"], 3),
             "pipeline_alg_bytes": 36 * n + 40 * m + 48 * E + 12 * T,
+            "pipeline_frac": (36 * n + 40 * m + 48 * E + 12 * T) / (stats["device_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "verify": {"check": "qdt_gpu_verify on the last step's result (exact orientation, manifold edges, "
                             "local Delaunay, convex boundary, Euler, every distinct point used)",
@@ -384,11 +538,14 @@ def main():
     if not args.no_dropin:
         st = ctx.execute(pts)  # warm the host path (pinned buffers)
         assert st == Q.QDT_OK
-        ts = time.perf_counter()
-        st = ctx.execute(pts)
-        dt = time.perf_counter() - ts
-        assert st == Q.QDT_OK
-        line["dropin"] = {"call": "qdt_execute (host input, host results)", "ms": round(dt * 1e3, 3),
+        ds = []
+        for _ in range(3):
+            ts = time.perf_counter()
+            st = ctx.execute(pts)
+            ds.append(time.perf_counter() - ts)
+            assert st == Q.QDT_OK
+        dt = statistics.median(ds)
+        line["dropin"] = {"call": "qdt_execute (host input, host results), median of 3", "ms": round(dt * 1e3, 3),
                           "value": round(n / dt / 1e6, 3), "unit": "Mpts/s"}
     ctx.close()
 
@@ -414,11 +571,56 @@ def main():
                               "g++ -O2 from the reference sources), median of %d runs, %.2f s each; "
                               "triangles %d" % (n, args.cpu_reps, med, tri_ref),
                     "nproc": len(os.sched_getaffinity(0)),
+                    "host": host_info(),
                 }
+                # the serial path on the same input: one run (about 10 s)
+                dt_s, tri_s = pyoracle.ref_time(pts, parallel=False)
+                line["cpu_baseline"]["serial"] = {"value": round(n / dt_s / 1e6, 3), "unit": "Mpts/s",
+                                                  "ms": round(dt_s * 1e3, 1), "runs": 1, "call": "qdt_execute",
+                                                  "cores": 1, "triangles": int(tri_s)}
             else:
                 line["cpu_baseline"] = None
         except Exception as e:  # the baseline is a report, never the measurement
             line["cpu_baseline"] = {"error": str(e)}
+
+    if not args.no_sections:
+        print("c2 gaussian / c3 / c1 ...", file=sys.stderr, flush=True)
+        g = W.gaussian(n, 1)
+        line["c2_gaussian"] = gpu_section(Q, torch, g, max(args.steps, 3))
+        if not args.no_cpu_baseline:
+            line["c2_gaussian"]["cpu_baseline"] = ref_threaded(g, 1)
+        del g
+        c3 = W.lattice_dups_circle()
+        line["c3"] = gpu_section(Q, torch, c3, max(args.steps, 3))
+        line["c3"]["input"] = "2000 x 2000 lattice (step 1e-3) + 200,000 copies + 1,000,000 circle points (jitter 1e-12)"
+        if not args.no_cpu_baseline:
+            line["c3"]["cpu_baseline"] = ref_threaded(c3, 1)
+        del c3
+        c1 = W.uniform(100_000, 0)
+        t1 = Q.Triangulation()
+        t1.execute(c1)
+        g1 = []
+        for _ in range(5):
+            ts = time.perf_counter()
+            assert t1.execute(c1) == Q.QDT_OK
+            g1.append((time.perf_counter() - ts) * 1e3)
+        line["c1"] = {"points": 100_000, "triangles": t1.triangles_size(),
+                      "gpu_dropin_ms_median": round(statistics.median(g1), 3)}
+        t1.close()
+        if not args.no_cpu_baseline:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle
+
+            if pyoracle.ref_available():
+                pyoracle.ref_time(c1, parallel=False)
+                r1 = [pyoracle.ref_time(c1, parallel=False)[0] * 1e3 for _ in range(5)]
+                line["c1"]["reference_serial_ms_median"] = round(statistics.median(r1), 3)
+    if not args.no_readme_sizes:
+        print("readme sizes (the reference's bench program, both libraries) ...", file=sys.stderr, flush=True)
+        try:
+            line["readme_sizes"] = readme_sizes(args)
+        except Exception as e:  # a report, never the measurement
+            line["readme_sizes"] = {"error": str(e)}
 
     if not args.no_batched:
         print("batched form ...", file=sys.stderr, flush=True)

@@ -50,6 +50,15 @@ typedef struct qdt_gpu_stats
     uint64_t cells_exact;         /* LDS cells rebuilt with the exact predicates: the first pass
                                      runs the floating-point filters alone and lists every cell
                                      where one could not decide */
+    /* batched form: */
+    uint64_t graph_replay;        /* 1: the call was one launch of the shape's captured HIP graph */
+    uint64_t graph_captured;      /* 1: the call was the shape's first (run, then captured) */
+    uint64_t graph_fallback;      /* 1: the graph's guards failed (the data needed other launches
+                                     than the recorded run): the call ran without the graph */
+    uint64_t cell_points;         /* largest kd node built in one LDS cell (128, 256 or 512) */
+    uint64_t merge_lanes;         /* merges per wavefront of the one-thread-per-node merges */
+    uint64_t tune_source;         /* 0 default, 1 timed on this call, 2 tuning cache, 3 environment */
+    double tune_ms;               /* the chosen pair's device time when it was timed */
 } qdt_gpu_stats;
 
 /* Like qdt_execute(), but d_pts points to n points in device memory.  The
@@ -243,6 +252,20 @@ QDT_API size_t qdt_gpu_batch_copy_rep(const qdt_gpu_batch* b, uint32_t* out);
 QDT_API int qdt_gpu_batch_get_stats(const qdt_gpu_batch* b, qdt_gpu_stats* out);
 QDT_API void qdt_gpu_batch_set_profiling(qdt_gpu_batch* b, int enable);
 QDT_API void qdt_gpu_batch_set_seams(qdt_gpu_batch* b, int enable);
+/* HIP graphs (default 1): the first batch of a shape (the sets' sizes) runs
+ * as usual while every value the pipeline sizes its launches by is recorded;
+ * the same pipeline is then captured in a graph, and later batches of that
+ * shape are one launch of it.  Device-side guards compare the recorded
+ * values with the new batch's (distinct counts, digit ranges of the sorts,
+ * triangle counts, ...); a batch that differs runs without the graph.
+ * Results are identical either way. */
+QDT_API void qdt_gpu_batch_set_graphs(qdt_gpu_batch* b, int enable);
+/* Launch parameters: LDS cell size in points (128, 256, 512) and merges per
+ * wavefront (16, 32, 64); 0 (the default) picks both by timing every pair on
+ * the first batch of each shape bucket (cached in memory and in the file
+ * $QDT_TUNE_FILE, default ~/.cache/qdt_amd/tune.txt; QDT_CELL_POINTS and
+ * QDT_MERGE_LANES override).  Results are identical for every choice. */
+QDT_API void qdt_gpu_batch_set_tuning(qdt_gpu_batch* b, int cell_points, int merge_lanes);
 
 /* ---- Serving form: a request loop with continuous batching ----
  *

@@ -46,6 +46,7 @@ GPU_EXPORTS = [
     "qdt_gpu_batch_unique_points", "qdt_gpu_batch_get_triangle", "qdt_gpu_batch_set_range",
     "qdt_gpu_batch_copy_triangles", "qdt_gpu_batch_device_triangles", "qdt_gpu_batch_copy_rep",
     "qdt_gpu_batch_get_stats", "qdt_gpu_batch_set_profiling", "qdt_gpu_batch_set_seams",
+    "qdt_gpu_batch_set_graphs", "qdt_gpu_batch_set_tuning",
     "qdt_gpu_server_create", "qdt_gpu_server_destroy", "qdt_gpu_server_submit", "qdt_gpu_server_poll",
     "qdt_gpu_server_wait", "qdt_gpu_server_times", "qdt_gpu_server_triangles_size",
     "qdt_gpu_server_unique_points", "qdt_gpu_server_copy_triangles", "qdt_gpu_server_release",
@@ -72,7 +73,9 @@ class GpuStats(ctypes.Structure):
         ("seam_level_ms_sum", ctypes.c_double), ("seam_retries", ctypes.c_uint64),
         ("seam_counters", ctypes.c_uint64 * 8), ("err_bits", ctypes.c_uint64), ("cells", ctypes.c_uint64),
         ("cell_level", ctypes.c_uint64), ("spare_records", ctypes.c_uint64),
-        ("cells_exact", ctypes.c_uint64),
+        ("cells_exact", ctypes.c_uint64), ("graph_replay", ctypes.c_uint64), ("graph_captured", ctypes.c_uint64),
+        ("graph_fallback", ctypes.c_uint64), ("cell_points", ctypes.c_uint64), ("merge_lanes", ctypes.c_uint64),
+        ("tune_source", ctypes.c_uint64), ("tune_ms", ctypes.c_double),
     ]
     STAGES = ("keys_sort_y", "sort_x", "dedupe", "frame1_sort", "kd_levels", "leaves", "merges", "extract")
 
@@ -204,7 +207,9 @@ def lib():
         L.qdt_gpu_batch_device_triangles.argtypes = [vp, ctypes.POINTER(vp)]
         L.qdt_gpu_batch_get_stats.restype = ctypes.c_int
         L.qdt_gpu_batch_get_stats.argtypes = [vp, ctypes.POINTER(GpuStats)]
-        for f in ("qdt_gpu_batch_set_profiling", "qdt_gpu_batch_set_seams"):
+        L.qdt_gpu_batch_set_tuning.restype = None
+        L.qdt_gpu_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        for f in ("qdt_gpu_batch_set_profiling", "qdt_gpu_batch_set_seams", "qdt_gpu_batch_set_graphs"):
             getattr(L, f).restype = None
             getattr(L, f).argtypes = [vp, ctypes.c_int]
         # serving form
@@ -487,6 +492,13 @@ class Batch:
 
     def set_seams(self, on=True):
         lib().qdt_gpu_batch_set_seams(self._h, 1 if on else 0)
+
+    def set_graphs(self, on=True):
+        lib().qdt_gpu_batch_set_graphs(self._h, 1 if on else 0)
+
+    def set_tuning(self, cell_points=0, merge_lanes=0):
+        """0, 0: autotuned per shape bucket."""
+        lib().qdt_gpu_batch_set_tuning(self._h, int(cell_points), int(merge_lanes))
 
 
 class Server:

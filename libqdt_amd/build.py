@@ -14,8 +14,8 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib", "libqdt_amd.so")
 
-SOURCES = ["engine.hip", "qdt_capi.cpp", "qdt_serve.cpp", "mesh_edit.cpp", "noalloc.hip"]
-HEADERS = ["engine.hpp", "mesh_edit.hpp", "predicates.hpp", "primitives.hpp", "quadedge.hpp", "seam.hpp"]
+SOURCES = ["engine.hip", "qdt_capi.cpp", "qdt_serve.cpp", "mesh_edit.cpp", "tuner.cpp", "noalloc.hip"]
+HEADERS = ["engine.hpp", "mesh_edit.hpp", "predicates.hpp", "primitives.hpp", "quadedge.hpp", "seam.hpp", "tuner.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -46,6 +46,13 @@ constexpr int kMergeBlock = 64;
         if (e_ != hipSuccess) return e_ == hipErrorOutOfMemory ? Status::out_of_memory : Status::internal; \
     } while (0)
 
+#define QDT_TRY(x)                                                                                       \
+    do                                                                                                   \
+    {                                                                                                    \
+        const Status st_try_ = (x);                                                                      \
+        if (st_try_ != Status::ok) return st_try_;                                                       \
+    } while (0)
+
 inline uint32_t cdiv(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
 
 __device__ __forceinline__ uint64_t order_key(double v)
@@ -328,20 +335,23 @@ __global__ __launch_bounds__(kBlock) void k_vstart(const uint32_t* dist, const u
     vstart[r] = p < n ? dist[p] : *m_total;
 }
 
-// rank of every triangle (by its first vertex's input index)
-__global__ __launch_bounds__(kBlock) void k_tri_rank(const uint32_t* tris, uint32_t T, const uint32_t* rin,
+// rank of every triangle (by its first vertex's input index); T on the device
+__global__ __launch_bounds__(kBlock) void k_tri_rank(const uint32_t* tris, const uint32_t* dT, const uint32_t* rin,
                                                      uint32_t* trank)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < T) trank[i] = rin[tris[3 * i]];
+    if (i < *dT) trank[i] = rin[tris[3 * i]];
 }
 
-// lower bounds of ranks 0..nr in a rank-sorted key array
+// lower bounds of ranks 0..nr in a rank-sorted key array of min(*dlen, len)
+// keys (dlen may be null)
 template <class K>
-__global__ __launch_bounds__(kBlock) void k_rank_bounds(const K* keys, uint32_t len, int nr, uint32_t* out)
+__global__ __launch_bounds__(kBlock) void k_rank_bounds(const K* keys, const uint32_t* dlen, uint32_t len, int nr,
+                                                        uint32_t* out)
 {
     const int r = blockIdx.x * kBlock + threadIdx.x;
     if (r > nr) return;
+    if (dlen && *dlen < len) len = *dlen;
     uint32_t lo = 0, hi = len;
     while (lo < hi)
     {
@@ -357,13 +367,14 @@ __global__ __launch_bounds__(kBlock) void k_rank_bounds(const K* keys, uint32_t 
 // triangles grouped by rank, with set-local indices.  Records below 3m gave
 // their triangles in rank order already (records of a set lie in its points'
 // record range); the spare records' triangles were sorted by rank.
-__global__ __launch_bounds__(kBlock) void k_batch_tris(const uint32_t* tris, uint32_t T, uint32_t tmain,
-                                                       const uint32_t* trank, const uint64_t* skey,
-                                                       const uint32_t* sidx, const uint32_t* mstart,
-                                                       const uint32_t* sstart, const uint32_t* toff,
-                                                       const uint32_t* in_off, uint32_t* out)
+__global__ __launch_bounds__(kBlock) void k_batch_tris(const uint32_t* tris, const uint32_t* dT,
+                                                       const uint32_t* dtmain, const uint32_t* trank,
+                                                       const uint64_t* skey, const uint32_t* sidx,
+                                                       const uint32_t* mstart, const uint32_t* sstart,
+                                                       const uint32_t* toff, const uint32_t* in_off, uint32_t* out)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t T = *dT, tmain = min(*dtmain, T);
     if (i >= T) return;
     uint32_t src, dst, r;
     if (i < tmain)
@@ -385,13 +396,71 @@ __global__ __launch_bounds__(kBlock) void k_batch_tris(const uint32_t* tris, uin
     out[3 * dst + 2] = tris[3 * src + 2] - base;
 }
 
-__global__ __launch_bounds__(kBlock) void k_spare_keys(const uint32_t* trank, uint32_t c, uint64_t* key,
-                                                       uint32_t* idx)
+// keys of the triangles of spare records (the ones after the first *dtmain):
+// their ranks, padded to cb keys with the rank `pad` (past every set)
+__global__ __launch_bounds__(kBlock) void k_spare_keys(const uint32_t* trank, const uint32_t* dT,
+                                                       const uint32_t* dtmain, uint32_t cb, uint32_t pad,
+                                                       uint64_t* key, uint32_t* idx)
 {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= c) return;
-    key[j] = trank[j];
+    if (j >= cb) return;
+    const uint32_t T = *dT, tmain = min(*dtmain, T);
+    key[j] = j < T - tmain ? trank[tmain + j] : pad;
     idx[j] = j;
+}
+
+// first output triangle of every rank: the triangles of its records below
+// the spare region and those of spare records (one block)
+__global__ __launch_bounds__(1024) void k_tri_offsets(const uint32_t* mstart, const uint32_t* sstart, int nr,
+                                                      uint32_t* toff)
+{
+    __shared__ uint32_t carry;
+    __shared__ uint32_t wsum[16];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nr; base += 1024)
+    {
+        const int r = base + (int)threadIdx.x;
+        const uint32_t v = r < nr ? (mstart[r + 1] - mstart[r]) + (sstart[r + 1] - sstart[r]) : 0;
+        const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+        const uint32_t inc = wave_inclusive_scan(v);
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int w = 0; w < 16; ++w)
+        {
+            if ((uint32_t)w < wave) before += wsum[w];
+            total += wsum[w];
+        }
+        const uint32_t c = carry;
+        if (r < nr) toff[r] = c + before + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) toff[nr] = carry;
+}
+
+// the scalars a run ends with, in one block for one read-back:
+// 0 triangles, 1 err, 2 spare demand, 3 cells redone, 4.. the 16 counters
+__global__ void k_result_block(const uint32_t* dT, const int* err, const unsigned* spare, const int* redo,
+                               const unsigned long long* counters, uint32_t* out)
+{
+    if (threadIdx.x != 0) return;
+    out[0] = *dT;
+    out[1] = (uint32_t)*err;
+    out[2] = *spare;
+    out[3] = redo ? (uint32_t)*redo : 0u;
+    unsigned long long* c = reinterpret_cast<unsigned long long*>(out + 4);
+    for (int i = 0; i < 16; ++i) c[i] = counters[i];
+}
+
+__global__ void k_guard_spare(const uint32_t* dT, const uint32_t* dtmain, uint32_t cb, unsigned long long* guard,
+                              unsigned long long bit)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t T = *dT, tmain = min(*dtmain, T);
+    if (T - tmain > cb) atomicOr(guard, bit);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rep_local(uint32_t* rep, uint32_t n, const uint32_t* soff, int k)
@@ -429,6 +498,20 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* a, uint32_t n, ui
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) a[i] = v;
+}
+
+// Clears device memory with a kernel: a run can then be captured in a HIP
+// graph and replayed (memset nodes are avoided in captured graphs).
+__global__ __launch_bounds__(kBlock) void k_fill_words(uint32_t* a, size_t n, uint32_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) a[i] = v;
+}
+
+inline Status fill_words(void* p, size_t words, uint32_t v, hipStream_t s)
+{
+    if (!words) return Status::ok;
+    k_fill_words<<<std::min<size_t>(cdiv(words, kBlock), 2048), kBlock, 0, s>>>((uint32_t*)p, words, v);
+    return Status::ok;
 }
 
 // ---------------------------------------------------------------------------
@@ -641,9 +724,9 @@ __global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const 
 
 // One merge per lane on the first kMergeLanes lanes of each wavefront: the
 // merges are diverging chains of dependent loads, and a level has only a few
-// thousand of them, so fewer per wavefront spread them over more SIMDs.
-constexpr uint32_t kMergeLanes = 16;
-
+// thousand of them, so fewer per wavefront spread them over more SIMDs (16,
+// 32 or 64 per wavefront; tuned per workload).
+template <uint32_t kMergeLanes>
 __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const double2* PT, int* VE, const int* off,
                                                        int d, uint32_t m, int lim_e, int4* res, int* err,
                                                        unsigned long long* counters)
@@ -682,9 +765,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
 // cell is the one the level kernels would have built.
 // ---------------------------------------------------------------------------
 
-constexpr int kCellMax = 256;             // points per cell at most
+constexpr int kCellMax = 512;             // points per cell at most (the tuned size is 128, 256 or 512)
 constexpr int kCellLevels = 12;
-constexpr int kCellPer = (kCellMax + 63) / 64; // positions per lane in the cell's scans
 
 // The fast pass runs the filters alone: no exact stage, no calls, fewer
 // registers; a cell where a filter cannot decide is listed and rebuilt by
@@ -760,7 +842,8 @@ struct CellLds
 
 // One cell: returns false (and writes nothing) when Q lacks the exact stage
 // and some filter could not decide.
-template <class Q>
+// kCellPer: positions per lane in the cell's scans (cap <= 64 kCellPer)
+template <class Q, int kCellPer>
 __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, int* OR, const double2* PT, int* VE,
                            const uint32_t* B0, const uint32_t* B1, const uint32_t* newid, const int* off, int dc,
                            int D, int4* res, int* err, unsigned long long* counters)
@@ -1058,13 +1141,15 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
 }
 
 // every cell, filters only; undecided cells go to redo (count, then ids)
+template <int kCellPer>
 __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
                                              const uint32_t* B1, const uint32_t* newid, const int* off, int dc, int D,
                                              int cap, int4* res,
                                              int* redo, int* err, unsigned long long* counters)
 {
     extern __shared__ __align__(16) unsigned char cell_lds[];
-    if (!cell_build<LQE>(cell_lds, cap, blockIdx.x, NX, OR, PT, VE, B0, B1, newid, off, dc, D, res, err, counters) &&
+    if (!cell_build<LQE, kCellPer>(cell_lds, cap, blockIdx.x, NX, OR, PT, VE, B0, B1, newid, off, dc, D, res, err,
+                                   counters) &&
         threadIdx.x == 0)
     {
         const int k = atomicAdd(redo, 1);
@@ -1073,6 +1158,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
 }
 
 // the listed cells again, with the exact predicates
+template <int kCellPer>
 __global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
                                                    const uint32_t* B1, const uint32_t* newid, const int* off, int dc,
                                                    int D, int cap,
@@ -1083,8 +1169,8 @@ __global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const doubl
     const int n = redo[0];
     for (int i = blockIdx.x; i < n; i += gridDim.x)
     {
-        cell_build<LQX>(cell_lds, cap, (uint32_t)redo[1 + i], NX, OR, PT, VE, B0, B1, newid, off, dc, D, res, err,
-                        counters);
+        cell_build<LQX, kCellPer>(cell_lds, cap, (uint32_t)redo[1 + i], NX, OR, PT, VE, B0, B1, newid, off, dc, D,
+                                  res, err, counters);
         __syncthreads();
     }
 }
@@ -2460,7 +2546,7 @@ struct Engine::Bufs
     // point-proportional (ensure)
     uint64_t *kA, *kB, *kx, *ky;
     uint32_t *vA, *vB, *dist, *ID0, *rep, *B0, *B1, *Bt, *nodeA, *nodeB, *Z, *tris, *face, *queue;
-    uint32_t *partials, *hist, *ghist, *vrank, *rin, *one_set;
+    uint32_t *partials, *hist, *ghist, *vrank, *rin, *one_set, *rblock;
     double2* P0;
     double2* P1; // the other order's coordinates (vertex ids renumbered for the cells)
     uint8_t* flagp;
@@ -2503,6 +2589,51 @@ struct Engine::Plan
     std::vector<int> K;       // anchors per node at the level (0: not cut into pieces)
     size_t nodes = 0, offs = 0, seg_nodes = 1, anchor_cap = 1;
     size_t qv_nodes = 1; // node index bound of the levels cut into pieces (qv is indexed by node)
+};
+
+// The values a batch run took host decisions on (recorded by a run in
+// record mode, replayed by the captured graph) and the pinned memory its
+// graph copies from.
+struct Engine::Trace
+{
+    std::vector<uint32_t> sort_mask; // per sort of the run, in order: the radix passes run
+    std::vector<uint32_t> setbad;    // per set: a non-finite point
+    uint32_t m = 0;                  // distinct points
+    std::vector<uint32_t> vstart;    // first distinct point of every rank (k + 1)
+    bool ytie = false;               // a run of equal y too long for the in-place fix
+    std::vector<int> K;              // anchors per node of every kd level after the kd phase
+    uint32_t T = 0, tmain = 0;       // triangles of the recorded run, and those below the spare records
+    uint32_t T_bound = 0, c_bound = 0; // replay bounds on the triangles and the spare records' triangles
+    std::vector<int> set_rank;       // rank of every set in the kd forest
+    int kg = 0;                      // sets in the forest
+    size_t upload_bytes = 0;         // host-to-device bytes of a run (staging size)
+    char* stage = nullptr;           // pinned staging of a replay's uploads (owned)
+    size_t stage_cap = 0, stage_used = 0;
+    ~Trace()
+    {
+        if (stage) (void)hipHostFree(stage);
+    }
+};
+
+struct Engine::GraphEntry
+{
+    std::vector<size_t> sizes; // the shape: points per set
+    bool seams = true;
+    int cell_max = 0, merge_lanes = 0;
+    uint64_t ws_gen = 0;
+    Trace tr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t in_node = nullptr; // the copy of the caller's points (its source is set per launch)
+    const double* in_src = nullptr;
+    uint32_t* rb = nullptr;           // pinned: the result block and the ranks' triangle offsets
+    uint64_t uses = 0;
+    ~GraphEntry()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rb) (void)hipHostFree(rb);
+    }
 };
 
 namespace
@@ -2573,6 +2704,8 @@ Status Engine::carve_region(const Item (&items)[N], char*& region, size_t& bytes
 {
     size_t total = 0;
     for (const Item& it : items) total += (it.bytes + 255) & ~(size_t)255;
+    if (mode_ == Mode::replay) return Status::internal; // a replay reuses the recorded run's regions
+    ++ws_gen_;
     if (ext_)
     {
         if (total > ext_bytes_ - ext_used_) return Status::out_of_memory;
@@ -2656,6 +2789,10 @@ void Engine::release()
     free_region(ws_nodes_, ws_nodes_bytes_);
     free_region(ws_anchors_, ws_anchors_bytes_);
     free_region(ws_sets_, ws_sets_bytes_);
+    free_region(ws_gin_, ws_gin_bytes_);
+    gin_ = nullptr;
+    gin_cap_ = 0;
+    drop_graphs(); // they hold the regions' addresses
     ext_used_ = ext_ws_end_ = 0;
     cap_n_ = 0;
     b_->nodes_cap = b_->offs_cap = b_->seg_nodes = b_->anchor_cap = b_->qv_nodes = 0;
@@ -2664,7 +2801,10 @@ void Engine::release()
     batch_ = BatchResult();
 }
 
-size_t Engine::workspace_bytes() const { return ws_bytes_ + ws_nodes_bytes_ + ws_anchors_bytes_ + ws_sets_bytes_; }
+size_t Engine::workspace_bytes() const
+{
+    return ws_bytes_ + ws_nodes_bytes_ + ws_anchors_bytes_ + ws_sets_bytes_ + ws_gin_bytes_;
+}
 
 namespace
 {
@@ -2688,7 +2828,7 @@ size_t Engine::bytes_after(size_t n, size_t k) const
         const size_t per[] = { 8 * N, 8 * N, 8 * N, 8 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N,
                                4 * N, 4 * N, 4 * N, 4 * N, 4 * 6 * N, 4 * R, 4 * qcap,
                                4 * (nb_scan + 1 + 256 * nb_sort), 4 * 256 * nb_sort, 4 * 8 * 256, 4 * N, 4 * N,
-                               16 * N, 16 * N, N, 4 * 4 * R, 4 * 2 * R, 8 * 16, 64, 4 * N, 4 * N, 64, 64 };
+                               16 * N, 16 * N, N, 4 * 4 * R, 4 * 2 * R, 8 * 16, 64, 4 * N, 4 * N, 64, 64, 256 };
         ws = 0;
         for (size_t b : per) ws += round256(b);
     }
@@ -2714,6 +2854,154 @@ size_t Engine::alloc_budget() const
     if (!mem_cap_) return SIZE_MAX;
     const size_t held = workspace_bytes();
     return mem_cap_ > held ? mem_cap_ - held : 0;
+}
+
+// The radix passes of a sort (bit p of mask: pass p runs); the result ends
+// in kA/vA.
+static void sort_passes(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t mask,
+                        uint32_t* hist, uint32_t* partials, hipStream_t s)
+{
+    const uint32_t nb = cdiv(n, kSortTile);
+    for (int pass = 0; pass < 8; ++pass)
+    {
+        if (!(mask >> pass & 1u)) continue;
+        const int shift = 8 * pass;
+        k_radix_hist<<<nb, kBlock, 0, s>>>(kA, n, shift, hist);
+        scan_u32(LoadU32{ hist }, 256 * nb, StoreU32{ hist }, partials, s);
+        k_radix_scatter<<<nb, kBlock, 0, s>>>(kA, vA, kB, vB, n, shift, hist);
+        std::swap(kA, kB);
+        std::swap(vA, vB);
+    }
+}
+
+// the passes whose digit is not the same for all n keys
+static uint32_t pass_mask(const uint32_t* host_ghist, uint32_t n)
+{
+    uint32_t mask = 0;
+    for (int pass = 0; pass < 8; ++pass)
+    {
+        bool trivial = false;
+        for (int dgt = 0; dgt < 256; ++dgt)
+            if (host_ghist[pass * 256 + dgt] == n) trivial = true;
+        if (!trivial) mask |= 1u << pass;
+    }
+    return mask;
+}
+
+// guard of a replayed sort: every pass the recorded mask skips must still
+// have one digit for all n keys
+__global__ __launch_bounds__(kBlock) void k_guard_sort(const uint32_t* ghist, uint32_t n, uint32_t mask,
+                                                       unsigned long long* guard)
+{
+    __shared__ int trivial[8];
+    if (threadIdx.x < 8) trivial[threadIdx.x] = 0;
+    __syncthreads();
+    for (int pass = 0; pass < 8; ++pass)
+        if (ghist[pass * 256 + threadIdx.x] == n) trivial[pass] = 1;
+    __syncthreads();
+    if (threadIdx.x < 8 && !(mask >> threadIdx.x & 1u) && !trivial[threadIdx.x]) atomicOr(guard, 1ull);
+}
+
+// guards of a replay: a device value against the recorded one
+__global__ __launch_bounds__(kBlock) void k_guard_words(const uint32_t* got, const uint32_t* want, uint32_t n,
+                                                        unsigned long long* guard, unsigned long long bit)
+{
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        if (got[i] != want[i]) atomicOr(guard, bit);
+}
+
+__global__ void k_guard_scalar(const uint32_t* got, uint32_t want, unsigned long long* guard, unsigned long long bit)
+{
+    if (threadIdx.x == 0 && *got != want) atomicOr(guard, bit);
+}
+
+__global__ void k_guard_le(const uint32_t* got, uint32_t bound, unsigned long long* guard, unsigned long long bit)
+{
+    if (threadIdx.x == 0 && *got > bound) atomicOr(guard, bit);
+}
+
+__global__ void k_guard_flag(const unsigned long long* got, int want_nonzero, unsigned long long* guard,
+                             unsigned long long bit)
+{
+    if (threadIdx.x == 0 && (*got != 0) != (want_nonzero != 0)) atomicOr(guard, bit);
+}
+
+// Sorts (k, v) pairs stably by key; the result ends in kA/vA.
+static Status sort_pairs(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t* hist,
+                         uint32_t* partials, uint32_t* ghist, uint32_t* host_ghist, hipStream_t s)
+{
+    if (n <= 1) return Status::ok;
+    fill_words(ghist, 8 * 256, 0, s);
+    k_radix_global_hist<<<std::min<uint32_t>(cdiv(n, kBlock), 1024), kBlock, 0, s>>>(kA, n, ghist);
+    QDT_HIP(hipMemcpyAsync(host_ghist, ghist, 4 * 8 * 256, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    sort_passes(kA, vA, kB, vB, n, pass_mask(host_ghist, n), hist, partials, s);
+    return Status::ok;
+}
+
+void Engine::set_tuning(int cell_points, int merge_lanes)
+{
+    const int c = cell_points <= 128 ? 128 : (cell_points <= 256 ? 256 : 512);
+    const int l = merge_lanes <= 16 ? 16 : (merge_lanes <= 32 ? 32 : 64);
+    if (c != cell_max_ || l != merge_lanes_) drop_graphs();
+    cell_max_ = c;
+    merge_lanes_ = l;
+}
+
+void Engine::drop_graphs()
+{
+    for (GraphEntry* g : graphs_cache_) delete g;
+    graphs_cache_.clear();
+}
+
+// A host-to-device copy of data the run fixed on the host.  In replay mode
+// (under capture) the bytes are kept in the graph's pinned staging, which
+// the captured copy reads at every launch.
+Status Engine::upload(void* dst, const void* src, size_t bytes)
+{
+    if (!bytes) return Status::ok;
+    if (mode_ == Mode::replay)
+    {
+        Trace& t = *tr_;
+        const size_t at = (t.stage_used + 255) & ~(size_t)255;
+        if (at + bytes > t.stage_cap) return Status::internal;
+        std::memcpy(t.stage + at, src, bytes);
+        t.stage_used = at + bytes;
+        QDT_HIP(hipMemcpyAsync(dst, t.stage + at, bytes, hipMemcpyHostToDevice, stream_));
+        return Status::ok;
+    }
+    if (mode_ == Mode::record) tr_->upload_bytes += ((bytes + 255) & ~(size_t)255) + 256;
+    QDT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+    return Status::ok;
+}
+
+// Sorts (k, v) pairs stably by key (result in kA/vA), skipping the radix
+// passes whose digit is the same for every key: read back in eager and
+// record mode (and recorded), taken from the trace in replay mode with a
+// device guard on the global histogram.
+Status Engine::sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n)
+{
+    if (n <= 1) return Status::ok;
+    Bufs& b = *b_;
+    hipStream_t s = stream_;
+    fill_words(b.ghist, 8 * 256, 0, s);
+    k_radix_global_hist<<<std::min<uint32_t>(cdiv(n, kBlock), 1024), kBlock, 0, s>>>(kA, n, b.ghist);
+    uint32_t mask;
+    if (mode_ == Mode::replay)
+    {
+        if (tr_cursor_ >= tr_->sort_mask.size()) return Status::internal;
+        mask = tr_->sort_mask[tr_cursor_++];
+        k_guard_sort<<<1, 256, 0, s>>>(b.ghist, n, mask, &b.counters[15]);
+    }
+    else
+    {
+        QDT_HIP(hipMemcpyAsync(ghist_host_, b.ghist, 4 * 8 * 256, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipStreamSynchronize(s));
+        mask = pass_mask(ghist_host_, n);
+        if (mode_ == Mode::record) tr_->sort_mask.push_back(mask);
+    }
+    sort_passes(kA, vA, kB, vB, n, mask, b.hist, b.partials, s);
+    return Status::ok;
 }
 
 Status Engine::ensure(size_t n)
@@ -2773,7 +3061,7 @@ Status Engine::ensure_ws(size_t n)
         { (void**)&b.P0, 16 * N },      { (void**)&b.P1, 16 * N },      { (void**)&b.flagp, N },         { (void**)&b.NX, 4 * 4 * R },
         { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
         { (void**)&b.VE, 4 * N },       { (void**)&b.owner, 4 * N },
-        { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },
+        { (void**)&b.spare, 64 },       { (void**)&b.one_set, 64 },     { (void**)&b.rblock, 256 },
     };
     const Status st = carve_region(items, ws_, ws_bytes_);
     if (st != Status::ok) return st;
@@ -2863,32 +3151,6 @@ Status Engine::ensure_sets(size_t k, size_t n)
     return Status::ok;
 }
 
-// Sorts (k, v) pairs stably by key; the result ends in kA/vA.
-static Status sort_pairs(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t* hist,
-                         uint32_t* partials, uint32_t* ghist, uint32_t* host_ghist, hipStream_t s)
-{
-    if (n <= 1) return Status::ok;
-    QDT_HIP(hipMemsetAsync(ghist, 0, 4 * 8 * 256, s));
-    k_radix_global_hist<<<std::min<uint32_t>(cdiv(n, kBlock), 1024), kBlock, 0, s>>>(kA, n, ghist);
-    QDT_HIP(hipMemcpyAsync(host_ghist, ghist, 4 * 8 * 256, hipMemcpyDeviceToHost, s));
-    QDT_HIP(hipStreamSynchronize(s));
-    const uint32_t nb = cdiv(n, kSortTile);
-    for (int pass = 0; pass < 8; ++pass)
-    {
-        bool trivial = false;
-        for (int dgt = 0; dgt < 256; ++dgt)
-            if (host_ghist[pass * 256 + dgt] == n) trivial = true;
-        if (trivial) continue;
-        const int shift = 8 * pass;
-        k_radix_hist<<<nb, kBlock, 0, s>>>(kA, n, shift, hist);
-        scan_u32(LoadU32{ hist }, 256 * nb, StoreU32{ hist }, partials, s);
-        k_radix_scatter<<<nb, kBlock, 0, s>>>(kA, vA, kB, vB, n, shift, hist);
-        std::swap(kA, kB);
-        std::swap(vA, vB);
-    }
-    return Status::ok;
-}
-
 Status Engine::run(const double* d_pts, size_t n)
 {
     res_ = DeviceResult();
@@ -2932,8 +3194,206 @@ Status Engine::run_batch(const double* d_pts, const size_t* set_off, int k)
     if (s != Status::ok) return s;
     s = ensure_sets((size_t)k, n);
     if (s != Status::ok) return s;
+    if (!graphs_ || ext_ || profiling_) return run_batch_eager(d_pts, set_off, k);
+
+    // the shape: the sets' sizes and the launch parameters
+    std::vector<size_t> sizes(k);
+    for (int i = 0; i < k; ++i) sizes[i] = set_off[i + 1] - set_off[i];
+    GraphEntry* g = nullptr;
+    for (GraphEntry* e : graphs_cache_)
+        if (e->ws_gen == ws_gen_ && e->seams == seams_ && e->cell_max == cell_max_ &&
+            e->merge_lanes == merge_lanes_ && e->sizes == sizes)
+            g = e;
+    if (g)
+    {
+        bool ok = false;
+        s = replay_graph(g, d_pts, set_off, k, ok);
+        if (s != Status::ok || ok) return s;
+        // a guard failed (the data asks for other launches than recorded):
+        // this batch runs eagerly, the graph stays for the next one
+        const RunStats keep = st_;
+        st_ = RunStats();
+        s = run_batch_eager(d_pts, set_off, k);
+        st_.graph_fallback = 1;
+        st_.err_bits |= keep.err_bits;
+        return s;
+    }
+
+    // a new shape: run it in record mode, then capture the same run
+    GraphEntry* e = new GraphEntry();
+    e->sizes = sizes;
+    e->seams = seams_;
+    e->cell_max = cell_max_;
+    e->merge_lanes = merge_lanes_;
+    mode_ = Mode::record;
+    tr_ = &e->tr;
     bool redo = false;
     s = run_impl(d_pts, n, set_off, k, seams_, redo);
+    mode_ = Mode::eager;
+    tr_ = nullptr;
+    if (s != Status::ok || redo)
+    {
+        delete e;
+        if (s != Status::ok) return s;
+        // the seam pieces ran out: this shape is run eagerly (no graph)
+        return run_batch_eager(d_pts, set_off, k);
+    }
+    const RunStats rec = st_;
+    const BatchResult rec_batch = batch_;
+    const DeviceResult rec_res = res_;
+    const Status cs = capture(e, d_pts, set_off, k);
+    st_ = rec;
+    batch_ = rec_batch;
+    res_ = rec_res;
+    if (cs == Status::ok)
+    {
+        e->ws_gen = ws_gen_;
+        if (graphs_cache_.size() >= 8)
+        {
+            // drop the least used entry
+            size_t lo = 0;
+            for (size_t i = 1; i < graphs_cache_.size(); ++i)
+                if (graphs_cache_[i]->uses < graphs_cache_[lo]->uses) lo = i;
+            delete graphs_cache_[lo];
+            graphs_cache_.erase(graphs_cache_.begin() + (long)lo);
+        }
+        graphs_cache_.push_back(e);
+        st_.graph_captured = 1;
+    }
+    else
+        delete e;
+    return Status::ok;
+}
+
+// The batch pipeline of a recorded shape, captured from the engine's stream
+// into a graph.  The first node copies the caller's points into the
+// engine's own input buffer (its source is reset before every launch), so
+// the graph does not depend on where the caller keeps them.
+Status Engine::capture(GraphEntry* e, const double* d_pts, const size_t* set_off, int k)
+{
+    const size_t n = set_off[k];
+    Bufs& b = *b_;
+    if (!gin_ || gin_cap_ < n)
+    {
+        const Item items[] = { { (void**)&gin_, 16 * std::max<size_t>(n, 64) } };
+        free_region(ws_gin_, ws_gin_bytes_);
+        gin_ = nullptr;
+        QDT_TRY(carve_region(items, ws_gin_, ws_gin_bytes_));
+        gin_cap_ = std::max<size_t>(n, 64);
+    }
+    Trace& t = e->tr;
+    t.stage_cap = t.upload_bytes + 4096;
+    if (hipHostMalloc((void**)&t.stage, t.stage_cap, hipHostMallocDefault) != hipSuccess)
+    {
+        t.stage = nullptr;
+        return Status::out_of_memory;
+    }
+    t.stage_used = 0;
+    if (hipHostMalloc((void**)&e->rb, 4 * (64 + (size_t)k + 1), hipHostMallocDefault) != hipSuccess)
+    {
+        e->rb = nullptr;
+        return Status::out_of_memory;
+    }
+    if (t.kg == 0) return Status::internal; // nothing to triangulate: not worth a graph
+    mode_ = Mode::replay;
+    tr_ = &t;
+    tr_cursor_ = 0;
+    hipStream_t s = stream_;
+    Status st = Status::ok;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        st = Status::internal;
+    else
+    {
+        if (hipMemcpyAsync(gin_, d_pts, 16 * n, hipMemcpyDeviceToDevice, s) != hipSuccess) st = Status::internal;
+        bool redo = false;
+        if (st == Status::ok) st = run_impl(gin_, n, set_off, k, e->seams, redo);
+        hipGraph_t graph = nullptr;
+        const hipError_t ee = hipStreamEndCapture(s, &graph);
+        if (ee != hipSuccess || !graph) st = Status::internal;
+        e->graph = graph;
+    }
+    mode_ = Mode::eager;
+    tr_ = nullptr;
+    if (st != Status::ok) return st;
+    if (tr_cursor_ != t.sort_mask.size()) return Status::internal; // the replay took another path
+    // the input copy is the graph's only root
+    size_t nroot = 0;
+    if (hipGraphGetRootNodes(e->graph, nullptr, &nroot) != hipSuccess || nroot != 1) return Status::internal;
+    if (hipGraphGetRootNodes(e->graph, &e->in_node, &nroot) != hipSuccess) return Status::internal;
+    hipGraphNodeType type;
+    if (hipGraphNodeGetType(e->in_node, &type) != hipSuccess || type != hipGraphNodeTypeMemcpy) return Status::internal;
+    if (hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0) != hipSuccess)
+    {
+        e->exec = nullptr;
+        return Status::internal;
+    }
+    e->in_src = d_pts;
+    (void)b;
+    return Status::ok;
+}
+
+// One launch of a captured graph on this batch's points, then the result
+// block: ok = false when a guard failed or the seam pieces ran out.
+Status Engine::replay_graph(GraphEntry* g, const double* d_pts, const size_t* set_off, int k, bool& ok)
+{
+    ok = false;
+    Bufs& b = *b_;
+    hipStream_t s = stream_;
+    const size_t n = set_off[k];
+    if (d_pts != g->in_src)
+    {
+        QDT_HIP(hipGraphExecMemcpyNodeSetParams1D(g->exec, g->in_node, gin_, d_pts, 16 * n,
+                                                  hipMemcpyDeviceToDevice));
+        g->in_src = d_pts;
+    }
+    QDT_HIP(hipEventRecord(ev0_, s));
+    QDT_HIP(hipGraphLaunch(g->exec, s));
+    QDT_HIP(hipEventRecord(ev1_, s));
+    uint32_t* H = g->rb;
+    const size_t K1 = (size_t)k + 1;
+    QDT_HIP(hipMemcpyAsync(H, b.rblock, 256, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipMemcpyAsync(H + 64, b.toff, 4 * K1, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    ++g->uses;
+    unsigned long long cnts[16];
+    std::memcpy(cnts, H + 4, sizeof cnts);
+    const uint32_t T = H[0], err = H[1];
+    st_.err_bits = err;
+    if (cnts[15] || (err & kSeamErr)) return Status::ok; // guard failed, or the pieces ran out
+    if (err & ~1u) return Status::internal;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+    st_.device_ms = ms;
+    st_.orient = cnts[0];
+    st_.orient_exact = cnts[1];
+    st_.incircle = cnts[2];
+    st_.incircle_exact = cnts[3];
+    st_.extract_queue = cnts[4];
+    for (int i = 0; i < 8; ++i) st_.seam[i] = cnts[5 + i];
+    st_.merge_edges = cnts[13];
+    st_.spare_records = H[2];
+    st_.cells_exact = H[3];
+    st_.triangles = T;
+    st_.unique = g->tr.m;
+    st_.graph_replay = 1;
+    st_.cell_points_max = (uint32_t)cell_max_;
+    st_.merge_lanes = (uint32_t)merge_lanes_;
+    for (int i = 0; i < k; ++i) batch_.status[i] = g->tr.setbad[i] ? (int)Status::not_finite : 0;
+    res_.n = n;
+    res_.rep = b.rep;
+    res_.unique = g->tr.m;
+    res_.tris = b.tris;
+    res_.triangles = T;
+    QDT_TRY(finish_batch(H + 64, T, k, g->tr.kg, g->tr.set_rank, g->tr.vstart));
+    ok = true;
+    return Status::ok;
+}
+
+Status Engine::run_batch_eager(const double* d_pts, const size_t* set_off, int k)
+{
+    const size_t n = set_off[k];
+    bool redo = false;
+    Status s = run_impl(d_pts, n, set_off, k, seams_, redo);
     if (s == Status::ok && redo)
     {
         const RunStats first = st_;
@@ -2945,6 +3405,8 @@ Status Engine::run_batch(const double* d_pts, const size_t* set_off, int k)
         st_.err_bits = first.err_bits;
         st_.spare_records = first.spare_records;
     }
+    st_.cell_points_max = (uint32_t)cell_max_;
+    st_.merge_lanes = (uint32_t)merge_lanes_;
     if (s != Status::ok) batch_ = BatchResult();
     return s;
 }
@@ -2959,11 +3421,12 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     const double2* pts = (const double2*)d_pts_raw;
     const bool multi = set_off != nullptr;
     uint32_t* hs = b.host_scalars;
-    std::vector<uint32_t> ghist_host(8 * 256);
-    uint32_t* hghist = ghist_host.data();
+    const bool replay = mode_ == Mode::replay;
+    const bool record = mode_ == Mode::record;
+    unsigned long long* guard = &b.counters[15];
 
     // profiling: event 0..8 = stage boundaries, 9.. = merge levels (start, end)
-    const bool prof = profiling_;
+    const bool prof = profiling_ && !replay;
     auto mark = [&](size_t i) {
         if (prof)
         {
@@ -2971,7 +3434,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             if (e) (void)hipEventRecord(e, s);
         }
     };
-    if (ev0_) QDT_HIP(hipEventRecord(ev0_, s));
+    if (ev0_ && !replay) QDT_HIP(hipEventRecord(ev0_, s));
     if (ext_)
     {
         // the node and seam regions are carved again after the points' region
@@ -2981,8 +3444,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         ext_used_ = ext_ws_end_;
     }
     mark(0);
-    QDT_HIP(hipMemsetAsync(b.counters, 0, 8 * 16, s));
-    QDT_HIP(hipMemsetAsync(b.err, 0, 64, s));
+    fill_words(b.counters, 2 * 16, 0, s);
+    fill_words(b.err, 16, 0, s);
 
     // 0. several sets: reject the sets holding a non-finite point, then give
     // every set its rank in the kd forest (root level ascending; roots only
@@ -2995,11 +3458,26 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         uint32_t* H = b.host_sets;
         const size_t K1 = (size_t)k + 1;
         for (int i = 0; i <= k; ++i) H[i] = (uint32_t)set_off[i];
-        QDT_HIP(hipMemcpyAsync(b.soff, H, 4 * K1, hipMemcpyHostToDevice, s));
-        QDT_HIP(hipMemsetAsync(b.setbad, 0, 4 * K1, s));
+        QDT_TRY(upload(b.soff, H, 4 * K1));
+        fill_words(b.setbad, K1, 0, s);
         k_check_sets<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, n, b.soff, k, b.setbad);
-        QDT_HIP(hipMemcpyAsync(H + K1, b.setbad, 4 * (size_t)k, hipMemcpyDeviceToHost, s));
-        QDT_HIP(hipStreamSynchronize(s));
+        if (replay)
+        {
+            // the recorded verdicts, guarded against this batch's
+            for (int i = 0; i < k; ++i) H[K1 + i] = tr_->setbad[i];
+            QDT_TRY(upload(b.mstart, H + K1, 4 * (size_t)k));
+            k_guard_words<<<cdiv((size_t)k, kBlock), kBlock, 0, s>>>(b.setbad, b.mstart, (uint32_t)k, guard, 2);
+        }
+        else
+        {
+            QDT_HIP(hipMemcpyAsync(H + K1, b.setbad, 4 * (size_t)k, hipMemcpyDeviceToHost, s));
+            QDT_HIP(hipStreamSynchronize(s));
+            if (record)
+            {
+                tr_->setbad.assign(H + K1, H + K1 + k);
+                tr_->upload_bytes += 4 * (size_t)k + 512;
+            }
+        }
         std::vector<int> depth(k, 0);
         int D0 = 0;
         for (int i = 0; i < k; ++i)
@@ -3039,6 +3517,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                 rank_set.push_back(i);
             }
         }
+        if (record)
+        {
+            tr_->set_rank = set_rank;
+            tr_->kg = kg;
+        }
         if (kg == 0) return Status::ok;
         uint32_t* Hr = H + K1;       // srank
         uint32_t* Hp = H + 2 * K1;   // rpos
@@ -3054,9 +3537,9 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         }
         Hp[k] = pos;
         Hi[k] = n;
-        QDT_HIP(hipMemcpyAsync(b.srank, Hr, 4 * K1, hipMemcpyHostToDevice, s));
-        QDT_HIP(hipMemcpyAsync(b.rpos, Hp, 4 * K1, hipMemcpyHostToDevice, s));
-        QDT_HIP(hipMemcpyAsync(b.in_off, Hi, 4 * K1, hipMemcpyHostToDevice, s));
+        QDT_TRY(upload(b.srank, Hr, 4 * K1));
+        QDT_TRY(upload(b.rpos, Hp, 4 * K1));
+        QDT_TRY(upload(b.in_off, Hi, 4 * K1));
         k_input_rank<<<cdiv(n, kBlock), kBlock, 0, s>>>(n, b.soff, k, b.srank, b.rin);
     }
 
@@ -3064,7 +3547,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     k_keys<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, n, b.kx, b.kA, b.vA, b.err);
     uint64_t *kA = b.kA, *kB = b.kB;
     uint32_t *vA = b.vA, *vB = b.vB;
-    Status st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
+    Status st = sort_keys(kA, vA, kB, vB, n);
     if (st != Status::ok) return st;
     // keep the y order for frame 1, runs of equal y put in x-descending
     // order (counters[14]: a run too long for that)
@@ -3077,16 +3560,16 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         QDT_HIP(hipStreamSynchronize(s));
         if (hs[0] & 1) return Status::not_finite;
     }
-    QDT_HIP(hipMemsetAsync(b.err, 0, 64, s));
+    fill_words(b.err, 16, 0, s);
     mark(1);
     // 2. stable sort by x -> (x, y, index); several sets: then by rank
     k_gather_u64<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.kx, n, kA);
-    st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
+    st = sort_keys(kA, vA, kB, vB, n);
     if (st != Status::ok) return st;
     if (multi)
     {
         k_gather_rank<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.rin, n, kA);
-        st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
+        st = sort_keys(kA, vA, kB, vB, n);
         if (st != Status::ok) return st;
     }
 
@@ -3097,22 +3580,46 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     k_dedupe_flags<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, vA, srank, n, b.flagp, b.P1);
     scan_u32(LoadU8{ b.flagp }, n, StoreU32{ b.dist }, b.partials, s);
     const uint32_t nbs = cdiv(n, kScanTile);
-    QDT_HIP(hipMemcpyAsync(hs, b.partials + nbs, 4, hipMemcpyDeviceToHost, s));
     k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.P1, vA, b.flagp, srank, b.dist, n, b.P0, b.ID0, b.vrank,
                                                       b.Z);
     k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.flagp, b.dist, n, b.ID0, b.rep);
     std::vector<uint32_t> vstart_h;
     if (multi)
-    {
         k_vstart<<<cdiv((size_t)k + 1, kBlock), kBlock, 0, s>>>(b.dist, b.rpos, k, n, b.partials + nbs, b.vstart);
-        QDT_HIP(hipMemcpyAsync(b.host_sets, b.vstart, 4 * ((size_t)k + 1), hipMemcpyDeviceToHost, s));
-    }
     unsigned long long ytie = 1;
-    QDT_HIP(hipMemcpyAsync(hs + 4, &b.counters[14], 8, hipMemcpyDeviceToHost, s));
-    QDT_HIP(hipStreamSynchronize(s));
-    const uint32_t m = hs[0];
-    memcpy(&ytie, hs + 4, 8);
-    if (multi) vstart_h.assign(b.host_sets, b.host_sets + k + 1);
+    uint32_t m;
+    if (replay)
+    {
+        // the recorded distinct counts and frame-1 method, guarded
+        m = tr_->m;
+        ytie = tr_->ytie ? 1 : 0;
+        vstart_h = tr_->vstart;
+        k_guard_scalar<<<1, 64, 0, s>>>(b.partials + nbs, m, guard, 4);
+        k_guard_flag<<<1, 64, 0, s>>>(&b.counters[14], tr_->ytie ? 1 : 0, guard, 8);
+        if (multi)
+        {
+            QDT_TRY(upload(b.sstart, vstart_h.data(), 4 * ((size_t)k + 1)));
+            k_guard_words<<<cdiv((size_t)k + 1, kBlock), kBlock, 0, s>>>(b.vstart, b.sstart, (uint32_t)k + 1,
+                                                                         guard, 16);
+        }
+    }
+    else
+    {
+        QDT_HIP(hipMemcpyAsync(hs, b.partials + nbs, 4, hipMemcpyDeviceToHost, s));
+        if (multi) QDT_HIP(hipMemcpyAsync(b.host_sets, b.vstart, 4 * ((size_t)k + 1), hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipMemcpyAsync(hs + 4, &b.counters[14], 8, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipStreamSynchronize(s));
+        m = hs[0];
+        memcpy(&ytie, hs + 4, 8);
+        if (multi) vstart_h.assign(b.host_sets, b.host_sets + k + 1);
+        if (record)
+        {
+            tr_->m = m;
+            tr_->ytie = ytie != 0;
+            tr_->vstart = vstart_h;
+            tr_->upload_bytes += 4 * ((size_t)k + 1) + 512;
+        }
+    }
     res_.n = n;
     res_.rep = b.rep;
     res_.unique = m;
@@ -3165,18 +3672,18 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             T[4 * d + 2] = in ? P.nodeb[d] : 0;
             T[4 * d + 3] = 0;
         }
-        QDT_HIP(hipMemcpyAsync(b.off, T, 4 * kLvlHdr, hipMemcpyHostToDevice, s));
+        QDT_TRY(upload(b.off, T, 4 * kLvlHdr));
     }
 
     mark(3);
-    QDT_HIP(hipMemsetAsync(b.VE, 0xff, 4 * (size_t)m, s));
-    QDT_HIP(hipMemsetAsync(b.owner, 0xff, 4 * (size_t)m, s));
+    fill_words(b.VE, m, 0xffffffffu, s);
+    fill_words(b.owner, m, 0xffffffffu, s);
     // the face flags of the extraction double as the deletion marks of the
     // seam anchors (-1 = kept) until the merges end
-    QDT_HIP(hipMemsetAsync(b.face, 0xff, 4 * b.rec_cap, s));
-    QDT_HIP(hipMemsetAsync(b.spare, 0, 64, s));
+    fill_words(b.face, b.rec_cap, 0xffffffffu, s);
+    fill_words(b.spare, 16, 0, s);
     if (m_forest < m) // records of rejected sets' points stay free
-        QDT_HIP(hipMemsetAsync(b.OR + 6 * (size_t)m_forest, 0xff, 4 * 6 * (size_t)(m - m_forest), s));
+        fill_words(b.OR + 6 * (size_t)m_forest, 6 * (size_t)(m - m_forest), 0xffffffffu, s);
     const int lim_e = (int)(4 * b.rec_cap);
     const int spare_base = (int)(3 * (size_t)m);
     const int spare_cap = (int)(spare_limit_ ? std::min<size_t>(spare_limit_, b.rec_cap - 3 * (size_t)m)
@@ -3199,7 +3706,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         if (multi)
         {
             k_rank_pairs<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.Bt, b.rin, n, kA, vA);
-            st = sort_pairs(kA, vA, kB, vB, n, b.hist, b.partials, b.ghist, hghist, s);
+            st = sort_keys(kA, vA, kB, vB, n);
             if (st != Status::ok) return st;
             perm = vA;
             w = reinterpret_cast<uint32_t*>(kA);
@@ -3214,12 +3721,12 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     else
     {
         k_frame1_keys<<<cdiv(m, kBlock), kBlock, 0, s>>>(b.P0, m, b.kA, b.vA);
-        st = sort_pairs(kA, vA, kB, vB, m, b.hist, b.partials, b.ghist, hghist, s);
+        st = sort_keys(kA, vA, kB, vB, m);
         if (st != Status::ok) return st;
         if (multi)
         {
             k_gather_rank<<<cdiv(m, kBlock), kBlock, 0, s>>>(vA, b.vrank, m, kA);
-            st = sort_pairs(kA, vA, kB, vB, m, b.hist, b.partials, b.ghist, hghist, s);
+            st = sort_keys(kA, vA, kB, vB, m);
             if (st != Status::ok) return st;
         }
     }
@@ -3238,17 +3745,19 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     mark(4);
     // 5. kd levels
     const int D = P.D;
-    const uint32_t* vstart = multi ? b.vstart : nullptr;
+    // (a replay builds the recorded forest from the recorded counts: its
+    // shape stays consistent with the launches even when a guard fails)
+    const uint32_t* vstart = multi ? (replay ? b.sstart : b.vstart) : nullptr;
     if (!multi)
     {
         hs[2] = 0;
         hs[3] = m;
-        QDT_HIP(hipMemcpyAsync(b.one_set, hs + 2, 8, hipMemcpyHostToDevice, s));
+        QDT_TRY(upload(b.one_set, hs + 2, 8));
         vstart = b.one_set; // {0, m}: the one set's distinct points
     }
     st_.levels = (uint32_t)D;
-    QDT_HIP(hipMemsetAsync(b.res, 0xff, 16 * P.nodes, s));
-    QDT_HIP(hipMemsetAsync(b.lvlK, 0, 4 * kMaxLevels, s));
+    fill_words(b.res, 4 * P.nodes, 0xffffffffu, s);
+    fill_words(b.lvlK, kMaxLevels, 0, s);
     uint32_t *nodeA = b.nodeA, *nodeB = b.nodeB;
     uint32_t* Bt = b.Bt;
     auto roots_at = [&](int d, uint32_t* node) {
@@ -3264,10 +3773,17 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     // (the level from which every node, the roots below included, fits a cell)
     int dc = D + 1;
     if (cells_)
-        while (dc > 0 && P.smax[dc - 1] <= kCellMax) --dc;
+        while (dc > 0 && P.smax[dc - 1] <= cell_max_) --dc;
     const bool use_cells = cells_ && dc <= D;
     if (!use_cells) dc = D;
     const int Dk = use_cells ? dc : D; // levels partitioned by the level kernels
+    // a replay cuts the levels the recorded run cut, with at most its anchors
+    // per node (fewer anchors than a node asks for give the same merge)
+    if (replay)
+    {
+        if (tr_->K.size() != (size_t)D + 1) return Status::internal;
+        for (int d = 0; d <= D; ++d) P.K[d] = tr_->K[d];
+    }
     roots_at(0, nodeA);
     for (int d = 0; d < Dk; ++d)
     {
@@ -3322,19 +3838,27 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     // the levels the nodes asked to cut into pieces, and their anchor counts
     {
         std::vector<int> lk(kMaxLevels, 0);
-        QDT_HIP(hipMemcpyAsync(lk.data(), b.lvlK, 4 * kMaxLevels, hipMemcpyDeviceToHost, s));
-        QDT_HIP(hipStreamSynchronize(s));
+        if (!replay)
+        {
+            QDT_HIP(hipMemcpyAsync(lk.data(), b.lvlK, 4 * kMaxLevels, hipMemcpyDeviceToHost, s));
+            QDT_HIP(hipStreamSynchronize(s));
+        }
         size_t seg_nodes = 1, anchor_cap = 1;
         for (int d = 0; d < D; ++d)
         {
-            P.K[d] = d < Dk ? std::min(P.K[d], lk[d]) : 0;
-            if (P.K[d] < kSegLevelAnchors || (P.count[d] >= kSegLevelNodes && P.smax[d] < kSegLevelBig)) P.K[d] = 0;
+            if (!replay)
+            {
+                P.K[d] = d < Dk ? std::min(P.K[d], lk[d]) : 0;
+                if (P.K[d] < kSegLevelAnchors || (P.count[d] >= kSegLevelNodes && P.smax[d] < kSegLevelBig))
+                    P.K[d] = 0;
+            }
             if (!P.K[d]) continue;
             seg_nodes = std::max(seg_nodes, (size_t)P.count[d]);
             anchor_cap = std::max(anchor_cap, (size_t)P.count[d] * P.K[d]);
         }
         st = ensure_anchors(seg_nodes, anchor_cap);
         if (st != Status::ok) return st;
+        if (record) tr_->K = P.K;
     }
 
     mark(5);
@@ -3347,17 +3871,21 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         st_.block_level = (uint32_t)dc;
         st_.cells = ncell;
         st_.cell_points = m_forest;
-        QDT_HIP(hipMemsetAsync(b.redo, 0, 4, s));
+        fill_words(b.redo, 1, 0, s);
         if (ncell)
         {
             int cap = 1;
             for (int d = dc; d <= D; ++d) cap = std::max(cap, (int)P.smax[d]);
             const size_t lds = cell_lds_bytes(cap);
-            k_cell<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, newid, b.off, dc, D, cap, b.res, b.redo,
+            // positions per lane of the cells' scans: the smallest that holds cap
+            auto cell = cap <= 128 ? k_cell<2> : (cap <= 256 ? k_cell<4> : k_cell<8>);
+            auto cell_x = cap <= 128 ? k_cell_exact<2> : (cap <= 256 ? k_cell_exact<4> : k_cell_exact<8>);
+            cell<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, newid, b.off, dc, D, cap, b.res, b.redo,
+                                        b.err, b.counters);
+            // one workgroup per listed cell (the rest end at once): on
+            // lattices nearly every cell is listed
+            cell_x<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, newid, b.off, dc, D, cap, b.res, b.redo,
                                           b.err, b.counters);
-            k_cell_exact<<<std::min<uint32_t>(ncell, 1024), 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, newid,
-                                                                         b.off, dc, D, cap, b.res, b.redo, b.err,
-                                                                         b.counters);
         }
     }
     else
@@ -3377,8 +3905,12 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         if (!cnt)
             ;
         else if (!K)
-            k_merge<<<cdiv(cnt, kMergeLanes), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res,
-                                                                    b.err, b.counters);
+        {
+            const uint32_t lanes = (uint32_t)merge_lanes_;
+            auto kern = lanes >= 64 ? k_merge<64> : (lanes >= 32 ? k_merge<32> : k_merge<16>);
+            kern<<<cdiv(cnt, lanes), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res, b.err,
+                                                          b.counters);
+        }
         else
         {
             const int total = (int)(cnt * (uint32_t)K);
@@ -3448,8 +3980,17 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                                                         multi ? 3 * m : 0xffffffffu, multi ? b.tmain : b.partials,
                                                         b.tris);
     mark(8);
-    if (ev1_) QDT_HIP(hipEventRecord(ev1_, s));
     const uint32_t nbx = cdiv(nrec, kScanTile);
+    if (replay)
+    {
+        // the counts stay on the device; one block of scalars is read after
+        // the graph (with the ranks' triangle offsets)
+        QDT_TRY(batch_output(b.partials + nbx, 0, k, kg, set_rank, vstart_h));
+        k_result_block<<<1, 64, 0, s>>>(b.partials + nbx, b.err, b.spare, use_cells ? b.redo : nullptr, b.counters,
+                                        b.rblock);
+        return Status::ok;
+    }
+    if (ev1_) QDT_HIP(hipEventRecord(ev1_, s));
     QDT_HIP(hipMemcpyAsync(hs, b.partials + nbx, 4, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipMemcpyAsync(hs + 1, b.err, 4, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipMemcpyAsync(hs + 2, b.spare, 4, hipMemcpyDeviceToHost, s));
@@ -3511,84 +4052,95 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     }
     res_.tris = b.tris;
     res_.triangles = hs[0];
-    if (multi) return batch_output(hs[0], k, kg, m);
+    if (multi) return batch_output(b.partials + nbx, hs[0], k, kg, set_rank, vstart_h);
     return Status::ok;
 }
 
 // Groups the triangles by set with set-local indices (batch_.tris) and makes
-// rep set-local.
-Status Engine::batch_output(uint32_t T, int k, int kg, uint32_t m)
+// rep set-local.  The counts the kernels need stay on the device (dT: the
+// triangle total, b.tmain: the triangles of records below the spare region),
+// so that the same launches serve a captured graph; set_rank: every set's
+// rank, vstart: every rank's first distinct point (host).
+Status Engine::batch_output(const uint32_t* dT, uint32_t T, int k, int kg, const std::vector<int>& set_rank,
+                            const std::vector<uint32_t>& vstart)
 {
     Bufs& b = *b_;
     hipStream_t s = stream_;
     uint32_t* hs = b.host_scalars;
     uint32_t* H = b.host_sets;
     const size_t K1 = (size_t)k + 1;
-    std::vector<uint32_t> ghist_host(8 * 256);
-    QDT_HIP(hipMemcpyAsync(hs, b.tmain, 4, hipMemcpyDeviceToHost, s));
-    QDT_HIP(hipStreamSynchronize(s));
-    const uint32_t tmain = std::min(hs[0], T);
-    const uint32_t c = T - tmain; // triangles of spare records
-    if (T > b.tris_cap || c > cap_n_)
+    const bool replay = mode_ == Mode::replay;
+    unsigned long long* guard = &b.counters[15];
+    uint32_t T_bound, c_bound;
+    if (replay)
     {
-        st_.err_bits = 1u << 20;
-        return Status::internal;
+        T_bound = tr_->T_bound;
+        c_bound = tr_->c_bound;
+        k_guard_le<<<1, 64, 0, s>>>(dT, T_bound, guard, 32);
+        k_guard_spare<<<1, 64, 0, s>>>(dT, b.tmain, c_bound, guard, 64);
     }
-    k_tri_rank<<<cdiv(std::max<uint32_t>(T, 1), kBlock), kBlock, 0, s>>>(b.tris, T, b.rin, b.trank);
-    k_rank_bounds<uint32_t><<<cdiv(K1, kBlock), kBlock, 0, s>>>(b.trank, tmain, kg, b.mstart);
+    else
+    {
+        QDT_HIP(hipMemcpyAsync(hs, b.tmain, 4, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipStreamSynchronize(s));
+        const uint32_t tmain = std::min(hs[0], T);
+        const uint32_t c = T - tmain; // triangles of spare records
+        if (T > b.tris_cap || c > cap_n_)
+        {
+            st_.err_bits = 1u << 20;
+            return Status::internal;
+        }
+        // the bounds a replay of this shape allows (the recorded run uses
+        // them too, so that the replay's sorts see the same keys)
+        T_bound = (uint32_t)std::min<size_t>(b.tris_cap, (size_t)T + std::max<size_t>(4096, T / 64));
+        c_bound = (uint32_t)std::min<size_t>(cap_n_, (size_t)c + std::max<size_t>(1024, c / 16));
+        if (mode_ == Mode::record)
+        {
+            tr_->T = T;
+            tr_->tmain = tmain;
+            tr_->T_bound = T_bound;
+            tr_->c_bound = c_bound;
+        }
+    }
+    k_tri_rank<<<cdiv(std::max<uint32_t>(T_bound, 1), kBlock), kBlock, 0, s>>>(b.tris, dT, b.rin, b.trank);
+    k_rank_bounds<uint32_t><<<cdiv(K1, kBlock), kBlock, 0, s>>>(b.trank, b.tmain, T_bound, kg, b.mstart);
     uint64_t *kA = b.kA, *kB = b.kB;
     uint32_t *vA = b.vA, *vB = b.vB;
-    if (c)
-    {
-        k_spare_keys<<<cdiv(c, kBlock), kBlock, 0, s>>>(b.trank + tmain, c, kA, vA);
-        const Status st = sort_pairs(kA, vA, kB, vB, c, b.hist, b.partials, b.ghist, ghist_host.data(), s);
-        if (st != Status::ok) return st;
-    }
-    k_rank_bounds<uint64_t><<<cdiv(K1, kBlock), kBlock, 0, s>>>(kA, c, kg, b.sstart);
-    QDT_HIP(hipMemcpyAsync(H, b.mstart, 4 * ((size_t)kg + 1), hipMemcpyDeviceToHost, s));
-    QDT_HIP(hipMemcpyAsync(H + K1, b.sstart, 4 * ((size_t)kg + 1), hipMemcpyDeviceToHost, s));
-    QDT_HIP(hipMemcpyAsync(H + 2 * K1, b.vstart, 4 * K1, hipMemcpyDeviceToHost, s));
+    k_spare_keys<<<cdiv(std::max<uint32_t>(c_bound, 1), kBlock), kBlock, 0, s>>>(b.trank, dT, b.tmain, c_bound,
+                                                                                 (uint32_t)kg, kA, vA);
+    QDT_TRY(sort_keys(kA, vA, kB, vB, c_bound));
+    k_rank_bounds<uint64_t><<<cdiv(K1, kBlock), kBlock, 0, s>>>(kA, nullptr, c_bound, kg, b.sstart);
+    k_tri_offsets<<<1, 1024, 0, s>>>(b.mstart, b.sstart, kg, b.toff);
+    k_batch_tris<<<cdiv(std::max<uint32_t>(T_bound, 1), kBlock), kBlock, 0, s>>>(
+        b.tris, dT, b.tmain, b.trank, kA, vA, b.mstart, b.sstart, b.toff, b.in_off, b.btris);
+    k_rep_local<<<cdiv(res_.n, kBlock), kBlock, 0, s>>>(b.rep, (uint32_t)res_.n, b.soff, k);
+    if (replay) return Status::ok; // the host reads the offsets after the graph
+    QDT_HIP(hipMemcpyAsync(H, b.toff, 4 * ((size_t)kg + 1), hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
-    uint32_t* Ht = H + 3 * K1;
-    uint32_t at = 0;
-    std::vector<uint32_t> cnt(kg);
-    for (int r = 0; r < kg; ++r)
-    {
-        Ht[r] = at;
-        cnt[r] = (H[r + 1] - H[r]) + (H[K1 + r + 1] - H[K1 + r]);
-        at += cnt[r];
-    }
-    Ht[kg] = at;
-    if (at != T)
+    return finish_batch(H, T, k, kg, set_rank, vstart);
+}
+
+// Per set, in the caller's order: distinct points and triangle range, from
+// the ranks' triangle offsets toff (host, kg + 1 entries).
+Status Engine::finish_batch(const uint32_t* toff, uint32_t T, int k, int kg, const std::vector<int>& set_rank,
+                            const std::vector<uint32_t>& vstart)
+{
+    Bufs& b = *b_;
+    if (toff[kg] != T)
     {
         st_.err_bits = 1u << 21;
         return Status::internal;
     }
-    QDT_HIP(hipMemcpyAsync(b.toff, Ht, 4 * ((size_t)kg + 1), hipMemcpyHostToDevice, s));
-    if (T)
-        k_batch_tris<<<cdiv(T, kBlock), kBlock, 0, s>>>(b.tris, T, tmain, b.trank, kA, vA, b.mstart, b.sstart,
-                                                        b.toff, b.in_off, b.btris);
-    k_rep_local<<<cdiv(res_.n, kBlock), kBlock, 0, s>>>(b.rep, (uint32_t)res_.n, b.soff, k);
-    QDT_HIP(hipStreamSynchronize(s));
-    // per set, in the caller's order
-    const uint32_t* vs = H + 2 * K1;
-    std::vector<int> rank_of(k, -1);
-    {
-        std::vector<uint32_t> srank(k);
-        QDT_HIP(hipMemcpy(srank.data(), b.srank, 4 * (size_t)k, hipMemcpyDeviceToHost));
-        for (int i = 0; i < k; ++i) rank_of[i] = (int)srank[i];
-    }
     for (int i = 0; i < k; ++i)
     {
-        const int r = rank_of[i];
+        const int r = set_rank[i];
         if (r < kg)
         {
-            batch_.unique[i] = vs[r + 1] - vs[r];
-            batch_.tri_start[i] = Ht[r];
-            batch_.tri_count[i] = cnt[r];
+            batch_.unique[i] = vstart[r + 1] - vstart[r];
+            batch_.tri_start[i] = toff[r];
+            batch_.tri_count[i] = toff[r + 1] - toff[r];
         }
     }
-    (void)m;
     batch_.tris = b.btris;
     batch_.rep = b.rep;
     batch_.n = res_.n;

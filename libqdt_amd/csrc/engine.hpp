@@ -42,6 +42,11 @@ struct RunStats
     uint32_t err_bits;
     uint64_t spare_records; // records drawn from the spare region (demand, may exceed it)
     uint64_t cells_exact;   // cells rebuilt with the exact predicates (a filter could not decide)
+    // batched form with graphs on: this call was a replay of a captured
+    // graph / the first of its shape (run eagerly, then captured) / a
+    // replay whose guards failed (run eagerly instead)
+    uint32_t graph_replay, graph_captured, graph_fallback;
+    uint32_t cell_points_max, merge_lanes; // the tuned parameters of the run
 };
 
 enum class Status : int
@@ -163,6 +168,17 @@ public:
     void set_profiling(bool on) { profiling_ = on; }
     // Cuts large merges into seam pieces (default on).
     void set_seams(bool on) { seams_ = on; }
+    // Batched form: capture each shape's pipeline in a HIP graph and replay
+    // it for later batches of the same shape (default off).
+    void set_graphs(bool on) { graphs_ = on; }
+    bool graphs() const { return graphs_; }
+    // The tunable launch parameters: the largest kd node built in one LDS
+    // cell (128, 256 or 512 points; the cell kernel's LDS per wavefront and
+    // so its wavefronts per CU follow from it) and the merges per wavefront
+    // of the one-thread-per-node merge kernel (16, 32 or 64).
+    void set_tuning(int cell_points, int merge_lanes);
+    int cell_points() const { return cell_max_; }
+    int merge_lanes() const { return merge_lanes_; }
     // Builds the bottom kd levels in LDS cells (default on).
     void set_cells(bool on) { cells_ = on; }
     // Device bytes currently held.
@@ -189,7 +205,42 @@ private:
     Status ensure_anchors(size_t seg_nodes, size_t anchor_cap);
     Status ensure_sets(size_t k, size_t n);
     Status run_impl(const double* d_pts, size_t n, const size_t* set_off, int k, bool seams, bool& redo);
-    Status batch_output(uint32_t T, int k, int kg, uint32_t m);
+    Status batch_output(const uint32_t* dT, uint32_t T, int k, int kg, const std::vector<int>& set_rank,
+                        const std::vector<uint32_t>& vstart);
+    Status finish_batch(const uint32_t* toff, uint32_t T, int k, int kg, const std::vector<int>& set_rank,
+                        const std::vector<uint32_t>& vstart);
+    Status run_batch_eager(const double* d_pts, const size_t* set_off, int k);
+
+    // Graph capture of the batched form.  A batch of a new shape runs
+    // eagerly in record mode, which keeps every value a host decision was
+    // taken on (a trace); the same run is then captured in replay mode,
+    // where those values come from the trace and device guards compare them
+    // with the values of the data at hand.  A replay whose guards fail is
+    // run eagerly.
+    enum class Mode
+    {
+        eager,
+        record,
+        replay
+    };
+    struct Trace;
+    struct GraphEntry;
+    Mode mode_ = Mode::eager;
+    Trace* tr_ = nullptr;
+    size_t tr_cursor_ = 0;
+    bool graphs_ = false;
+    std::vector<GraphEntry*> graphs_cache_;
+    uint64_t ws_gen_ = 0; // bumped whenever a region is carved (cached graphs hold its pointers)
+    int cell_max_ = 256, merge_lanes_ = 16;
+    void drop_graphs();
+    Status sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n);
+    Status upload(void* dst, const void* src, size_t bytes);
+    Status replay_graph(GraphEntry* g, const double* d_pts, const size_t* set_off, int k, bool& ok);
+    Status capture(GraphEntry* e, const double* d_pts, const size_t* set_off, int k);
+    char* ws_gin_ = nullptr; // the graphs' input buffer
+    size_t ws_gin_bytes_ = 0;
+    double* gin_ = nullptr;
+    size_t gin_cap_ = 0;
 
     hipStream_t stream_;
     bool own_stream_;
@@ -211,6 +262,7 @@ private:
     size_t ext_bytes_ = 0, ext_used_ = 0, ext_ws_end_ = 0;
     uint32_t ext_scalars_[16] = {};
     int ext_table_[4 * 64] = {};
+    uint32_t ghist_host_[8 * 256] = {}; // a sort's global digit histogram, read back
     template <size_t N>
     Status carve_region(const struct Item (&items)[N], char*& region, size_t& bytes);
     void free_region(char*& region, size_t& bytes);

@@ -10,6 +10,7 @@
 #include "../../include/qdt_gpu.h"
 #include "engine.hpp"
 #include "mesh_edit.hpp"
+#include "tuner.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -240,6 +241,11 @@ void copy_stats(const qdt::RunStats& s, qdt_gpu_stats& o)
     o.cell_level = s.block_level;
     o.spare_records = s.spare_records;
     o.cells_exact = s.cells_exact;
+    o.graph_replay = s.graph_replay;
+    o.graph_captured = s.graph_captured;
+    o.graph_fallback = s.graph_fallback;
+    o.cell_points = s.cell_points_max;
+    o.merge_lanes = s.merge_lanes;
 }
 
 void record_stats(qdt_t* Q)
@@ -882,6 +888,9 @@ struct qdt_gpu_batch
     qdt_gpu_stats stats{};
     bool profiling = false;
     bool seams = true;
+    bool graphs = true;
+    int cell_points = 0, merge_lanes = 0; // 0: autotuned
+    qdt::TuneChoice tune;
     ~qdt_gpu_batch()
     {
         delete eng;
@@ -908,6 +917,7 @@ qdt_status batch_engine(qdt_gpu_batch* B)
     if (!B->eng) return QDT_ERR_OUT_OF_MEMORY;
     B->eng->set_profiling(B->profiling);
     B->eng->set_seams(B->seams);
+    B->eng->set_graphs(B->graphs);
     if (!B->eng->stream())
     {
         delete B->eng;
@@ -960,7 +970,17 @@ qdt_status batch_run(qdt_gpu_batch* B, const qdt_vec2d* pts, const size_t* offse
             if (hipMemcpy(B->d_in, pts, bytes, hipMemcpyHostToDevice) != hipSuccess) return QDT_ERR_INTERNAL;
             src = B->d_in;
         }
-        const qdt::Status s = B->eng->run_batch(src, offsets, (int)k);
+        qdt::Status s = qdt::Status::ok;
+        if (B->cell_points && B->merge_lanes)
+        {
+            B->eng->set_tuning(B->cell_points, B->merge_lanes);
+            B->tune = qdt::TuneChoice();
+            B->tune.cell_points = B->eng->cell_points();
+            B->tune.merge_lanes = B->eng->merge_lanes();
+        }
+        else
+            s = qdt::tune_batch(*B->eng, src, offsets, (int)k, B->tune);
+        if (s == qdt::Status::ok) s = B->eng->run_batch(src, offsets, (int)k);
         if (s != qdt::Status::ok)
         {
             reset_batch(B);
@@ -968,6 +988,8 @@ qdt_status batch_run(qdt_gpu_batch* B, const qdt_vec2d* pts, const size_t* offse
             return map_status(s);
         }
         copy_stats(B->eng->stats(), B->stats);
+        B->stats.tune_source = (uint64_t)B->tune.source;
+        B->stats.tune_ms = B->tune.device_ms;
         B->k = k;
         B->n = n;
         return QDT_OK;
@@ -1113,6 +1135,20 @@ void qdt_gpu_batch_set_seams(qdt_gpu_batch* b, int enable)
     if (!b) return;
     b->seams = enable != 0;
     if (b->eng) b->eng->set_seams(b->seams);
+}
+
+void qdt_gpu_batch_set_graphs(qdt_gpu_batch* b, int enable)
+{
+    if (!b) return;
+    b->graphs = enable != 0;
+    if (b->eng) b->eng->set_graphs(b->graphs);
+}
+
+void qdt_gpu_batch_set_tuning(qdt_gpu_batch* b, int cell_points, int merge_lanes)
+{
+    if (!b) return;
+    b->cell_points = cell_points > 0 ? cell_points : 0;
+    b->merge_lanes = merge_lanes > 0 ? merge_lanes : 0;
 }
 
 } // extern "C"

@@ -1,0 +1,169 @@
+// tuner.cpp -- see tuner.hpp.
+#include "tuner.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <sys/stat.h>
+#include <utility>
+
+namespace qdt
+{
+
+namespace
+{
+
+int log2_bucket(size_t v)
+{
+    int b = 0;
+    while (v > 1)
+    {
+        v >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+std::string tune_path()
+{
+    if (const char* p = std::getenv("QDT_TUNE_FILE")) return p;
+    std::string dir;
+    if (const char* x = std::getenv("XDG_CACHE_HOME"))
+        dir = x;
+    else if (const char* h = std::getenv("HOME"))
+        dir = std::string(h) + "/.cache";
+    else
+        return std::string();
+    return dir + "/qdt_amd/tune.txt";
+}
+
+struct Cache
+{
+    std::mutex mu;
+    bool loaded = false;
+    std::map<std::pair<int, int>, TuneChoice> m;
+
+    void load()
+    {
+        if (loaded) return;
+        loaded = true;
+        const std::string p = tune_path();
+        if (p.empty()) return;
+        FILE* f = std::fopen(p.c_str(), "r");
+        if (!f) return;
+        int a, b, c, l;
+        double ms;
+        while (std::fscanf(f, "%d %d %d %d %lf", &a, &b, &c, &l, &ms) == 5)
+        {
+            TuneChoice t;
+            t.cell_points = c;
+            t.merge_lanes = l;
+            t.device_ms = ms;
+            t.source = 2;
+            m[{ a, b }] = t;
+        }
+        std::fclose(f);
+    }
+
+    void save()
+    {
+        const std::string p = tune_path();
+        if (p.empty()) return;
+        // the directory chain (a failure shows up at fopen)
+        for (size_t at = p.find('/', 1); at != std::string::npos; at = p.find('/', at + 1))
+            (void)mkdir(p.substr(0, at).c_str(), 0755);
+        const std::string tmp = p + ".tmp";
+        FILE* f = std::fopen(tmp.c_str(), "w");
+        if (!f) return;
+        for (const auto& kv : m)
+            std::fprintf(f, "%d %d %d %d %.4f\n", kv.first.first, kv.first.second, kv.second.cell_points,
+                         kv.second.merge_lanes, kv.second.device_ms);
+        std::fclose(f);
+        (void)std::rename(tmp.c_str(), p.c_str());
+    }
+};
+
+Cache& cache()
+{
+    static Cache c;
+    return c;
+}
+
+int env_int(const char* name)
+{
+    const char* v = std::getenv(name);
+    return v ? std::atoi(v) : 0;
+}
+
+} // namespace
+
+Status tune_batch(Engine& eng, const double* d_pts, const size_t* set_off, int k, TuneChoice& out)
+{
+    out = TuneChoice();
+    const int ec = env_int("QDT_CELL_POINTS"), el = env_int("QDT_MERGE_LANES");
+    if (ec || el)
+    {
+        out.cell_points = ec ? ec : out.cell_points;
+        out.merge_lanes = el ? el : out.merge_lanes;
+        out.source = 3;
+        eng.set_tuning(out.cell_points, out.merge_lanes);
+        return Status::ok;
+    }
+    if (k <= 0) return Status::ok;
+    const std::pair<int, int> key{ log2_bucket((size_t)k), log2_bucket(set_off[k]) };
+    Cache& c = cache();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        c.load();
+        auto it = c.m.find(key);
+        if (it != c.m.end())
+        {
+            out = it->second;
+            out.source = 2;
+            eng.set_tuning(out.cell_points, out.merge_lanes);
+            return Status::ok;
+        }
+    }
+    // time every pair: one eager run each after a warm-up of the default
+    static const int kCells[3] = { 128, 256, 512 };
+    static const int kLanes[3] = { 16, 32, 64 };
+    struct Eager
+    {
+        Engine& e;
+        bool g;
+        ~Eager() { e.set_graphs(g); }
+    } eager{ eng, eng.graphs() };
+    eng.set_graphs(false);
+    eng.set_tuning(256, 16);
+    Status st = eng.run_batch(d_pts, set_off, k);
+    if (st != Status::ok) return st;
+    double best = INFINITY;
+    for (int c_ : kCells)
+        for (int l : kLanes)
+        {
+            eng.set_tuning(c_, l);
+            st = eng.run_batch(d_pts, set_off, k);
+            if (st != Status::ok) return st;
+            const double ms = eng.stats().device_ms;
+            if (ms > 0 && ms < best)
+            {
+                best = ms;
+                out.cell_points = c_;
+                out.merge_lanes = l;
+            }
+        }
+    out.device_ms = std::isfinite(best) ? best : 0;
+    out.source = 1;
+    eng.set_tuning(out.cell_points, out.merge_lanes);
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        c.m[key] = out;
+        c.save();
+    }
+    return Status::ok;
+}
+
+} // namespace qdt
