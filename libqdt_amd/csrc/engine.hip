@@ -390,6 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_tris(const uint32_t* tris, con
         src = tmain + sidx[j];
         dst = toff[r] + (mstart[r + 1] - mstart[r]) + (j - sstart[r]);
     }
+    if (dst >= T || src >= T) return; // offsets of another batch's counts (a failed replay guard)
     const uint32_t base = in_off[r];
     out[3 * dst] = tris[3 * src] - base;
     out[3 * dst + 1] = tris[3 * src + 1] - base;
@@ -2305,7 +2306,8 @@ struct FacePrefix
 // lowest-duplicate indices)
 __global__ __launch_bounds__(kBlock) void k_face_write(const uint32_t* face, const int* NX, const int* OR,
                                                        const uint32_t* ID0, uint32_t nrec, uint32_t m, int lim_e,
-                                                       uint32_t split, uint32_t* tmain, uint32_t* tris)
+                                                       uint32_t split, uint32_t* tmain, uint32_t* tris,
+                                                       uint32_t tri_cap, int* err)
 {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= nrec) return;
@@ -2314,6 +2316,13 @@ __global__ __launch_bounds__(kBlock) void k_face_write(const uint32_t* face, con
     if (k == split) *tmain = pre; // triangles of the records below the spare region
     const uint32_t fl = f & 3u;
     if (!fl) return;
+    // a store that is not a planar triangulation can show more faces than a
+    // triangulation of m points has (at most 2m - 5): never past the buffer
+    if (pre + (uint32_t)__popc(fl) > tri_cap)
+    {
+        atomicOr(err, 1 << 22);
+        return;
+    }
     // both faces' loads before any store
     bool ok[2];
     int v[2][3];
@@ -2622,8 +2631,12 @@ struct Engine::GraphEntry
     int cell_max = 0, merge_lanes = 0;
     uint64_t ws_gen = 0;
     Trace tr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // two graphs: the first runs up to the distinct points and ends with
+    // the guards on every count the rest of the launches are sized by; the
+    // second (the kd forest, merges, extraction, output) is launched only
+    // when those guards hold
+    hipGraph_t graph = nullptr, graph2 = nullptr;
+    hipGraphExec_t exec = nullptr, exec2 = nullptr;
     hipGraphNode_t in_node = nullptr; // the copy of the caller's points (its source is set per launch)
     const double* in_src = nullptr;
     uint32_t* rb = nullptr;           // pinned: the result block and the ranks' triangle offsets
@@ -2631,7 +2644,9 @@ struct Engine::GraphEntry
     ~GraphEntry()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
+        if (exec2) (void)hipGraphExecDestroy(exec2);
         if (graph) (void)hipGraphDestroy(graph);
+        if (graph2) (void)hipGraphDestroy(graph2);
         if (rb) (void)hipHostFree(rb);
     }
 };
@@ -2888,6 +2903,14 @@ static uint32_t pass_mask(const uint32_t* host_ghist, uint32_t n)
     return mask;
 }
 
+// the radix passes that can vary among keys in [0, v]
+static uint32_t passes_upto(uint64_t v)
+{
+    uint32_t mask = 0;
+    for (int p = 0; p < 8 && (v >> (8 * p)); ++p) mask |= 1u << p;
+    return mask;
+}
+
 // guard of a replayed sort: every pass the recorded mask skips must still
 // have one digit for all n keys
 __global__ __launch_bounds__(kBlock) void k_guard_sort(const uint32_t* ghist, uint32_t n, uint32_t mask,
@@ -2979,7 +3002,7 @@ Status Engine::upload(void* dst, const void* src, size_t bytes)
 // passes whose digit is the same for every key: read back in eager and
 // record mode (and recorded), taken from the trace in replay mode with a
 // device guard on the global histogram.
-Status Engine::sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n)
+Status Engine::sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t must)
 {
     if (n <= 1) return Status::ok;
     Bufs& b = *b_;
@@ -2990,7 +3013,10 @@ Status Engine::sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*&
     if (mode_ == Mode::replay)
     {
         if (tr_cursor_ >= tr_->sort_mask.size()) return Status::internal;
-        mask = tr_->sort_mask[tr_cursor_++];
+        // a replay also runs the passes `must` names: every pass whose digit
+        // can vary for this sort's keys, where a skipped pass would leave
+        // them unsorted in launches the guards do not precede
+        mask = tr_->sort_mask[tr_cursor_++] | must;
         k_guard_sort<<<1, 256, 0, s>>>(b.ghist, n, mask, &b.counters[15]);
     }
     else
@@ -3179,6 +3205,14 @@ Status Engine::run(const double* d_pts, size_t n)
 
 Status Engine::run_batch(const double* d_pts, const size_t* set_off, int k)
 {
+    const Status s = run_batch_impl(d_pts, set_off, k);
+    st_.cell_points_max = (uint32_t)cell_max_;
+    st_.merge_lanes = (uint32_t)merge_lanes_;
+    return s;
+}
+
+Status Engine::run_batch_impl(const double* d_pts, const size_t* set_off, int k)
+{
     res_ = DeviceResult();
     batch_ = BatchResult();
     st_ = RunStats();
@@ -3265,6 +3299,17 @@ Status Engine::run_batch(const double* d_pts, const size_t* set_off, int k)
     return Status::ok;
 }
 
+// Ends the capture of a replay's first graph and begins its second.
+Status Engine::split_capture()
+{
+    if (!cap_) return Status::internal;
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(stream_, &g) != hipSuccess || !g) return Status::internal;
+    cap_->graph = g;
+    if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) return Status::internal;
+    return Status::ok;
+}
+
 // The batch pipeline of a recorded shape, captured from the engine's stream
 // into a graph.  The first node copies the caller's points into the
 // engine's own input buffer (its source is reset before every launch), so
@@ -3298,6 +3343,7 @@ Status Engine::capture(GraphEntry* e, const double* d_pts, const size_t* set_off
     mode_ = Mode::replay;
     tr_ = &t;
     tr_cursor_ = 0;
+    cap_ = e;
     hipStream_t s = stream_;
     Status st = Status::ok;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -3310,11 +3356,13 @@ Status Engine::capture(GraphEntry* e, const double* d_pts, const size_t* set_off
         hipGraph_t graph = nullptr;
         const hipError_t ee = hipStreamEndCapture(s, &graph);
         if (ee != hipSuccess || !graph) st = Status::internal;
-        e->graph = graph;
+        e->graph2 = graph;
     }
     mode_ = Mode::eager;
     tr_ = nullptr;
+    cap_ = nullptr;
     if (st != Status::ok) return st;
+    if (!e->graph || !e->graph2) return Status::internal;
     if (tr_cursor_ != t.sort_mask.size()) return Status::internal; // the replay took another path
     // the input copy is the graph's only root
     size_t nroot = 0;
@@ -3325,6 +3373,11 @@ Status Engine::capture(GraphEntry* e, const double* d_pts, const size_t* set_off
     if (hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0) != hipSuccess)
     {
         e->exec = nullptr;
+        return Status::internal;
+    }
+    if (hipGraphInstantiate(&e->exec2, e->graph2, nullptr, nullptr, 0) != hipSuccess)
+    {
+        e->exec2 = nullptr;
         return Status::internal;
     }
     e->in_src = d_pts;
@@ -3346,11 +3399,17 @@ Status Engine::replay_graph(GraphEntry* g, const double* d_pts, const size_t* se
                                                   hipMemcpyDeviceToDevice));
         g->in_src = d_pts;
     }
-    QDT_HIP(hipEventRecord(ev0_, s));
-    QDT_HIP(hipGraphLaunch(g->exec, s));
-    QDT_HIP(hipEventRecord(ev1_, s));
     uint32_t* H = g->rb;
     const size_t K1 = (size_t)k + 1;
+    QDT_HIP(hipEventRecord(ev0_, s));
+    QDT_HIP(hipGraphLaunch(g->exec, s));
+    // the guards of the first graph decide whether the recorded launches of
+    // the second apply to this batch
+    QDT_HIP(hipMemcpyAsync(H, &b.counters[15], 8, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    if (H[0] | H[1]) return Status::ok; // ok = false: run eagerly
+    QDT_HIP(hipGraphLaunch(g->exec2, s));
+    QDT_HIP(hipEventRecord(ev1_, s));
     QDT_HIP(hipMemcpyAsync(H, b.rblock, 256, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipMemcpyAsync(H + 64, b.toff, 4 * K1, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
@@ -3602,6 +3661,9 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             k_guard_words<<<cdiv((size_t)k + 1, kBlock), kBlock, 0, s>>>(b.vstart, b.sstart, (uint32_t)k + 1,
                                                                          guard, 16);
         }
+        // everything after this is sized by the recorded counts: it goes
+        // into the second graph, launched only when these guards hold
+        QDT_TRY(split_capture());
     }
     else
     {
@@ -3706,7 +3768,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         if (multi)
         {
             k_rank_pairs<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.Bt, b.rin, n, kA, vA);
-            st = sort_keys(kA, vA, kB, vB, n);
+            st = sort_keys(kA, vA, kB, vB, n, passes_upto((uint64_t)k));
             if (st != Status::ok) return st;
             perm = vA;
             w = reinterpret_cast<uint32_t*>(kA);
@@ -3721,12 +3783,12 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     else
     {
         k_frame1_keys<<<cdiv(m, kBlock), kBlock, 0, s>>>(b.P0, m, b.kA, b.vA);
-        st = sort_keys(kA, vA, kB, vB, m);
+        st = sort_keys(kA, vA, kB, vB, m, 0xffu);
         if (st != Status::ok) return st;
         if (multi)
         {
             k_gather_rank<<<cdiv(m, kBlock), kBlock, 0, s>>>(vA, b.vrank, m, kA);
-            st = sort_keys(kA, vA, kB, vB, m);
+            st = sort_keys(kA, vA, kB, vB, m, passes_upto((uint64_t)k));
             if (st != Status::ok) return st;
         }
     }
@@ -3978,7 +4040,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     scan_u32(FaceCount{ b.face }, nrec, FacePrefix{ b.face }, b.partials, s);
     k_face_write<<<cdiv(nrec, kBlock), kBlock, 0, s>>>(b.face, b.NX, b.OR, b.ID0, nrec, m, lim_e,
                                                         multi ? 3 * m : 0xffffffffu, multi ? b.tmain : b.partials,
-                                                        b.tris);
+                                                        b.tris, (uint32_t)(2 * cap_n_), b.err);
     mark(8);
     const uint32_t nbx = cdiv(nrec, kScanTile);
     if (replay)
@@ -4108,7 +4170,7 @@ Status Engine::batch_output(const uint32_t* dT, uint32_t T, int k, int kg, const
     uint32_t *vA = b.vA, *vB = b.vB;
     k_spare_keys<<<cdiv(std::max<uint32_t>(c_bound, 1), kBlock), kBlock, 0, s>>>(b.trank, dT, b.tmain, c_bound,
                                                                                  (uint32_t)kg, kA, vA);
-    QDT_TRY(sort_keys(kA, vA, kB, vB, c_bound));
+    QDT_TRY(sort_keys(kA, vA, kB, vB, c_bound, passes_upto((uint64_t)kg)));
     k_rank_bounds<uint64_t><<<cdiv(K1, kBlock), kBlock, 0, s>>>(kA, nullptr, c_bound, kg, b.sstart);
     k_tri_offsets<<<1, 1024, 0, s>>>(b.mstart, b.sstart, kg, b.toff);
     k_batch_tris<<<cdiv(std::max<uint32_t>(T_bound, 1), kBlock), kBlock, 0, s>>>(

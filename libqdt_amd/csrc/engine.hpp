@@ -210,6 +210,7 @@ private:
     Status finish_batch(const uint32_t* toff, uint32_t T, int k, int kg, const std::vector<int>& set_rank,
                         const std::vector<uint32_t>& vstart);
     Status run_batch_eager(const double* d_pts, const size_t* set_off, int k);
+    Status run_batch_impl(const double* d_pts, const size_t* set_off, int k);
 
     // Graph capture of the batched form.  A batch of a new shape runs
     // eagerly in record mode, which keeps every value a host decision was
@@ -233,10 +234,12 @@ private:
     uint64_t ws_gen_ = 0; // bumped whenever a region is carved (cached graphs hold its pointers)
     int cell_max_ = 256, merge_lanes_ = 16;
     void drop_graphs();
-    Status sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n);
+    Status sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t must = 0);
     Status upload(void* dst, const void* src, size_t bytes);
     Status replay_graph(GraphEntry* g, const double* d_pts, const size_t* set_off, int k, bool& ok);
     Status capture(GraphEntry* e, const double* d_pts, const size_t* set_off, int k);
+    Status split_capture();
+    GraphEntry* cap_ = nullptr; // the entry being captured
     char* ws_gin_ = nullptr; // the graphs' input buffer
     size_t ws_gin_bytes_ = 0;
     double* gin_ = nullptr;
