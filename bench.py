@@ -202,14 +202,29 @@ def bench_batched(args, Q, W, np, torch):
     n = len(pts)
     dev = torch.from_numpy(pts).to("cuda:0")
     torch.cuda.synchronize()
+    # the same sets without graphs (one eager pass per call), tuned alike
+    b0 = Q.Batch()
+    b0.set_graphs(False)
+    assert b0.execute_device(dev.data_ptr(), offs) == Q.QDT_OK
+    te = []
+    for _ in range(max(args.steps // 2, 2)):
+        t0 = time.perf_counter()
+        assert b0.execute_device(dev.data_ptr(), offs) == Q.QDT_OK
+        te.append(time.perf_counter() - t0)
+    t_eager = statistics.median(te)
+    b0.close()
+    # the batched form as shipped: autotuned, the first call captures the
+    # shape's graphs, the timed calls replay them
     b = Q.Batch()
-    for _ in range(1):
+    for _ in range(2):
         assert b.execute_device(dev.data_ptr(), offs) == Q.QDT_OK
     ts = []
+    replays = 0
     for _ in range(max(args.steps // 2, 2)):
         t0 = time.perf_counter()
         assert b.execute_device(dev.data_ptr(), offs) == Q.QDT_OK
         ts.append(time.perf_counter() - t0)
+        replays += int(b.stats()["graph_replay"])
     t_batch = statistics.median(ts)
     tri_total = int(b.all_triangles().shape[0])
     bst = b.stats()
@@ -238,6 +253,11 @@ def bench_batched(args, Q, W, np, torch):
         "one_call_per_set": {"sets_per_s": round(len(sets) / t_single, 1),
                              "mpts_per_s": round(n / t_single / 1e6, 3), "ms": round(t_single * 1e3, 1)},
         "speedup_vs_one_call_per_set": round(t_single / t_batch, 2),
+        "graph": {"timed_calls_replayed": replays, "timed_calls": len(ts),
+                  "launch": "two HIP graph launches per call (the second after the first's guards are read)",
+                  "ms_per_batch_without_graphs": round(t_eager * 1e3, 3)},
+        "tuning": {"cell_points": int(bst["cell_points"]), "merge_lanes": int(bst["merge_lanes"]),
+                   "source": ["default", "timed", "cache", "environment"][int(bst["tune_source"])]},
         "device_ms": round(bst["device_ms"], 3),
         "pipeline_alg_bytes": alg_b,
         "pipeline_frac": alg_b / (bst["device_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS if bst["device_ms"] else None,
