@@ -272,8 +272,9 @@ QDT_API void qdt_gpu_batch_set_tuning(qdt_gpu_batch* b, int cell_points, int mer
  * No counterpart in the reference.  Requests (host point sets) can be
  * submitted at any time from any thread.  `workers` threads, each with its
  * own HIP stream and resident device workspace, take every queued request
- * that fits beside the running batches within hbm_budget_bytes (about
- * 320 bytes per point are counted) and triangulate them together in one
+ * that fits beside the running batches within hbm_budget_bytes (each
+ * worker is charged its engine's exact workspace for the batch plus its input
+ * buffer) and triangulate them together in one
  * batch, so requests that arrive while others run are admitted as memory
  * frees up and their batches overlap on the device.  A request's result is
  * the same as qdt_execute gives for it alone. */
