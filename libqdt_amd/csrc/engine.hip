@@ -726,7 +726,9 @@ __global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const 
 // One merge per lane on the first kMergeLanes lanes of each wavefront: the
 // merges are diverging chains of dependent loads, and a level has only a few
 // thousand of them, so fewer per wavefront spread them over more SIMDs (16,
-// 32 or 64 per wavefront; tuned per workload).
+// 32 or 64 per wavefront, tuned per workload; 8 or 4 on levels too small to
+// give each SIMD two wavefronts).
+constexpr uint32_t kMergeWavesMin = 2048;
 template <uint32_t kMergeLanes>
 __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const double2* PT, int* VE, const int* off,
                                                        int d, uint32_t m, int lim_e, int4* res, int* err,
@@ -803,11 +805,19 @@ __device__ __forceinline__ bool cell_node(const int* off, int dc, int D, uint32_
     return false;
 }
 
+// Node slots of a cell's sub-levels (heap order): the partition of s points
+// stops below the level whose nodes all hold fewer than four, so the deepest
+// level L has 2^L < 2s/3 nodes and levels 0..L take 2^(L+1) - 1 < 4s/3 slots
+// (checked for every s <= cap <= 512 on the host; the partition loop still
+// checks it).
+__host__ __device__ constexpr int cell_nodes(int cap) { return (4 * cap + 2) / 3 + 2; }
+
 // LDS of a cell of at most cap points (bytes), laid out by CellLds
 __host__ __device__ constexpr size_t cell_lds_bytes(int cap)
 {
-    return 16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 4 * (size_t)cap +
-        2 * (size_t)(2 * cap + kCellLevels + 2) + 8 * (size_t)(2 * cap) + 64;
+    return ((16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 4 * (size_t)cap +
+             2 * (size_t)(cell_nodes(cap) + kCellLevels + 2) + 7) & ~(size_t)7) +
+        8 * (size_t)cell_nodes(cap) + 64;
 }
 
 struct CellLds
@@ -818,11 +828,11 @@ struct CellLds
     short *A0, *A1; // the cell's positions in frame-0 / frame-1 order (after the
                     // merges: an int per vertex, its lowest out-going edge)
     short* loff;  // sub-level node offsets
-    short4* lres; // sub-level node results (2 cap slots)
+    short4* lres; // sub-level node results (cell_nodes(cap) slots)
     // partition scratch, inside the record area (unused until the leaves)
     short *T, *Z;
     uint8_t* right;
-    int* gid;     // global vertex ids (load phase)
+    uint32_t* rk; // frame-1 ranks of the cell's points (load phase)
     __device__ CellLds(unsigned char* lds, int cap)
     {
         P = (double2*)lds;
@@ -832,22 +842,24 @@ struct CellLds
         A1 = A0 + cap;
         loff = A1 + cap;
         lres = (short4*)(lds + ((16 * (size_t)cap + 24 * (size_t)cap + 12 * (size_t)cap + 4 * (size_t)cap +
-                                 2 * (size_t)(2 * cap + kCellLevels + 2) + 7) & ~(size_t)7));
+                                 2 * (size_t)(cell_nodes(cap) + kCellLevels + 2) + 7) & ~(size_t)7));
         unsigned char* x = (unsigned char*)LNX; // 24 cap bytes
         T = (short*)x;
         Z = (short*)(x + 2 * (size_t)cap);
         right = x + 4 * (size_t)cap + 4;
-        gid = (int*)(x + 8 * (size_t)cap);
+        rk = (uint32_t*)(x + 8 * (size_t)cap);
     }
 };
 
 // One cell: returns false (and writes nothing) when Q lacks the exact stage
 // and some filter could not decide.
 // kCellPer: positions per lane in the cell's scans (cap <= 64 kCellPer)
+// B0: the frame-0 array (vertex ids, which after the renumbering are its
+// positions); rank1: beside it, each entry's frame-1 rank.
 template <class Q, int kCellPer>
 __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, int* OR, const double2* PT, int* VE,
-                           const uint32_t* B0, const uint32_t* B1, const uint32_t* newid, const int* off, int dc,
-                           int D, int4* res, int* err, unsigned long long* counters)
+                           const uint32_t* B0, const uint32_t* rank1, const int* off, int dc, int D, int4* res,
+                           int* err, unsigned long long* counters)
 {
     __shared__ int lbase[kCellLevels + 1]; // first offset of each sub-level in loff
     __shared__ int bad;
@@ -855,9 +867,9 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
     double2* P = W.P;
     short *A0 = W.A0, *A1 = W.A1, *T = W.T, *Z = W.Z, *loff = W.loff, *LNX = W.LNX, *LOR = W.LOR;
     uint8_t* right = W.right;
-    int* gid = W.gid;
+    uint32_t* rk = W.rk;
     short4* lres = W.lres;
-    const int kNodes = 2 * cap;
+    const int kNodes = cell_nodes(cap);
 
     const int lane = threadIdx.x;
     int dl;
@@ -876,9 +888,9 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
     for (int i = lane; i < s; i += 64)
     {
         const int v = (int)B0[o + i];
-        gid[i] = v;
         P[i] = PT[v];
         A0[i] = (short)i;
+        rk[i] = rank1[o + i];
     }
     if (lane == 0)
     {
@@ -888,13 +900,31 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
         loff[1] = (short)s;
     }
     __syncthreads();
-    // frame 1 holds the ids from before the renumbering: newid maps them to
-    // positions in frame 0, which are the cell's ids
-    for (int i = lane; i < s; i += 64)
+    // frame 1: the cell's points ordered by their frame-1 ranks (distinct),
+    // each placed at the count of smaller ranks in the cell (read from LDS,
+    // no gather from HBM)
     {
-        const int lo = (int)newid[B1[o + i]] - o;
-        if (lo < 0 || lo >= s || gid[lo] != o + lo) bad = 1;
-        A1[i] = (short)(lo < 0 || lo >= s ? 0 : lo);
+        uint32_t kk[kCellPer];
+        int rr[kCellPer];
+#pragma unroll
+        for (int r = 0; r < kCellPer; ++r)
+        {
+            const int i = lane + 64 * r;
+            kk[r] = i < s ? rk[i] : 0xffffffffu;
+            rr[r] = 0;
+        }
+        for (int jx = 0; jx < s; ++jx)
+        {
+            const uint32_t kj = rk[jx];
+#pragma unroll
+            for (int r = 0; r < kCellPer; ++r) rr[r] += kj < kk[r] ? 1 : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < kCellPer; ++r)
+        {
+            const int i = lane + 64 * r;
+            if (i < s) A1[rr[r]] = (short)i;
+        }
     }
     // the cell's extent: every coordinate difference of two of its points is
     // at most wx in x and wy in y, which fixes the filters' bounds for the cell
@@ -1144,12 +1174,11 @@ __device__ bool cell_build(unsigned char* lds, int cap, uint32_t cell, int* NX, 
 // every cell, filters only; undecided cells go to redo (count, then ids)
 template <int kCellPer>
 __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
-                                             const uint32_t* B1, const uint32_t* newid, const int* off, int dc, int D,
-                                             int cap, int4* res,
-                                             int* redo, int* err, unsigned long long* counters)
+                                             const uint32_t* rank1, const int* off, int dc, int D, int cap,
+                                             int4* res, int* redo, int* err, unsigned long long* counters)
 {
     extern __shared__ __align__(16) unsigned char cell_lds[];
-    if (!cell_build<LQE, kCellPer>(cell_lds, cap, blockIdx.x, NX, OR, PT, VE, B0, B1, newid, off, dc, D, res, err,
+    if (!cell_build<LQE, kCellPer>(cell_lds, cap, blockIdx.x, NX, OR, PT, VE, B0, rank1, off, dc, D, res, err,
                                    counters) &&
         threadIdx.x == 0)
     {
@@ -1161,8 +1190,7 @@ __global__ __launch_bounds__(64) void k_cell(int* NX, int* OR, const double2* PT
 // the listed cells again, with the exact predicates
 template <int kCellPer>
 __global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const double2* PT, int* VE, const uint32_t* B0,
-                                                   const uint32_t* B1, const uint32_t* newid, const int* off, int dc,
-                                                   int D, int cap,
+                                                   const uint32_t* rank1, const int* off, int dc, int D, int cap,
                                                    int4* res, const int* redo, int* err,
                                                    unsigned long long* counters)
 {
@@ -1170,7 +1198,7 @@ __global__ __launch_bounds__(64) void k_cell_exact(int* NX, int* OR, const doubl
     const int n = redo[0];
     for (int i = blockIdx.x; i < n; i += gridDim.x)
     {
-        cell_build<LQX, kCellPer>(cell_lds, cap, (uint32_t)redo[1 + i], NX, OR, PT, VE, B0, B1, newid, off, dc, D,
+        cell_build<LQX, kCellPer>(cell_lds, cap, (uint32_t)redo[1 + i], NX, OR, PT, VE, B0, rank1, off, dc, D,
                                   res, err, counters);
         __syncthreads();
     }
@@ -3942,11 +3970,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             // positions per lane of the cells' scans: the smallest that holds cap
             auto cell = cap <= 128 ? k_cell<2> : (cap <= 256 ? k_cell<4> : k_cell<8>);
             auto cell_x = cap <= 128 ? k_cell_exact<2> : (cap <= 256 ? k_cell_exact<4> : k_cell_exact<8>);
-            cell<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, newid, b.off, dc, D, cap, b.res, b.redo,
-                                        b.err, b.counters);
+            cell<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, R0, b.off, dc, D, cap, b.res, b.redo, b.err,
+                                        b.counters);
             // one workgroup per listed cell (the rest end at once): on
             // lattices nearly every cell is listed
-            cell_x<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, B1, newid, b.off, dc, D, cap, b.res, b.redo,
+            cell_x<<<ncell, 64, lds, s>>>(b.NX, b.OR, b.P0, b.VE, B0, R0, b.off, dc, D, cap, b.res, b.redo,
                                           b.err, b.counters);
         }
     }
@@ -3968,8 +3996,14 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             ;
         else if (!K)
         {
-            const uint32_t lanes = (uint32_t)merge_lanes_;
-            auto kern = lanes >= 64 ? k_merge<64> : (lanes >= 32 ? k_merge<32> : k_merge<16>);
+            // the tuned merges per wavefront, fewer on levels whose merges
+            // would not give every SIMD two wavefronts (10M uniform: the
+            // three levels 2.35 -> 2.16 ms)
+            uint32_t lanes = (uint32_t)merge_lanes_;
+            while (lanes > 4 && cdiv(cnt, lanes) < kMergeWavesMin) lanes >>= 1;
+            auto kern = lanes >= 64 ? k_merge<64>
+                                    : (lanes >= 32 ? k_merge<32>
+                                                   : (lanes >= 16 ? k_merge<16> : (lanes >= 8 ? k_merge<8> : k_merge<4>)));
             kern<<<cdiv(cnt, lanes), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res, b.err,
                                                           b.counters);
         }
