@@ -684,6 +684,28 @@ __global__ __launch_bounds__(kBlock) void k_renum_ids(const uint32_t* newid, uin
 
 using GQE = QuadEdges<int*, const double2*>;
 
+// A merged node's result as its parent needs it.  k_seg_final stores
+// (-(ldo + 2), rdo, pool) when the hull edges still have to be re-walked to
+// the parent's frame; the re-walk (builder.hxx:340-346) is then done by the
+// first reader (the anchor kernel's tangent lane or k_merge), whose launch
+// hides its latency, and written back.  c: the child's index at level dc.
+__device__ __forceinline__ int4 child_result(GQE& q, int4* res, const int* off, int dc, uint32_t c)
+{
+    const size_t ni = node_index(off, dc, c);
+    int4 r = res[ni];
+    if (r.x <= -2)
+    {
+        const int* od = off + off_base(off, dc);
+        const int sz = od[c + 1] - od[c];
+        int first, second;
+        q.rewalk(-r.x - 2, r.y, dc & 1, (dc - 1) & 1, sz, first, second);
+        r.x = first;
+        r.y = second;
+        res[ni] = r;
+    }
+    return r;
+}
+
 __global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const double2* PT, int* VE,
                                                         const uint32_t* B0, const uint32_t* B1, const int* off, int D,
                                                         uint32_t m, int lim_e, int4* res, int* err,
@@ -743,8 +765,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
         const int s = od[j + 1] - od[j];
         if (s >= 4)
         {
-            const int4 L = res[node_index(off, d + 1, 2 * j)];
-            const int4 R = res[node_index(off, d + 1, 2 * j + 1)];
+            const int4 L = child_result(q, res, off, d + 1, 2 * j);
+            const int4 R = child_result(q, res, off, d + 1, 2 * j + 1);
             Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
             const int axis = d & 1;
             const int out_axis = d > 0 ? ((d - 1) & 1) : 0;
@@ -1340,7 +1362,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
                                                             const uint32_t* newid, const int* kn,
                                                             int D, int d, int K, uint32_t m,
                                                             int lim_e, int* arec, double4* acoord, double* akey,
-                                                            int* astate, const int4* res, int4* nstate, int* err,
+                                                            int* astate, int4* res, int4* nstate, int* err,
                                                             unsigned long long* counters)
 {
     const uint32_t lane2 = blockIdx.x * kMergeBlock + threadIdx.x;
@@ -1357,8 +1379,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
         const int s = od[j + 1] - od[j];
         if (s >= 4)
         {
-            const int4 L = res[node_index(off, d + 1, 2 * j)];
-            const int4 R = res[node_index(off, d + 1, 2 * j + 1)];
+            const int4 L = child_result(q, res, off, d + 1, 2 * j);
+            const int4 R = child_result(q, res, off, d + 1, 2 * j + 1);
             int ldi = L.y, rdi = R.x;
             int budget = 16 * s + 64;
             q.tangent_walk(ldi, rdi, budget);
@@ -2139,13 +2161,11 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_final(int* NX, int* OR, con
             atomicOr(err, 2 << 1);
     }
     if (k != 0) return;
-    GQE q;
-    q.init(NX, OR, PT, VE, lim_e, (int)m);
     const int head = total ? lists[0].x : -1, tail = total ? lists[total - 1].y : -1;
-    int first, second;
-    q.rewalk(ns.x, ns.y, d & 1, d > 0 ? ((d - 1) & 1) : 0, s, first, second);
-    res[node_index(off, d, j)] = make_int4(first, second, head, tail);
-    if (q.err) atomicOr(err, 2 << q.err);
+    // the hull edges in this level's frame; below the root the parent's frame
+    // differs and its first reader re-walks them (child_result)
+    const bool rewalk = d > 0 && ns.x >= 0;
+    res[node_index(off, d, j)] = make_int4(rewalk ? -(ns.x + 2) : ns.x, ns.y, head, tail);
 }
 
 
