@@ -1353,9 +1353,10 @@ constexpr int kMarkSlotsMax = 4096;            // anchors per level up to which 
 // lanes take depends only on values they exchange, so a pair never diverges
 // and its shuffles always find the partner active; the two walks, ring scans
 // and dry runs of an anchor overlap their memory latencies.  One more thread
-// per node, after all the anchors' lanes, walks the node's lower common
-// tangent (reads only, like the dry runs; its result goes to nstate, and
-// k_seg_tangent makes the connect), so that walk overlaps the anchors.
+// per node, after all the anchors' lanes, re-walks the children's hull
+// edges to this level's frame (child_result) and walks the node's lower
+// common tangent (reads only, like the dry runs; its result goes to nstate,
+// and k_seg_select makes the connect), so those walks overlap the anchors.
 __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, const double2* PT, int* VE,
                                                             const int* off, const int* splitv, const int* qv,
                                                             const uint32_t* B0, const uint32_t* B1,
@@ -1477,39 +1478,6 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
     flush_counts(counters, q.pc);
 }
 
-// lower common tangent of every segmented node (the first cut): the connect
-// of the walk that k_seg_anchor made
-__global__ __launch_bounds__(kMergeBlock) void k_seg_tangent(int* NX, int* OR, const double2* PT, int* VE,
-                                                             const int* off, int d, uint32_t m, int lim_e,
-                                                             unsigned* spare, int spare_base, int spare_cap,
-                                                             const int4* res, int* segstart, int4* nstate, int* err,
-                                                             unsigned long long* counters)
-{
-    const uint32_t j = blockIdx.x * kMergeBlock + threadIdx.x;
-    GQE q;
-    q.init(NX, OR, PT, VE, lim_e, (int)m);
-    q.spare = spare;
-    q.spare_base = spare_base;
-    q.spare_cap = spare_cap;
-    if (j < lcount(off, d))
-    {
-        const int* od = off + off_base(off, d);
-        const int s = od[j + 1] - od[j];
-        if (s >= 4)
-        {
-            const int4 L = res[node_index(off, d + 1, 2 * j)];
-            const int4 R = res[node_index(off, d + 1, 2 * j + 1)];
-            Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
-            int ldo = L.x, rdo = R.y;
-            const int4 w = nstate[j]; // the tangent's walk, done by k_seg_anchor
-            segstart[(size_t)j * kSegSlots] = q.tangent_connect(ldo, w.x, w.y, rdo, pool);
-            nstate[j] = make_int4(ldo, rdo, pool.head, pool.tail);
-        }
-    }
-    flush_counts(counters, q.pc);
-    if (q.err) atomicOr(err, 2 << q.err);
-}
-
 // Anchors in chain order, each strictly above the previous cut, and no two
 // cuts with a common end vertex.  One workgroup per node (a power of two of
 // threads, at least the node's candidate count): the candidates and their
@@ -1583,11 +1551,13 @@ __device__ __forceinline__ int sel_scan(int v, int* wtmp)
     return kMax ? max(pre, v) : pre + v;
 }
 
-__global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const double2* PT, const int* off, int d,
-                                                         int K, uint32_t m, int lim_e, const int* segstart,
-                                                         const int* arec, const double4* acoord,
-                                                         const double* akey, int* astate, int* selk, int* nsel,
-                                                         int* alist_n, unsigned long long* counters)
+__global__ __launch_bounds__(kSegMaxK) void k_seg_select(int* NX, int* OR, const double2* PT, int* VE, const int* off,
+                                                         int d, int K, uint32_t m, int lim_e, unsigned* spare,
+                                                         int spare_base, int spare_cap, const int4* res,
+                                                         int4* nstate, int* segstart, const int* arec,
+                                                         const double4* acoord, const double* akey, int* astate,
+                                                         int* selk, int* nsel, int* alist_n, int* err,
+                                                         unsigned long long* counters)
 {
     __shared__ double skey[kSegMaxK];
     __shared__ int sslot[kSegMaxK];
@@ -1604,6 +1574,26 @@ __global__ __launch_bounds__(kSegMaxK) void k_seg_select(const int* OR, const do
     const int s = od[j + 1] - od[j];
     if (k == 0) nsel[j] = 0;
     if (s < 4) return;
+    if (k == 0)
+    {
+        // the lower common tangent (walked by the anchor kernel) connected:
+        // the first cross edge, and the parent's hull edges and pool
+        GQE q;
+        q.init(NX, OR, PT, VE, lim_e, (int)m);
+        q.spare = spare;
+        q.spare_base = spare_base;
+        q.spare_cap = spare_cap;
+        const int4 L = res[node_index(off, d + 1, 2 * j)];
+        const int4 R = res[node_index(off, d + 1, 2 * j + 1)];
+        Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
+        int ldo = L.x, rdo = R.y;
+        const int4 w = nstate[j];
+        segstart[(size_t)j * kSegSlots] = q.tangent_connect(ldo, w.x, w.y, rdo, pool);
+        nstate[j] = make_int4(ldo, rdo, pool.head, pool.tail);
+        flush_counts(counters, q.pc);
+        if (q.err) atomicOr(err, 2 << q.err);
+    }
+    __syncthreads();
     const int b0 = segstart[(size_t)j * kSegSlots];
     if (b0 < 0 || b0 >= lim_e) return; // the tangent failed: the run's error flag is set
     const int pu0 = OR[(b0 ^ 2) >> 1], pv0 = OR[b0 >> 1];
@@ -4034,14 +4024,12 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                 b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, newid, b.kn, Dk, d, K, m, lim_e, b.arec, b.acoord,
                 b.akey,
                 b.astate, b.res, b.nstate, b.err, b.counters);
-            k_seg_tangent<<<cdiv(cnt, kMergeBlock), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e,
-                                                                          b.spare, spare_base, spare_cap, b.res,
-                                                                          b.segstart, b.nstate, b.err, b.counters);
             int sel_threads = 64;
             while (sel_threads < K) sel_threads <<= 1;
-            k_seg_select<<<cnt, sel_threads, 0, s>>>(b.OR, b.P0, b.off, d, K, m, lim_e, b.segstart,
-                                                                         b.arec, b.acoord, b.akey, b.astate, b.selk,
-                                                                         b.nsel, b.alist_n, b.counters);
+            k_seg_select<<<cnt, sel_threads, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m, lim_e, b.spare,
+                                                     spare_base, spare_cap, b.res, b.nstate, b.segstart, b.arec,
+                                                     b.acoord, b.akey, b.astate, b.selk, b.nsel, b.alist_n, b.err,
+                                                     b.counters);
             const int acap = (int)(b.anchor_cap * (2 + 4 * kCross));
             const uint32_t ga = cdiv((size_t)total, kMergeBlock);
             k_seg_mark<<<ga, kMergeBlock, 0, s>>>(b.OR, total, b.arec, b.astate, b.spare, spare_base, spare_cap,
