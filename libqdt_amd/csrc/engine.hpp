@@ -205,6 +205,7 @@ private:
     Status ensure_anchors(size_t seg_nodes, size_t anchor_cap);
     Status ensure_sets(size_t k, size_t n);
     Status run_impl(const double* d_pts, size_t n, const size_t* set_off, int k, bool seams, bool& redo);
+    Status run_small(const double* d_pts, size_t n);
     Status batch_output(const uint32_t* dT, uint32_t T, int k, int kg, const std::vector<int>& set_rank,
                         const std::vector<uint32_t>& vstart);
     Status finish_batch(const uint32_t* toff, uint32_t T, int k, int kg, const std::vector<int>& set_rank,
