@@ -3142,13 +3142,12 @@ size_t Engine::bytes_after(size_t n, size_t k) const
     size_t ws = ws_bytes_;
     if (n > cap_n_ || !ws_)
     {
-        const size_t nb_sort = cdiv(N, kSortTile);
         const size_t R = 3 * N + N / 2 + 65536;
         const size_t nb_scan = cdiv(R, kScanTile) + 1;
         const size_t qcap = std::max<size_t>(4096, N / 4);
         const size_t per[] = { 8 * N, 8 * N, 8 * N, 8 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N,
                                4 * N, 4 * N, 4 * N, 4 * N, 4 * 6 * N, 4 * R, 4 * qcap,
-                               4 * (nb_scan + 1 + 256 * nb_sort), 4 * 256 * nb_sort, 4 * 8 * 256, 4 * N, 4 * N,
+                               4 * (nb_scan + 1), 4 * radix_state_words(N), 4 * 8 * 256, 4 * N, 4 * N,
                                16 * N, 16 * N, N, 4 * 4 * R, 4 * 2 * R, 8 * 16, 64, 4 * N, 4 * N, 64, 64, 256 };
         ws = 0;
         for (size_t b : per) ws += round256(b);
@@ -3177,19 +3176,27 @@ size_t Engine::alloc_budget() const
     return mem_cap_ > held ? mem_cap_ - held : 0;
 }
 
+constexpr uint32_t kSortErr = 1u << 23; // err bit: a radix pass's look-back gave up
+
 // The radix passes of a sort (bit p of mask: pass p runs); the result ends
-// in kA/vA.
+// in kA/vA.  ghist: the keys' digit histograms (k_radix_global_hist); state:
+// radix_state_words(n) words of look-back state; a look-back that gives up
+// sets err_bit in *err.
 static void sort_passes(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t mask,
-                        uint32_t* hist, uint32_t* partials, hipStream_t s)
+                        const uint32_t* ghist, uint32_t* state, uint32_t* err, uint32_t err_bit, hipStream_t s)
 {
+    if (!mask) return;
     const uint32_t nb = cdiv(n, kSortTile);
+    const size_t half = radix_half_words(n);
+    // the first pass's half; each pass clears the other for the next
+    fill_words(state, half, 0, s);
+    int h = 0;
     for (int pass = 0; pass < 8; ++pass)
     {
         if (!(mask >> pass & 1u)) continue;
-        const int shift = 8 * pass;
-        k_radix_hist<<<nb, kBlock, 0, s>>>(kA, n, shift, hist);
-        scan_u32(LoadU32{ hist }, 256 * nb, StoreU32{ hist }, partials, s);
-        k_radix_scatter<<<nb, kBlock, 0, s>>>(kA, vA, kB, vB, n, shift, hist);
+        k_radix_pass<<<nb, kBlock, 0, s>>>(kA, vA, kB, vB, n, 8 * pass, ghist, state + h * half,
+                                           state + (1 - h) * half, err, err_bit);
+        h ^= 1;
         std::swap(kA, kB);
         std::swap(vA, vB);
     }
@@ -3255,17 +3262,25 @@ __global__ void k_guard_flag(const unsigned long long* got, int want_nonzero, un
     if (threadIdx.x == 0 && (*got != 0) != (want_nonzero != 0)) atomicOr(guard, bit);
 }
 
-// Sorts (k, v) pairs stably by key; the result ends in kA/vA.
-static Status sort_pairs(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t* hist,
-                         uint32_t* partials, uint32_t* ghist, uint32_t* host_ghist, hipStream_t s)
+// Sorts (k, v) pairs stably by key; the result ends in kA/vA.  state:
+// radix_state_words(n) words (its second word reports a look-back that gave
+// up, read back here).
+static Status sort_pairs(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t* state,
+                         uint32_t* ghist, uint32_t* host_ghist, hipStream_t s)
 {
     if (n <= 1) return Status::ok;
+    if (n > kSortMaxKeys) return Status::internal;
     fill_words(ghist, 8 * 256, 0, s);
     k_radix_global_hist<<<std::min<uint32_t>(cdiv(n, kBlock), 1024), kBlock, 0, s>>>(kA, n, ghist);
     QDT_HIP(hipMemcpyAsync(host_ghist, ghist, 4 * 8 * 256, hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
-    sort_passes(kA, vA, kB, vB, n, pass_mask(host_ghist, n), hist, partials, s);
-    return Status::ok;
+    const uint32_t mask = pass_mask(host_ghist, n);
+    if (!mask) return Status::ok;
+    sort_passes(kA, vA, kB, vB, n, mask, ghist, state, state + 1, 1u, s);
+    uint32_t gave_up = 0;
+    QDT_HIP(hipMemcpyAsync(&gave_up, state + 1, 4, hipMemcpyDeviceToHost, s));
+    QDT_HIP(hipStreamSynchronize(s));
+    return gave_up ? Status::internal : Status::ok;
 }
 
 void Engine::set_tuning(int cell_points, int merge_lanes)
@@ -3311,6 +3326,7 @@ Status Engine::upload(void* dst, const void* src, size_t bytes)
 Status Engine::sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*& vB, uint32_t n, uint32_t must)
 {
     if (n <= 1) return Status::ok;
+    if (n > kSortMaxKeys) return Status::internal;
     Bufs& b = *b_;
     hipStream_t s = stream_;
     fill_words(b.ghist, 8 * 256, 0, s);
@@ -3332,7 +3348,7 @@ Status Engine::sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*&
         mask = pass_mask(ghist_host_, n);
         if (mode_ == Mode::record) tr_->sort_mask.push_back(mask);
     }
-    sort_passes(kA, vA, kB, vB, n, mask, b.hist, b.partials, s);
+    sort_passes(kA, vA, kB, vB, n, mask, b.ghist, b.hist, (uint32_t*)b.err, kSortErr, s);
     return Status::ok;
 }
 
@@ -3369,7 +3385,6 @@ Status Engine::ensure_ws(size_t n)
     free_region(ws_, ws_bytes_);
     cap_n_ = 0;
     const size_t N = std::max<size_t>(n, 64);
-    const uint32_t nb_sort = cdiv(N, kSortTile);
     // Seam pieces start with empty pools: each takes the records for the
     // edges it adds beyond those it deletes (about the children's hull
     // vertices near its cut) from here, while as many records stay free in
@@ -3386,8 +3401,8 @@ Status Engine::ensure_ws(size_t n)
         { (void**)&b.B0, 4 * N },       { (void**)&b.B1, 4 * N },       { (void**)&b.Bt, 4 * N },
         { (void**)&b.nodeA, 4 * N },    { (void**)&b.nodeB, 4 * N },    { (void**)&b.Z, 4 * N },
         { (void**)&b.tris, 4 * 6 * N }, { (void**)&b.face, 4 * R },     { (void**)&b.queue, 4 * qcap },
-        { (void**)&b.partials, 4 * (size_t)(nb_scan + 1 + 256 * nb_sort) },
-        { (void**)&b.hist, 4 * 256 * (size_t)nb_sort },
+        { (void**)&b.partials, 4 * (size_t)(nb_scan + 1) },
+        { (void**)&b.hist, 4 * radix_state_words(N) },
         { (void**)&b.ghist, 4 * 8 * 256 },
         { (void**)&b.vrank, 4 * N },    { (void**)&b.rin, 4 * N },
         { (void**)&b.P0, 16 * N },      { (void**)&b.P1, 16 * N },      { (void**)&b.flagp, N },         { (void**)&b.NX, 4 * 4 * R },
@@ -4758,18 +4773,16 @@ Status verify_triangulation(const double* pts_raw, uint32_t n, const uint32_t* t
     const double2* P = (const double2*)pts_raw;
     const size_t E = 3 * (size_t)T;
     if (E > 0xffffffffull) return Status::internal;
-    const uint32_t nb = cdiv(std::max<size_t>(E, 1), kSortTile);
     TempBuffers tmp;
     uint64_t *kA = nullptr, *kB = nullptr;
-    uint32_t *vA = nullptr, *vB = nullptr, *hist = nullptr, *partials = nullptr, *ghist = nullptr;
+    uint32_t *vA = nullptr, *vB = nullptr, *state = nullptr, *ghist = nullptr;
     uint8_t* used = nullptr;
     int* nextb = nullptr;
     unsigned long long* cnt = nullptr;
     Status st;
     if ((st = tmp.get(kA, E)) != Status::ok || (st = tmp.get(kB, E)) != Status::ok ||
         (st = tmp.get(vA, E)) != Status::ok || (st = tmp.get(vB, E)) != Status::ok ||
-        (st = tmp.get(hist, 256 * (size_t)nb)) != Status::ok ||
-        (st = tmp.get(partials, cdiv(256 * (size_t)nb, kScanTile) + 2 + 256 * (size_t)nb)) != Status::ok ||
+        (st = tmp.get(state, radix_state_words(E))) != Status::ok ||
         (st = tmp.get(ghist, 8 * 256)) != Status::ok || (st = tmp.get(used, n)) != Status::ok ||
         (st = tmp.get(nextb, n)) != Status::ok || (st = tmp.get(cnt, kvCount)) != Status::ok)
         return st;
@@ -4800,7 +4813,7 @@ Status verify_triangulation(const double* pts_raw, uint32_t n, const uint32_t* t
     if (T)
     {
         k_ver_tris<<<cdiv(T, kBlock), kBlock, 0, s>>>(P, n, tris, T, rep, kA, vA, used, cnt);
-        st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, hist, partials, ghist, host_ghist, s);
+        st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, state, ghist, host_ghist, s);
         if (st != Status::ok) return st;
         k_ver_edges<<<cdiv(E, kBlock), kBlock, 0, s>>>(P, tris, kA, vA, (uint32_t)E, nextb, cnt);
     }
@@ -4841,17 +4854,15 @@ Status build_mesh(const uint32_t* tris, uint32_t T, uint32_t n, int4* he, int2* 
     const size_t E = 3 * (size_t)T;
     if (E >= 0x80000000ull) return Status::internal; // half-edge ids and a direction bit share 32 bits
     out.halfedges = E;
-    const uint32_t nb = cdiv(std::max<size_t>(E, 1), kSortTile);
     TempBuffers tmp;
     uint64_t *kA = nullptr, *kB = nullptr;
-    uint32_t *vA = nullptr, *vB = nullptr, *hist = nullptr, *partials = nullptr, *ghist = nullptr, *first = nullptr;
+    uint32_t *vA = nullptr, *vB = nullptr, *state = nullptr, *partials = nullptr, *ghist = nullptr, *first = nullptr;
     unsigned long long* cnt = nullptr;
     Status st;
     if ((st = tmp.get(kA, E)) != Status::ok || (st = tmp.get(kB, E)) != Status::ok ||
         (st = tmp.get(vA, E)) != Status::ok || (st = tmp.get(vB, E)) != Status::ok ||
-        (st = tmp.get(hist, 256 * (size_t)nb)) != Status::ok ||
-        (st = tmp.get(partials, cdiv(std::max<size_t>(256 * (size_t)nb, E), kScanTile) + 2 +
-                                    std::max<size_t>(256 * (size_t)nb, E))) != Status::ok ||
+        (st = tmp.get(state, radix_state_words(E))) != Status::ok ||
+        (st = tmp.get(partials, cdiv(std::max<size_t>(E, 1), kScanTile) + 2)) != Status::ok ||
         (st = tmp.get(ghist, 8 * 256)) != Status::ok || (st = tmp.get(first, n)) != Status::ok ||
         (st = tmp.get(cnt, kmCount)) != Status::ok)
         return st;
@@ -4890,7 +4901,7 @@ Status build_mesh(const uint32_t* tris, uint32_t T, uint32_t n, int4* he, int2* 
         const uint32_t nbs = cdiv((uint32_t)E, kScanTile);
         QDT_HIP(hipMemcpyAsync(&nv, partials + nbs, 4, hipMemcpyDeviceToHost, s));
         k_mesh_fill<<<cdiv(E, kBlock), kBlock, 0, s>>>(tris, (uint32_t)E, n, orig_to_local, he);
-        st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, hist, partials, ghist, host_ghist, s);
+        st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, state, ghist, host_ghist, s);
         if (st != Status::ok) return st;
         k_mesh_twins<<<cdiv(E, kBlock), kBlock, 0, s>>>(kA, vA, (uint32_t)E, he, cnt);
     }
@@ -4914,22 +4925,20 @@ Status mesh_twins(const uint32_t* tris, uint32_t T, int* twin, hipStream_t s, ui
     const size_t E = 3 * (size_t)T;
     if (E >= 0x80000000ull) return Status::internal;
     if (!T) return Status::ok;
-    const uint32_t nb = cdiv(E, kSortTile);
     TempBuffers tmp;
     uint64_t *kA = nullptr, *kB = nullptr;
-    uint32_t *vA = nullptr, *vB = nullptr, *hist = nullptr, *partials = nullptr, *ghist = nullptr;
+    uint32_t *vA = nullptr, *vB = nullptr, *state = nullptr, *ghist = nullptr;
     unsigned long long* cnt = nullptr;
     Status st;
     if ((st = tmp.get(kA, E)) != Status::ok || (st = tmp.get(kB, E)) != Status::ok ||
         (st = tmp.get(vA, E)) != Status::ok || (st = tmp.get(vB, E)) != Status::ok ||
-        (st = tmp.get(hist, 256 * (size_t)nb)) != Status::ok ||
-        (st = tmp.get(partials, cdiv(256 * (size_t)nb, kScanTile) + 2 + 256 * (size_t)nb)) != Status::ok ||
+        (st = tmp.get(state, radix_state_words(E))) != Status::ok ||
         (st = tmp.get(ghist, 8 * 256)) != Status::ok || (st = tmp.get(cnt, 1)) != Status::ok)
         return st;
     std::vector<uint32_t> host_ghist(8 * 256);
     QDT_HIP(hipMemsetAsync(cnt, 0, 8, s));
     k_twin_keys<<<cdiv(T, kBlock), kBlock, 0, s>>>(tris, T, kA, vA, twin);
-    st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, hist, partials, ghist, host_ghist.data(), s);
+    st = sort_pairs(kA, vA, kB, vB, (uint32_t)E, state, ghist, host_ghist.data(), s);
     if (st != Status::ok) return st;
     k_twin_pairs<<<cdiv(E, kBlock), kBlock, 0, s>>>(kA, vA, (uint32_t)E, twin, cnt);
     unsigned long long c = 0;

@@ -20,8 +20,6 @@ constexpr int kBlock = 256;          // threads per block for streaming kernels
 constexpr int kWaves = kBlock / 64;  // waves per block
 constexpr int kScanItems = 8;        // items per thread in scans
 constexpr int kScanTile = kBlock * kScanItems;
-constexpr int kSortItems = 8;        // keys per lane in the radix sort
-constexpr int kSortTile = kBlock * kSortItems;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -194,39 +192,66 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(In in, uint32_t n, const u
 }
 
 // ---------------------------------------------------------------------------
-// LSD radix sort pass over (uint64 key, uint32 value) pairs, 8-bit digits.
+// LSD radix sort over (uint64 key, uint32 value) pairs, 8-bit digits, one
+// kernel per digit.  The digit histograms of all eight passes come first,
+// from one read of the keys; each pass then reads its tile once, ranks the
+// keys inside the tile, and learns where its run of each digit starts from
+// the tiles before it by a decoupled look-back: every tile publishes its
+// digit counts as soon as it has them (an aggregate) and, once it knows the
+// counts of all earlier tiles, their sum plus its own (an inclusive prefix).
+// A tile reads the words of earlier tiles backwards, adding aggregates, until
+// it reaches an inclusive prefix.  Tiles are numbered in the order their
+// workgroups start (an atomic counter), so a tile only waits for tiles whose
+// workgroups are already running and publish without waiting themselves.
 // ---------------------------------------------------------------------------
 
-// per-block digit counts, digit-major: hist[d * nblocks + b]
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* keys, uint32_t n, int shift, uint32_t* hist)
-{
-    __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t base = blockIdx.x * kSortTile;
-#pragma unroll 4
-    for (int k = 0; k < kSortItems; ++k)
-    {
-        const uint32_t i = base + k * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255], 1u);
-    }
-    __syncthreads();
-    hist[threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
-}
+constexpr int kSortItems = 16;                  // keys per lane in a radix pass
+constexpr int kSortTile = kBlock * kSortItems;  // keys per tile
+constexpr uint32_t kLbAgg = 1u << 30;           // look-back word: aggregate of one tile
+constexpr uint32_t kLbInc = 2u << 30;           // look-back word: inclusive prefix
+constexpr uint32_t kLbCount = (1u << 30) - 1;   // its count (sorts hold fewer than 2^30 keys)
+constexpr uint32_t kLbSpins = 1u << 22;         // polls of one word before the wait gives up
+constexpr size_t kSortMaxKeys = kLbCount;
 
-// all eight digit histograms of the whole array in one read (to skip passes
-// whose digit is the same for every key)
+// Look-back state of a sort: two halves (consecutive passes alternate), each
+// a tile counter padded to 64 words and then 256 digit words per tile.
+// Words: the whole state is cleared before the sort's first pass; each pass
+// clears the other half for the next.
+__host__ __device__ inline size_t radix_half_words(size_t n) { return 64 + 256 * ((n + kSortTile - 1) / kSortTile); }
+__host__ __device__ inline size_t radix_state_words(size_t n) { return 2 * radix_half_words(n); }
+
+// all eight digit histograms of the whole array in one read (the passes'
+// digit offsets, and the passes whose digit is the same for every key).
+// A digit that all active lanes of a wave share (the high bytes of keys of
+// one sign and exponent) is counted by one lane: 64 LDS atomics on one
+// address would run one after the other.
 __global__ __launch_bounds__(kBlock) void k_radix_global_hist(const uint64_t* keys, uint32_t n, uint32_t* ghist)
 {
     __shared__ uint32_t cnt[8][256];
     for (int d = 0; d < 8; ++d) cnt[d][threadIdx.x] = 0;
     __syncthreads();
     const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += stride)
     {
-        const uint64_t k = keys[i];
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < n;
+        const uint64_t k = valid ? keys[i] : 0;
+        const uint64_t act = __ballot(valid);
+        const uint32_t first = __builtin_amdgcn_readfirstlane((uint32_t)__ffsll((unsigned long long)act) - 1);
+        const uint64_t k0 = ((uint64_t)__shfl(k, first, 64));
 #pragma unroll
-        for (int d = 0; d < 8; ++d) atomicAdd(&cnt[d][(k >> (8 * d)) & 255], 1u);
+        for (int d = 0; d < 8; ++d)
+        {
+            const uint32_t dg = (uint32_t)(k >> (8 * d)) & 255;
+            const uint32_t d0 = (uint32_t)(k0 >> (8 * d)) & 255;
+            const uint64_t same = __ballot(valid && dg == d0);
+            if (same == act)
+            {
+                if (lane_id() == first) atomicAdd(&cnt[d][d0], (uint32_t)__popcll(act));
+            }
+            else if (valid)
+                atomicAdd(&cnt[d][dg], 1u);
+        }
     }
     __syncthreads();
     for (int d = 0; d < 8; ++d)
@@ -242,27 +267,41 @@ __device__ __forceinline__ uint64_t lanemask_lt()
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-// Stable scatter.  Each wave owns a contiguous run of 64*kSortItems keys and
-// walks it in 64-key steps; the 8 ballots give, for each lane, the mask of
-// lanes holding the same digit.  offsets[d * nblocks + b] is the exclusive
-// scan of the per-block histograms.  The block's keys are first placed in
-// LDS in their sorted order, so that the global writes of each digit's run
-// are contiguous (coalesced) instead of one scattered 8-byte store per key.
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
-                                                          uint32_t* vout, uint32_t n, int shift,
-                                                          const uint32_t* offsets)
+// One stable pass over digit (key >> shift) & 255, one tile of kSortTile keys
+// per workgroup; launch exactly cdiv(n, kSortTile) workgroups.  state is this
+// pass's half of the look-back state (cleared), next the other half (cleared
+// here for the next pass).  Each wave ranks its contiguous run of
+// 64*kSortItems keys in 64-key steps; 8 ballots give each lane the mask of
+// lanes holding the same digit.  The tile's keys, then its values, are
+// placed in LDS in sorted order, so each digit's run is written to HBM
+// contiguously; a position outside [0, n) (only after a look-back that gave
+// up) is not written.  A look-back
+// that polls one word kLbSpins times sets *err and goes on (the sort's
+// result is then wrong and the caller reports an internal error).
+__global__ __launch_bounds__(kBlock) void k_radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                                                       uint32_t* vout, uint32_t n, int shift, const uint32_t* ghist,
+                                                       uint32_t* state, uint32_t* next, uint32_t* err,
+                                                       uint32_t err_bit)
 {
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t bbase[256];
+    __shared__ uint32_t s_tile;
     __shared__ uint64_t skey[kSortTile];
-    __shared__ uint32_t sval[kSortTile];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / 64;
-    for (int w = 0; w < kWaves; ++w) wcnt[w][threadIdx.x] = 0;
-    const uint32_t gbase = offsets[threadIdx.x * gridDim.x + blockIdx.x];
-    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    if (t == 0) s_tile = atomicAdd(state, 1u);
+    for (int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
+    // where digit t starts in the output: the exclusive prefix of the pass's
+    // histogram (the scan's barriers also publish s_tile and wcnt)
+    uint32_t total;
+    const uint32_t gdig = block_exclusive_scan(ghist[(shift >> 3) * 256 + t], total);
+    const uint32_t tile = s_tile;
+    uint32_t* const st = state + 64;
+    next[64 + (size_t)tile * 256 + t] = 0;
+    if (tile == 0 && t == 0) next[0] = 0;
 
-    const uint32_t wbase = blockIdx.x * kSortTile + wave * 64 * kSortItems;
+    const uint32_t wbase = tile * kSortTile + wave * 64 * kSortItems;
     uint64_t key[kSortItems];
     uint32_t val[kSortItems];
     uint32_t rank[kSortItems];
@@ -274,6 +313,11 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, c
         const bool valid = i < n;
         key[k] = valid ? kin[i] : 0;
         val[k] = valid ? vin[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k)
+    {
+        const bool valid = wbase + k * 64 + lane < n;
         const uint32_t dg = (uint32_t)(key[k] >> shift) & 255;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -291,46 +335,84 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, c
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
+    uint32_t run = 0;
+    for (int w = 0; w < kWaves; ++w)
     {
-        // per digit: exclusive prefix of the per-wave counts, then where the
-        // digit's run starts in the block (a scan over the digits); bbase is
-        // the global position of the block's run minus that local start
-        uint32_t run = 0;
-        for (int w = 0; w < kWaves; ++w)
-        {
-            const uint32_t c = wcnt[w][threadIdx.x];
-            wcnt[w][threadIdx.x] = run;
-            run += c;
-        }
-        uint32_t total;
-        const uint32_t start = block_exclusive_scan(run, total);
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) wcnt[w][threadIdx.x] += start;
-        bbase[threadIdx.x] = gbase - start;
+        const uint32_t c = wcnt[w][t];
+        wcnt[w][t] = run;
+        run += c;
     }
+    // publish the tile's count of digit t, then look back for the count of
+    // digit t in all earlier tiles
+    __hip_atomic_store(&st[(size_t)tile * 256 + t], (tile ? kLbAgg : kLbInc) | run, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t start = block_exclusive_scan(run, total);
+    uint32_t prefix = 0;
+    if (tile)
+    {
+        uint32_t j = tile - 1, spins = 0;
+        for (;;)
+        {
+            const uint32_t w = __hip_atomic_load(&st[(size_t)j * 256 + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (w & (kLbAgg | kLbInc))
+            {
+                prefix += w & kLbCount;
+                if ((w & kLbInc) || j == 0) break;
+                --j;
+            }
+            else if (++spins >= kLbSpins)
+            {
+                atomicOr(err, err_bit);
+                break;
+            }
+            else
+                __builtin_amdgcn_s_sleep(1);
+        }
+        __hip_atomic_store(&st[(size_t)tile * 256 + t], kLbInc | (prefix + run), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wcnt[w][t] += start;
+    bbase[t] = gdig + prefix - start;
     __syncthreads();
+    // the keys, then the values, pass through one LDS area in sorted order
+    // (the tile's position of item k stays in rank[k], its output position
+    // of LDS slot t + j * kBlock in opos[j])
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k)
     {
-        const uint32_t i = wbase + k * 64 + lane;
-        if (i < n)
+        if (wbase + k * 64 + lane < n)
         {
             const uint32_t dg = (uint32_t)(key[k] >> shift) & 255;
-            const uint32_t pos = wcnt[wave][dg] + rank[k];
-            skey[pos] = key[k];
-            sval[pos] = val[k];
+            rank[k] += wcnt[wave][dg];
+            skey[rank[k]] = key[k];
         }
     }
     __syncthreads();
-    const uint32_t b0 = blockIdx.x * kSortTile;
+    const uint32_t b0 = tile * kSortTile;
     const uint32_t cnt = n - b0 < (uint32_t)kSortTile ? n - b0 : (uint32_t)kSortTile;
-    for (uint32_t i = threadIdx.x; i < cnt; i += kBlock)
+    uint32_t opos[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j)
     {
-        const uint64_t k = skey[i];
-        const uint32_t pos = bbase[(uint32_t)(k >> shift) & 255] + i;
-        kout[pos] = k;
-        vout[pos] = sval[i];
+        const uint32_t i = t + j * kBlock;
+        opos[j] = n;
+        if (i < cnt)
+        {
+            const uint64_t k = skey[i];
+            opos[j] = bbase[(uint32_t)(k >> shift) & 255] + i;
+            if (opos[j] < n) kout[opos[j]] = k;
+        }
     }
+    __syncthreads();
+    uint32_t* const sval = (uint32_t*)skey;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k)
+        if (wbase + k * 64 + lane < n) sval[rank[k]] = val[k];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j)
+        if (opos[j] < n) vout[opos[j]] = sval[t + j * kBlock];
 }
 
 } // namespace qdt
