@@ -558,28 +558,6 @@ __global__ __launch_bounds__(kBlock) void k_children(int* off, int d)
     if (j == cnt - 1) oc[2 * cnt] = od[cnt];
 }
 
-// node ids at level d+1 of the points of the sets rooted there (ranks
-// [ra, rb), positions [vstart[ra], vstart[rb])); until then they belong to
-// no node
-__global__ __launch_bounds__(kBlock) void k_root_nodes(const int* off, int d1, int ra, int rb,
-                                                       const uint32_t* vstart, uint32_t first, uint32_t* node)
-{
-    const uint32_t p = vstart[ra] + blockIdx.x * kBlock + threadIdx.x;
-    if (p >= vstart[rb]) return;
-    int lo = ra, hi = rb - 1; // the rank whose range holds p
-    while (lo < hi)
-    {
-        const int mid = (lo + hi + 1) >> 1;
-        if (vstart[mid] <= p)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    node[p] = first + (uint32_t)(lo - ra);
-    (void)off;
-    (void)d1;
-}
-
 // Each kd array carries, beside its vertex ids, their frame-1 ranks (R):
 // an id is the point's frame-0 rank, so a point's key in either frame is at
 // hand without a lookup.  Within a node both arrays are sorted by their own
@@ -597,55 +575,289 @@ __global__ __launch_bounds__(kBlock) void k_kd_init(uint32_t* B0, const uint32_t
     R0[B1[p]] = p;
 }
 
-// "goes left" of position p of the other frame's array (K: that array's keys
-// in the cut's frame; positions from `end` on belong to sets rooted deeper).
-// It only loads, so the scan's passes issue the loads of all their items at
-// once; both passes and the partition evaluate it.
-struct SideFlag
-{
-    const uint32_t* K;
-    const uint32_t* node;
-    const uint32_t* medk;
-    uint32_t end;
-    __device__ uint32_t operator()(uint32_t p) const { return p < end && K[p] < medk[node[p]] ? 1u : 0u; }
-};
-
 struct LoadU8
 {
     const uint8_t* F;
     __device__ uint32_t operator()(uint32_t p) const { return F[p]; }
 };
 
-// Stable two-way partition, within each node, of the array sorted in the
-// other frame (position p goes left when its key, the id on levels cut in
-// frame 0 and the frame-1 rank on the others, is below its node's medk), ids
-// and frame-1 ranks together; it also gives every position its node one
-// level down (node_next).  Z: exclusive prefix counts of "goes left".
-__global__ __launch_bounds__(kBlock) void k_partition(const uint32_t* Bo, const uint32_t* Ro, const uint32_t* node,
-                                                      const int* off, int d, const uint32_t* medk, const uint32_t* Z,
-                                                      uint32_t m, uint32_t* Bout, uint32_t* Rout,
-                                                      uint32_t* node_next)
+// One kd level in one pass: the stable two-way partition, within each node,
+// of the array sorted in the other frame (position p goes left when its key,
+// the id on levels cut in frame 0 and the frame-1 rank on the others, is
+// below its node's medk), ids and frame-1 ranks together, with the count of
+// "goes left" before p in its node from a segmented scan whose segments are
+// the level's nodes.  Position p's node is the last j with od[j] <= p (the
+// level's nodes cover [0, od[cnt]) in order; positions from od[cnt] on
+// belong to sets rooted deeper and keep their place), so no node array is
+// read or written.
+// A tile of kKdTile positions: each wave owns a contiguous run of
+// 64 kKdItems positions and takes it in 64-position steps (lane l holds
+// position run + 64 k + l of step k), so loads are whole lines and the
+// stores of a step fall into one run per side of each node it touches.
+// The tile's count since its first position's node start comes from the
+// tiles before it by a decoupled look-back (tiles numbered in start order by
+// an atomic ticket, as in the radix passes): a tile publishes the count
+// since its last node start (flagged when it holds one) and, once it knows
+// its carry, its inclusive value; a tile reads the words of the 64 tiles
+// before it at once, and the nearest that is inclusive or holds a node start
+// ends the look-back (the counts up to it add up).
+// state: this level's look-back words (a ticket, 63 pad words, one word per
+// tile; cleared); next: the other level's, cleared here for the next level.
+// A look-back that polls one word kLbSpins times sets kKdErr and goes on.
+constexpr int kKdItems = 16;
+constexpr int kKdTile = kBlock * kKdItems;
+constexpr int kKdNodes = 512;              // node offsets of a tile kept in LDS (beyond: read from HBM)
+constexpr uint32_t kKdErr = 1u << 24;      // err bit: a kd level's look-back gave up
+constexpr uint32_t kKdHead = 1u << 29;     // segmented value: a node starts inside (count since its start)
+constexpr uint32_t kKdCount = kKdHead - 1;
+
+__host__ __device__ inline size_t kd_half_words(size_t m) { return 64 + (m + kKdTile - 1) / kKdTile; }
+
+// segmented "then": b after a (a's count is dropped when b holds a node start)
+__device__ __forceinline__ uint32_t kd_seg(uint32_t a, uint32_t b) { return (b & kKdHead) ? b : a + b; }
+
+__global__ __launch_bounds__(kBlock) void k_kd_level(const uint32_t* Bo, const uint32_t* Ro, const int* off, int d,
+                                                     const uint32_t* medk, uint32_t m, uint32_t* Bout,
+                                                     uint32_t* Rout, uint32_t* state, uint32_t* next, int* err)
 {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= m) return;
+    __shared__ uint32_t s_tile, s_carry;
+    __shared__ int s_j0, s_nl;
+    __shared__ int s_od[kKdNodes + 1];
+    __shared__ uint32_t s_mk[kKdNodes];
+    __shared__ uint32_t wagg[kWaves];
+    const uint32_t t = threadIdx.x, lane = lane_id(), wave = t / 64;
+    const int cnt = (int)lcount(off, d);
     const int* od = off + off_base(off, d);
-    const uint32_t v = Bo[p], r = Ro[p];
-    if (p >= (uint32_t)od[lcount(off, d)])
+    if (t == 0) s_tile = atomicAdd(state, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    next[64 + tile] = 0;
+    if (tile == 0 && t == 0) next[0] = 0;
+    const uint32_t t0 = tile * kKdTile;
+    const uint32_t run0 = t0 + wave * (64 * kKdItems) + lane; // step k: run0 + 64 k
+    uint32_t v[kKdItems], r[kKdItems];
+#pragma unroll
+    for (int k = 0; k < kKdItems; ++k)
     {
-        Bout[p] = v; // sets rooted deeper keep their order
-        Rout[p] = r;
-        return;
+        const uint32_t p = run0 + 64 * k;
+        v[k] = p < m ? Bo[p] : 0;
+        r[k] = p < m ? Ro[p] : 0;
     }
-    const uint32_t j = node[p];
-    const int o = od[j];
-    const int s = od[j + 1] - o;
-    const int ls = left_size(s);
-    const int z = (int)(Z[p] - Z[o]);
-    const bool left = ((d & 1) ? r : v) < medk[j];
-    const int pos = left ? o + z : o + ls + ((int)p - o - z);
-    Bout[pos] = v;
-    Rout[pos] = r;
-    node_next[p] = 2 * j + (((int)p - o) >= ls ? 1u : 0u);
+    const int end = od[cnt];
+    // the node of the tile's first position (wave 0): the nodes of a level
+    // have nearly equal sizes, so 64 offsets around the proportional guess
+    // hold it; otherwise a 64-way search
+    if (wave == 0)
+    {
+        int lo = cnt - 1;
+        if ((int)t0 < end)
+        {
+            const int g = (int)(((unsigned long long)t0 * (unsigned)cnt) / (unsigned)end);
+            const int a = max(0, min(cnt - 1, g) - 31);
+            const int idx = a + (int)lane;
+            const uint64_t le = __ballot(idx <= cnt && od[idx] <= (int)t0);
+            if ((le & 1) && ~le)
+                lo = a + __popcll(le) - 1;
+            else
+            {
+                lo = 0;
+                int hi = cnt; // od[lo] <= t0 < od[hi] (od[0] = 0)
+                while (hi - lo > 1)
+                {
+                    const int step = (hi - lo + 63) / 64;
+                    const int ix = lo + (int)lane * step;
+                    const uint64_t bl = __ballot(ix < hi && od[ix] <= (int)t0);
+                    const int c = max(__popcll(bl), 1);
+                    lo += (c - 1) * step;
+                    hi = min(hi, lo + step);
+                }
+            }
+        }
+        if (lane == 0)
+        {
+            s_j0 = lo;
+            s_nl = min(kKdNodes, cnt - lo);
+        }
+    }
+    __syncthreads();
+    const int j0 = s_j0, nl = s_nl;
+    for (int i = (int)t; i <= nl; i += kBlock)
+    {
+        s_od[i] = od[j0 + i];
+        if (i < nl) s_mk[i] = medk[j0 + i];
+    }
+    __syncthreads();
+    // offsets and keys of nodes beyond the cached ones come from HBM
+    auto odv = [&](int j) { return j - j0 <= nl ? s_od[j - j0] : od[j]; };
+    auto mkv = [&](int j) { return j - j0 < nl ? s_mk[j - j0] : medk[j]; };
+    // this lane's first node: last j with od[j] <= run0
+    int j = j0;
+    if ((int)run0 < end)
+    {
+        int lo, hi;
+        if ((int)run0 >= s_od[nl])
+        {
+            lo = j0 + nl;
+            hi = cnt - 1;
+        }
+        else
+        {
+            lo = j0;
+            hi = j0 + nl - 1;
+        }
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (odv(mid) <= (int)run0)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        j = lo;
+    }
+    // each step's node, side and node start; the wave's run: the count of
+    // lefts after its last node start (flagged when it holds one)
+    const bool odd = d & 1;
+    int nj[kKdItems];
+    uint32_t flags = 0, heads = 0;
+    {
+        int e = odv(j + 1);
+        uint32_t mk = mkv(j);
+        int o = odv(j);
+        int last_head = -1; // position of the run's last node start
+#pragma unroll
+        for (int k = 0; k < kKdItems; ++k)
+        {
+            const int p = (int)run0 + 64 * k;
+            nj[k] = -1; // passes through
+            if (p >= end) continue;
+            while (p >= e && j + 1 < cnt)
+            {
+                ++j;
+                o = e;
+                e = odv(j + 1);
+                mk = mkv(j);
+            }
+            nj[k] = j;
+            if ((odd ? r[k] : v[k]) < mk) flags |= 1u << k;
+            if (p == o)
+            {
+                heads |= 1u << k;
+                last_head = p;
+            }
+        }
+        // the wave's last node start, then its lefts from there on
+        int lh = last_head;
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) lh = max(lh, __shfl_xor(lh, dd, 64));
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < kKdItems; ++k)
+            c += ((flags >> k) & 1) && (int)run0 + 64 * k >= lh ? 1u : 0u;
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) c += __shfl_xor(c, dd, 64);
+        if (lane == 0) wagg[wave] = (lh >= 0 ? kKdHead : 0u) | c;
+    }
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+    {
+        if ((uint32_t)w < wave) before = kd_seg(before, wagg[w]);
+        total = kd_seg(total, wagg[w]);
+    }
+    // the tile's carry (wave 0)
+    if (wave == 0)
+    {
+        uint32_t* const st = state + 64;
+        uint32_t carry = 0;
+        if (tile == 0)
+        {
+            if (lane == 0)
+                __hip_atomic_store(&st[0], kLbInc | (total & kKdCount), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        else
+        {
+            if (lane == 0) __hip_atomic_store(&st[tile], kLbAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int base = (int)tile - 1;
+            uint32_t spins = 0;
+            for (;;)
+            {
+                const int jt = base - (int)lane;
+                const uint32_t w = jt >= 0 ? __hip_atomic_load(&st[jt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                           : kLbInc; // before tile 0: nothing
+                const bool ready = (w & (kLbAgg | kLbInc)) != 0;
+                const uint64_t stops = __ballot(ready && (w & (kLbInc | kKdHead)));
+                const uint64_t waiting = __ballot(!ready);
+                const int sl = stops ? __ffsll((unsigned long long)stops) - 1 : 64;
+                const uint64_t need = sl >= 63 ? ~0ull : ((2ull << sl) - 1);
+                if (waiting & need)
+                {
+                    if (++spins >= kLbSpins)
+                    {
+                        if (lane == 0) atomicOr(err, (int)kKdErr);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                uint32_t c = (int)lane <= sl ? (w & kKdCount) : 0;
+#pragma unroll
+                for (int dd = 32; dd >= 1; dd >>= 1) c += __shfl_xor(c, dd, 64);
+                carry += c;
+                if (sl < 64) break;
+                base -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&st[tile], kLbInc | (kd_seg(carry, total) & kKdCount), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_carry = carry;
+    }
+    __syncthreads();
+    // each step: a segmented scan across the wave gives every position the
+    // count of lefts before it in its node; the destinations follow
+    uint32_t carry = kd_seg(s_carry, before) & kKdCount;
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int k = 0; k < kKdItems; ++k)
+    {
+        const uint32_t p = run0 + 64 * k;
+        const bool left = (flags >> k) & 1;
+        const uint64_t hb = __ballot((heads >> k) & 1);
+        const uint64_t lb = __ballot(left);
+        // lefts before this lane since the nearest node start at or before it
+        const uint64_t hb_le = hb & (lt | (1ull << lane));
+        uint32_t z;
+        if (hb_le)
+        {
+            const int h = 63 - __clzll((long long)hb_le); // the lane of that node start
+            z = (uint32_t)__popcll(lb & lt & ~((1ull << h) - 1));
+        }
+        else
+            z = carry + (uint32_t)__popcll(lb & lt);
+        // the next step's carry: the count after the step's last node start
+        if (hb)
+        {
+            const int h = 63 - __clzll((long long)hb);
+            carry = (uint32_t)__popcll(lb & ~((1ull << h) - 1));
+        }
+        else
+            carry += (uint32_t)__popcll(lb);
+        if (p >= m) continue;
+        uint32_t dst = p;
+        if (nj[k] >= 0)
+        {
+            const int o = odv(nj[k]);
+            const int ls = left_size(odv(nj[k] + 1) - o);
+            dst = left ? (uint32_t)(o + (int)z) : (uint32_t)(o + ls + ((int)p - o - (int)z));
+        }
+        if (dst < m)
+        {
+            Bout[dst] = v[k];
+            Rout[dst] = r[k];
+        }
+    }
 }
 
 // Vertex ids renumbered into the order of the frame-0 array at the cell
@@ -2860,7 +3072,7 @@ struct Engine::Bufs
 {
     // point-proportional (ensure)
     uint64_t *kA, *kB, *kx, *ky;
-    uint32_t *vA, *vB, *dist, *ID0, *rep, *B0, *B1, *Bt, *nodeA, *nodeB, *Z, *tris, *face, *queue;
+    uint32_t *vA, *vB, *dist, *ID0, *rep, *B0, *B1, *Bt, *ID1, *Z, *tris, *face, *queue;
     uint32_t *partials, *hist, *ghist, *vrank, *rin, *one_set, *rblock;
     double2* P0;
     double2* P1; // the other order's coordinates (vertex ids renumbered for the cells)
@@ -3146,7 +3358,7 @@ size_t Engine::bytes_after(size_t n, size_t k) const
         const size_t nb_scan = cdiv(R, kScanTile) + 1;
         const size_t qcap = std::max<size_t>(4096, N / 4);
         const size_t per[] = { 8 * N, 8 * N, 8 * N, 8 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N,
-                               4 * N, 4 * N, 4 * N, 4 * N, 4 * 6 * N, 4 * R, 4 * qcap,
+                               4 * N, 4 * N, 4 * N, 4 * 6 * N, 4 * R, 4 * qcap,
                                4 * (nb_scan + 1), 4 * radix_state_words(N), 4 * 8 * 256, 4 * N, 4 * N,
                                16 * N, 16 * N, N, 4 * 4 * R, 4 * 2 * R, 8 * 16, 64, 4 * N, 4 * N, 64, 64, 256 };
         ws = 0;
@@ -3399,7 +3611,7 @@ Status Engine::ensure_ws(size_t n)
         { (void**)&b.ky, 8 * N },       { (void**)&b.vA, 4 * N },       { (void**)&b.vB, 4 * N },
         { (void**)&b.dist, 4 * N },     { (void**)&b.ID0, 4 * N },      { (void**)&b.rep, 4 * N },
         { (void**)&b.B0, 4 * N },       { (void**)&b.B1, 4 * N },       { (void**)&b.Bt, 4 * N },
-        { (void**)&b.nodeA, 4 * N },    { (void**)&b.nodeB, 4 * N },    { (void**)&b.Z, 4 * N },
+        { (void**)&b.ID1, 4 * N },    { (void**)&b.Z, 4 * N },
         { (void**)&b.tris, 4 * 6 * N }, { (void**)&b.face, 4 * R },     { (void**)&b.queue, 4 * qcap },
         { (void**)&b.partials, 4 * (size_t)(nb_scan + 1) },
         { (void**)&b.hist, 4 * radix_state_words(N) },
@@ -4195,15 +4407,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     st_.levels = (uint32_t)D;
     fill_words(b.res, 4 * P.nodes, 0xffffffffu, s);
     fill_words(b.lvlK, kMaxLevels, 0, s);
-    uint32_t *nodeA = b.nodeA, *nodeB = b.nodeB;
     uint32_t* Bt = b.Bt;
-    auto roots_at = [&](int d, uint32_t* node) {
+    auto roots_at = [&](int d) {
         const int ra = P.ra[d], rb = P.ra[d + 1];
         if (ra == rb) return;
         k_roots<<<cdiv((size_t)(rb - ra) + 1, kBlock), kBlock, 0, s>>>(b.off, d, ra, rb, vstart);
-        const uint32_t span = multi ? vstart_h[rb] - vstart_h[ra] : m;
-        k_root_nodes<<<cdiv(std::max<uint32_t>(span, 1), kBlock), kBlock, 0, s>>>(
-            b.off, d, ra, rb, vstart, d ? 2 * (uint32_t)P.count[d - 1] : 0u, node);
     };
     // cells: the first level whose nodes all fit in LDS builds everything
     // below it in one kernel (and so do the roots of sets rooted deeper)
@@ -4221,7 +4429,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         if (tr_->K.size() != (size_t)D + 1) return Status::internal;
         for (int d = 0; d <= D; ++d) P.K[d] = tr_->K[d];
     }
-    roots_at(0, nodeA);
+    roots_at(0);
+    // the kd levels' look-back state: two halves, used by alternate launches
+    const size_t kd_half = kd_half_words(m);
+    uint32_t kd_launches = 0;
+    fill_words(b.hist, 2 * kd_half, 0, s);
     for (int d = 0; d < Dk; ++d)
     {
         const uint32_t cnt = (uint32_t)P.count[d];
@@ -4239,15 +4451,15 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                                                               b.kn, b.lvlK, medk);
             if (P.K[d])
                 k_node_qv<<<cdiv((size_t)cnt * P.K[d], kBlock), kBlock, 0, s>>>(Bo, b.off, d, P.K[d], b.kn, b.qv);
-            // positions of the sets rooted at this level or above
-            const uint32_t end = multi ? vstart_h[P.ra[d + 1]] : m;
-            scan_u32(SideFlag{ Ko, nodeA, medk, end }, m, StoreU32{ b.Z }, b.partials, s);
-            k_partition<<<cdiv(m, kBlock), kBlock, 0, s>>>(Bo, Ro, nodeA, b.off, d, medk, b.Z, m, Bt, Rt, nodeB);
+            (void)Ko;
+            uint32_t* const half = b.hist + (kd_launches & 1) * kd_half;
+            uint32_t* const other = b.hist + ((kd_launches + 1) & 1) * kd_half;
+            k_kd_level<<<cdiv(m, kKdTile), kBlock, 0, s>>>(Bo, Ro, b.off, d, medk, m, Bt, Rt, half, other, b.err);
+            ++kd_launches;
             std::swap(Bo, Bt);
             std::swap(Ro, Rt);
         }
-        roots_at(d + 1, nodeB);
-        std::swap(nodeA, nodeB);
+        roots_at(d + 1);
     }
     if (use_cells)
         for (int d = Dk + 1; d <= D; ++d) // offsets of the roots below the cells' level
@@ -4264,9 +4476,9 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     if (use_cells)
     {
         uint32_t* nid = reinterpret_cast<uint32_t*>(b.kx);
-        k_renum<<<cdiv(m, kBlock), kBlock, 0, s>>>(B0, m, b.P0, b.ID0, nid, b.P1, b.nodeB);
+        k_renum<<<cdiv(m, kBlock), kBlock, 0, s>>>(B0, m, b.P0, b.ID0, nid, b.P1, b.ID1);
         std::swap(b.P0, b.P1);
-        std::swap(b.ID0, b.nodeB);
+        std::swap(b.ID0, b.ID1);
         k_renum_ids<<<1024, kBlock, 0, s>>>(nid, m, b.splitv, (size_t)P.nodes);
         k_renum_ids<<<1024, kBlock, 0, s>>>(nid, m, b.qv, (size_t)kSegMaxK * b.qv_nodes);
         newid = nid;
