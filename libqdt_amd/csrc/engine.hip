@@ -1008,7 +1008,7 @@ constexpr int kCellLevels = 12;
 // The fast pass runs the filters alone: no exact stage, no calls, fewer
 // registers; a cell where a filter cannot decide is listed and rebuilt by
 // the exact pass.
-using LQE = QuadEdges<short*, const double2*, false, false>; // cell records in LDS: local ids fit 16 bits
+using LQE = QuadEdges<short*, const double2*, false, false, true>; // cell records in LDS: local ids fit 16 bits
 using LQX = QuadEdges<short*, const double2*, false, true>;
 
 // cell c -> (level, node index): the nodes of level dc, then the roots of
@@ -1309,6 +1309,10 @@ __device__ __forceinline__ bool cell_build(unsigned char* lds, int cap, uint32_t
     q.init(LNX, LOR, P, nullptr, 4 * 3 * s, s);
     if constexpr (!Q::kHasExact)
         if (wx > 0 && wy > 0 && wx < INFINITY && wy < INFINITY) static_bounds(wx, wy, q.sb_orient, q.sb_incircle);
+    // without usable static bounds (an extent of zero, or products beyond the
+    // filters' range) the cell goes to the exact rebuild
+    if constexpr (Q::kStaticBounds)
+        if (!(q.sb_orient > 0 && q.sb_incircle > 0)) return false;
     // leaves at sub-level L (absolute dl + L); the partition scratch that
     // shares the records' space is dead by now (the loop above ends with a
     // barrier after its last use)

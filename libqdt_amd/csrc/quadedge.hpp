@@ -66,10 +66,13 @@ struct PredCount
 // stores (the LDS cells) also take no claims and keep no vertex-edge entries.
 // kExact = false leaves out the exact predicate stage (and with it the calls
 // that raise every kernel's register count): an undecided filter sets err 5.
-template <class IntPtr, class PtPtr, bool kChecked = true, bool kExact = true>
+// kStatic = true (without the exact stage): the static bounds are always set
+// (the caller makes sure), so the filters compare against them alone.
+template <class IntPtr, class PtPtr, bool kChecked = true, bool kExact = true, bool kStatic = false>
 struct QuadEdges
 {
     static constexpr bool kHasExact = kExact;
+    static constexpr bool kStaticBounds = kStatic && !kExact;
     IntPtr NX;
     IntPtr OR;
     PtPtr PT;
@@ -290,6 +293,12 @@ struct QuadEdges
     {
         ++pc.orient;
         if constexpr (kExact) return orient2d(A.x, A.y, B.x, B.y, C.x, C.y, pc.orient_exact);
+        if constexpr (kStaticBounds)
+        {
+            const double det = orient2d_det(A.x, A.y, B.x, B.y, C.x, C.y);
+            if (!(fabs(det) > sb_orient)) err = err ? err : 5;
+            return det;
+        }
         double det;
         if (sb_orient > 0)
         {
@@ -304,6 +313,12 @@ struct QuadEdges
     {
         ++pc.incircle;
         if constexpr (kExact) return incircle(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y, pc.incircle_exact);
+        if constexpr (kStaticBounds)
+        {
+            const double det = incircle_det(A.x, A.y, B.x, B.y, C.x, C.y, D.x, D.y);
+            if (!(fabs(det) > sb_incircle)) err = err ? err : 5;
+            return det;
+        }
         double det;
         if (sb_incircle > 0)
         {
@@ -506,11 +521,14 @@ struct QuadEdges
     //    read before the deletion's stores.
     // Where a carried word is one the step itself rewrote (an end vertex of
     // degree one or two), the value written is used instead.
+    // (kStop = false: no stop edge, the run ends at the upper tangent; the
+    // stop tests are compiled out)
+    template <bool kStop = true>
     __device__ int bubble(int& basel, int stop, Pool& pool, int& budget)
     {
-        const int srec = stop >= 0 ? (stop >> 2) : -1;
-        const int so = stop >= 0 ? org(stop) : -1;
-        const int sd = stop >= 0 ? dest(stop) : -1;
+        const int srec = kStop && stop >= 0 ? (stop >> 2) : -1;
+        const int so = kStop && stop >= 0 ? org(stop) : -1;
+        const int sd = kStop && stop >= 0 ? dest(stop) : -1;
         int bo = org(basel), bd = dest(basel);
         double2 Po = pt(bo), Pd = pt(bd);
         bool fresh = true;
@@ -521,7 +539,7 @@ struct QuadEdges
             if (err) return 0;
             if (--budget < 0)
             {
-                err = 3;
+                err = err ? err : 3;
                 return 0;
             }
             if (!take(bo) || !take(bd)) return 2;
@@ -543,7 +561,7 @@ struct QuadEdges
             }
             const int b2 = basel ^ 2;
             bool lv = false;
-            if ((lc >> 2) != srec)
+            if (!kStop || (lc >> 2) != srec)
             {
                 lv = ccw_p(Pl, Pd, Po);
                 if (lv)
@@ -552,7 +570,7 @@ struct QuadEdges
                     for (;;)
                     {
                         if (err || --budget < 0) break;
-                        if ((ln >> 2) == srec) break;
+                        if (kStop && (ln >> 2) == srec) break;
                         if (!in_circle_p(bd, bo, lcd, lnd, Pd, Po, Pl, Pln)) break;
                         if (!take(lcd)) return 2;
                         const int nl = onext(ln);
@@ -569,7 +587,7 @@ struct QuadEdges
                 }
             }
             bool rv = false;
-            if ((rc >> 2) != srec)
+            if (!kStop || (rc >> 2) != srec)
             {
                 rv = ccw_p(Pr, Pd, Po);
                 if (rv)
@@ -578,7 +596,7 @@ struct QuadEdges
                     for (;;)
                     {
                         if (err || --budget < 0) break;
-                        if ((rp >> 2) == srec) break;
+                        if (kStop && (rp >> 2) == srec) break;
                         if (!in_circle_p(bd, bo, rcd, rpd, Pd, Po, Pr, Prp)) break;
                         if (!take(rcd)) return 2;
                         const int pr = onext(e_rot(rp));
@@ -597,7 +615,7 @@ struct QuadEdges
             if (!lv && !rv) return 0;
             const bool right = !lv || (rv && in_circle_p(lcd, bd, bo, rcd, Pl, Pd, Po, Pr));
             // the new cross edge would run rcd -> bd or bo -> lcd
-            if (stop >= 0)
+            if (kStop && stop >= 0)
             {
                 const int no = right ? rcd : bo;
                 const int nd = right ? bd : lcd;
@@ -658,7 +676,7 @@ struct QuadEdges
     __device__ void merge(int& ldo, int ldi, int rdi, int& rdo, Pool& pool, int budget)
     {
         int basel = tangent(ldo, ldi, rdi, rdo, pool, budget);
-        bubble(basel, -1, pool, budget);
+        bubble<false>(basel, -1, pool, budget);
     }
 
     // builder.hxx:287-301
@@ -669,7 +687,7 @@ struct QuadEdges
             if (err) return e;
             if (--budget < 0)
             {
-                err = 3;
+                err = err ? err : 3;
                 return e;
             }
             const int nx = rprev(e);
