@@ -1829,6 +1829,16 @@ __device__ int kd_descend(const int* off, const int* splitv, const double2* PT, 
 }
 
 constexpr int kCross = 16;                     // crossing edges per side of an anchor at most
+// Work bounds of one anchor.  An anchor is optional (a piece simply runs
+// longer without it) while a level's anchor kernel lasts as long as its
+// slowest anchor, so the walks toward the query point are cut short: steps
+// of the first walk from the kd descent's vertex, steps of each walk after
+// the point moves to the bisector, and bisector moves.  Over 99.7% of the
+// anchors finish within these (10M uniform and Gaussian inputs); the rest,
+// whose walks run for hundreds of steps, were the kernel's tail.
+constexpr int kAnchorWalk = 24;
+constexpr int kAnchorRewalk = 8;
+constexpr int kAnchorMoves = 3;
 constexpr int kRec = 6 + 2 * kCross;           // ints per anchor record: u v xu xv nl nr crossings
 constexpr int kMarkSlotsMax = 4096;            // anchors per level up to which crossings get a thread each
 
@@ -1890,10 +1900,10 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
             // own side: x (nearest vertex) and ex (an edge out of it)
             const int st = kd_descend(off, splitv, PT, B0, B1, newid, D, d + 1, 2 * j + side, p, axis, side);
             int ex = -1;
-            int x = st >= 0 ? nn_walk(q, st, p, 512, ex) : -1;
+            int x = st >= 0 ? nn_walk(q, st, p, kAnchorWalk, ex) : -1;
             int xo = __shfl_xor(x, 1), exo = __shfl_xor(ex, 1);
             int u = side ? xo : x, v = side ? x : xo;
-            for (int it = 0; it < 8 && u >= 0 && v >= 0; ++it)
+            for (int it = 0; it < kAnchorMoves && u >= 0 && v >= 0; ++it)
             {
                 if (q.err | __shfl_xor(q.err, 1)) break;
                 const double2 U = PT[u], V = PT[v];
@@ -1903,7 +1913,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
                 const double X = ((vx - ux) * (vx + ux) + (Y - vy) * (Y - vy) - (Y - uy) * (Y - uy)) / (2.0 * (vx - ux));
                 p = from_frame(X, Y, axis);
                 int ex2 = -1;
-                const int x2 = nn_walk(q, x, p, 512, ex2);
+                const int x2 = nn_walk(q, x, p, kAnchorRewalk, ex2);
                 const int xo2 = __shfl_xor(x2, 1), exo2 = __shfl_xor(ex2, 1);
                 const bool same = x2 == x && xo2 == xo;
                 x = x2;
