@@ -901,7 +901,7 @@ using GQE = QuadEdges<int*, const double2*>;
 // the parent's frame; the re-walk (builder.hxx:340-346) is then done by the
 // first reader (the anchor kernel's tangent lane or k_merge), whose launch
 // hides its latency, and written back.  c: the child's index at level dc.
-__device__ __forceinline__ int4 child_result(GQE& q, int4* res, const int* off, int dc, uint32_t c)
+__device__ __forceinline__ int4 child_result(GQE& q, int4* res, const int2* ext, const int* off, int dc, uint32_t c)
 {
     const size_t ni = node_index(off, dc, c);
     int4 r = res[ni];
@@ -909,8 +909,9 @@ __device__ __forceinline__ int4 child_result(GQE& q, int4* res, const int* off, 
     {
         const int* od = off + off_base(off, dc);
         const int sz = od[c + 1] - od[c];
+        const int2 x = ext[ni];
         int first, second;
-        q.rewalk(-r.x - 2, r.y, dc & 1, (dc - 1) & 1, sz, first, second);
+        q.rewalk_ext(-r.x - 2, r.y, dc & 1, (dc - 1) & 1, sz, x.x, x.y, first, second);
         r.x = first;
         r.y = second;
         res[ni] = r;
@@ -965,8 +966,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_leaves(int* NX, int* OR, const 
 constexpr uint32_t kMergeWavesMin = 2048;
 template <uint32_t kMergeLanes>
 __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const double2* PT, int* VE, const int* off,
-                                                       int d, uint32_t m, int lim_e, int4* res, int* err,
-                                                       unsigned long long* counters)
+                                                       int d, uint32_t m, int lim_e, int4* res, const int2* ext,
+                                                       int* err, unsigned long long* counters)
 {
     const uint32_t j = blockIdx.x * kMergeLanes + threadIdx.x;
     GQE q;
@@ -977,13 +978,14 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge(int* NX, int* OR, const d
         const int s = od[j + 1] - od[j];
         if (s >= 4)
         {
-            const int4 L = child_result(q, res, off, d + 1, 2 * j);
-            const int4 R = child_result(q, res, off, d + 1, 2 * j + 1);
+            const int4 L = child_result(q, res, ext, off, d + 1, 2 * j);
+            const int4 R = child_result(q, res, ext, off, d + 1, 2 * j + 1);
             Pool pool = q.cat(Pool{ L.z, L.w }, Pool{ R.z, R.w });
             const int axis = d & 1;
             const int out_axis = d > 0 ? ((d - 1) & 1) : 0;
+            const int2 x = d > 0 ? ext[node_index(off, d, j)] : make_int2(-1, -1);
             int first, second;
-            q.merge_node(L.x, L.y, R.x, R.y, axis, out_axis, s, first, second, pool);
+            q.merge_node(L.x, L.y, R.x, R.y, axis, out_axis, s, first, second, pool, x.x, x.y);
             res[node_index(off, d, j)] = make_int4(first, second, pool.head, pool.tail);
         }
     }
@@ -1752,7 +1754,7 @@ __host__ __device__ inline int seam_anchors_max(double s)
 // split vertex of every internal node; anchor heights of segmented levels
 __global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const uint32_t* Bo, const uint32_t* Kf,
                                                       const int* off, int d, int K, const double2* PT, int* splitv,
-                                                      int* qv, int* kn, int* lvlK, uint32_t* medk)
+                                                      int* qv, int* kn, int* lvlK, uint32_t* medk, int2* ext)
 {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= lcount(off, d)) return;
@@ -1763,6 +1765,9 @@ __global__ __launch_bounds__(kBlock) void k_node_info(const uint32_t* Bf, const 
     if (s < 4) return;
     const size_t ni = node_index(off, d, j);
     splitv[ni] = (int)Bf[o + left_size(s)];
+    // the node's first and last points in the other frame, which is its
+    // parent's: the ends its merged hull edges are re-walked to
+    ext[ni] = make_int2((int)Bo[o], (int)Bo[o + s - 1]);
     int Kn = 0;
     if (K)
     {
@@ -1857,8 +1862,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
                                                             const uint32_t* newid, const int* kn,
                                                             int D, int d, int K, uint32_t m,
                                                             int lim_e, int* arec, double4* acoord, double* akey,
-                                                            int* astate, int4* res, int4* nstate, int* err,
-                                                            unsigned long long* counters)
+                                                            int* astate, int4* res, const int2* ext, int4* nstate,
+                                                            int* err, unsigned long long* counters)
 {
     const uint32_t lane2 = blockIdx.x * kMergeBlock + threadIdx.x;
     const uint32_t t = lane2 >> 1; // anchor
@@ -1874,8 +1879,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_anchor(int* NX, int* OR, co
         const int s = od[j + 1] - od[j];
         if (s >= 4)
         {
-            const int4 L = child_result(q, res, off, d + 1, 2 * j);
-            const int4 R = child_result(q, res, off, d + 1, 2 * j + 1);
+            const int4 L = child_result(q, res, ext, off, d + 1, 2 * j);
+            const int4 R = child_result(q, res, ext, off, d + 1, 2 * j + 1);
             int ldi = L.y, rdi = R.x;
             int budget = 16 * s + 64;
             q.tangent_walk(ldi, rdi, budget);
@@ -3102,6 +3107,7 @@ struct Engine::Bufs
     int* off; // level table, then every level's node offsets
     int4* res;
     int *splitv, *qv, *kn, *lvlK, *redo, *segstart, *nseg, *segresume;
+    int2* ext; // per node: its extreme points in its parent's frame (min, max)
     int *arec, *astate, *aedge, *selk, *nsel, *alist, *alist_n;
     double4* acoord;
     double* akey;
@@ -3656,7 +3662,7 @@ Status Engine::ensure_nodes(const Plan& P)
     const Item items[] = {
         { (void**)&b.off, 4 * (kLvlHdr + offs) }, { (void**)&b.res, 16 * nodes }, { (void**)&b.splitv, 4 * nodes },
         { (void**)&b.kn, 4 * nodes }, { (void**)&b.lvlK, 4 * kMaxLevels }, { (void**)&b.redo, 4 * (nodes + 1) },
-        { (void**)&b.qv, 4 * (size_t)kSegMaxK * qv_nodes },
+        { (void**)&b.qv, 4 * (size_t)kSegMaxK * qv_nodes }, { (void**)&b.ext, 8 * nodes },
     };
     const Status st = carve_region(items, ws_nodes_, ws_nodes_bytes_);
     if (st != Status::ok) return st;
@@ -4462,7 +4468,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             const uint32_t* Ko = (d & 1) ? Ro : Bo;
             k_children<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(b.off, d);
             k_node_info<<<cdiv(cnt, kBlock), kBlock, 0, s>>>(Bf, Bo, Kf, b.off, d, P.K[d], b.P0, b.splitv, b.qv,
-                                                              b.kn, b.lvlK, medk);
+                                                              b.kn, b.lvlK, medk, b.ext);
             if (P.K[d])
                 k_node_qv<<<cdiv((size_t)cnt * P.K[d], kBlock), kBlock, 0, s>>>(Bo, b.off, d, P.K[d], b.kn, b.qv);
             (void)Ko;
@@ -4494,6 +4500,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         std::swap(b.P0, b.P1);
         std::swap(b.ID0, b.ID1);
         k_renum_ids<<<1024, kBlock, 0, s>>>(nid, m, b.splitv, (size_t)P.nodes);
+        k_renum_ids<<<1024, kBlock, 0, s>>>(nid, m, reinterpret_cast<int*>(b.ext), 2 * (size_t)P.nodes);
         k_renum_ids<<<1024, kBlock, 0, s>>>(nid, m, b.qv, (size_t)kSegMaxK * b.qv_nodes);
         newid = nid;
     }
@@ -4577,8 +4584,8 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             auto kern = lanes >= 64 ? k_merge<64>
                                     : (lanes >= 32 ? k_merge<32>
                                                    : (lanes >= 16 ? k_merge<16> : (lanes >= 8 ? k_merge<8> : k_merge<4>)));
-            kern<<<cdiv(cnt, lanes), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res, b.err,
-                                                          b.counters);
+            kern<<<cdiv(cnt, lanes), kMergeBlock, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.res, b.ext,
+                                                          b.err, b.counters);
         }
         else
         {
@@ -4586,7 +4593,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             k_seg_anchor<<<cdiv(2 * (size_t)total + cnt, kMergeBlock), kMergeBlock, 0, s>>>(
                 b.NX, b.OR, b.P0, b.VE, b.off, b.splitv, b.qv, B0, B1, newid, b.kn, Dk, d, K, m, lim_e, b.arec, b.acoord,
                 b.akey,
-                b.astate, b.res, b.nstate, b.err, b.counters);
+                b.astate, b.res, b.ext, b.nstate, b.err, b.counters);
             int sel_threads = 64;
             while (sel_threads < K) sel_threads <<= 1;
             k_seg_select<<<cnt, sel_threads, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m, lim_e, b.spare,

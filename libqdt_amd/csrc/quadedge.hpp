@@ -709,6 +709,59 @@ struct QuadEdges
         }
     }
 
+    // The hull edge out of v with the outer face on its right (the class of
+    // ldo and of hull_walk's result), for v a vertex of the convex hull that
+    // is not inside a straight stretch of it: the ring's faces are strictly
+    // counter-clockwise triangles except the outer face, whose sweep around
+    // v is the one with orient(v, dest(a), dest(onext(a))) <= 0 (a single
+    // edge sweeps 360 degrees); the edge after that sweep.  -1 if the ring
+    // shows no such sweep within the budget.
+    __device__ int hull_edge_out(int v, int budget)
+    {
+        if (!VE || (unsigned)v >= (unsigned)lim_v) return -1;
+        const int e0 = VE[v];
+        if (e0 < 0) return -1;
+        const double2 V = pt(v);
+        int a = e0, da = dest(e0);
+        double2 A = pt(da);
+        for (int i = 0; i < budget && !err; ++i)
+        {
+            const int b = onext(a), db = dest(b);
+            const double2 B = pt(db);
+            if (!(pred_orient(V, A, B) > 0)) return b;
+            a = b;
+            A = B;
+            if (a == e0) break;
+        }
+        return -1;
+    }
+
+    // re-walk of a merged node's extreme hull edges to the parent's frame
+    // (builder.hxx:340-346) when the extreme points in that frame are known
+    // (vmin, vmax; < 0: not known): the hull edges out of them directly, in
+    // O(degree) instead of a walk along the hull (which points in convex
+    // position make as long as the node).  hull_walk finds the same edges:
+    // it follows the hull in the class of its start edge to the vertex that
+    // is extreme in the frame (the order is unimodal along a convex hull),
+    // and the frame's extreme point is a hull vertex whose interior angle is
+    // below 180 degrees.
+    __device__ __forceinline__ void rewalk_ext(int ldo, int rdo, int axis, int out_axis, int npts, int vmin,
+                                               int vmax, int& first, int& second)
+    {
+        if (out_axis != axis && vmin >= 0 && vmax >= 0)
+        {
+            const int f = hull_edge_out(vmin, npts + 8);
+            const int g = hull_edge_out(vmax, npts + 8);
+            if (f >= 0 && g >= 0)
+            {
+                first = f;
+                second = oprev(g);
+                return;
+            }
+        }
+        rewalk(ldo, rdo, axis, out_axis, npts, first, second);
+    }
+
     // re-walk of a merged node's extreme hull edges to the parent's frame
     // (builder.hxx:340-346)
     __device__ __forceinline__ void rewalk(int ldo, int rdo, int axis, int out_axis, int npts, int& first,
@@ -727,14 +780,16 @@ struct QuadEdges
         }
     }
 
-    // npts = points of the node (bounds every loop)
+    // npts = points of the node (bounds every loop); vmin, vmax: the node's
+    // extreme points in the parent's frame when known (else -1)
     __device__ __forceinline__ void merge_node(int lfirst, int lsecond, int rfirst, int rsecond, int axis,
-                                               int out_axis, int npts, int& first, int& second, Pool& pool)
+                                               int out_axis, int npts, int& first, int& second, Pool& pool,
+                                               int vmin = -1, int vmax = -1)
     {
         int ldo = lfirst, rdo = rsecond;
         const int budget = 16 * npts + 64;
         merge(ldo, lsecond, rfirst, rdo, pool, budget);
-        rewalk(ldo, rdo, axis, out_axis, npts, first, second);
+        rewalk_ext(ldo, rdo, axis, out_axis, npts, vmin, vmax, first, second);
     }
 };
 
