@@ -112,9 +112,16 @@ __device__ int nn_walk(Q& q, int s, double2 p, int budget, int& e_out)
 // neighbour wr" is max c(right) < min c(left), so it suffices to find the
 // extreme neighbour of each side by a streaming scan (the side's point inside
 // the circles of all the others; comparisons are the same exact incircle
-// tests) and test that one pair.  A neighbour collinear with u and v rejects
-// the anchor.  certify_ring scans the ring of e0's origin (u's ring or v's
-// ring; both are needed), certify_join combines the two scans.
+// tests) and test that one pair.  A circle through u and v meets their line
+// at u and v only, so a neighbour on that line lies strictly inside every
+// such circle when it is between u and v (the anchor is rejected) and
+// strictly outside every one otherwise (it constrains nothing; lattices
+// give such neighbours at every edge).  Why the neighbours suffice: a circle
+// through u that holds a point of L inside (or on it) shrunk toward u keeps
+// u on it and first meets a Delaunay neighbour of u in L, which is then
+// inside (or on) the original circle; the same holds for v and R.
+// certify_ring scans the ring of e0's origin (u's ring or v's ring; both are
+// needed), certify_join combines the two scans.
 template <class Q>
 __device__ bool certify_ring(Q& q, int u, int v, int e0, int& wl, int& wr)
 {
@@ -129,8 +136,16 @@ __device__ bool certify_ring(Q& q, int u, int v, int e0, int& wl, int& wr)
         if (w != u && w != v)
         {
             const double o = q.orient(u, v, w);
-            if (o == 0) return false;
-            if (o > 0)
+            if (o == 0)
+            {
+                // on the line of u and v: between them (in the coordinate
+                // that varies along the line) rejects, outside is no constraint
+                const double2 U = q.pt(u), V = q.pt(v), X = q.pt(w);
+                const bool byx = U.x != V.x;
+                const double a = byx ? U.x : U.y, b = byx ? V.x : V.y, c = byx ? X.x : X.y;
+                if ((a < c && c < b) || (b < c && c < a)) return false;
+            }
+            else if (o > 0)
             {
                 // left: keep the one inside the others' circles (u, v, w ccw)
                 if (wl < 0 || q.incircle_v(u, v, wl, w) > 0) wl = w;
