@@ -3118,7 +3118,8 @@ struct Engine::Bufs
     uint32_t *soff, *srank, *rpos, *vstart, *setbad, *in_off, *mstart, *sstart, *toff, *trank, *btris, *tmain;
     size_t sets_cap, tris_cap;
     // pinned host staging
-    uint32_t* host_scalars;
+    uint32_t* host_scalars; // 64 words, pinned: scalars and the result block read back
+    uint32_t* host_hist;    // 2 x 8 x 256 words, pinned: the sorts' digit histograms read back
     int* host_table;
     uint32_t* host_sets;
     size_t host_sets_cap;
@@ -3297,6 +3298,7 @@ Engine::Engine(hipStream_t stream, void* ext, size_t ext_bytes)
     std::memset(b_, 0, sizeof(Bufs));
     b_->host_scalars = ext_scalars_;
     b_->host_table = ext_table_;
+    b_->host_hist = ext_hist_;
 }
 
 Engine::Engine(hipStream_t stream) : stream_(stream), own_stream_(stream == nullptr)
@@ -3328,6 +3330,7 @@ Engine::~Engine()
     {
         if (b_->host_scalars) (void)hipHostFree(b_->host_scalars);
         if (b_->host_table) (void)hipHostFree(b_->host_table);
+        if (b_->host_hist) (void)hipHostFree(b_->host_hist);
         if (b_->host_sets) (void)hipHostFree(b_->host_sets);
     }
     delete b_;
@@ -3379,7 +3382,7 @@ size_t Engine::bytes_after(size_t n, size_t k) const
         const size_t qcap = std::max<size_t>(4096, N / 4);
         const size_t per[] = { 8 * N, 8 * N, 8 * N, 8 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N, 4 * N,
                                4 * N, 4 * N, 4 * N, 4 * 6 * N, 4 * R, 4 * qcap,
-                               4 * (nb_scan + 1), 4 * radix_state_words(N), 4 * 8 * 256, 4 * N, 4 * N,
+                               4 * (nb_scan + 1), 4 * radix_state_words(N), 4 * 2 * 8 * 256, 4 * N, 4 * N,
                                16 * N, 16 * N, N, 4 * 4 * R, 4 * 2 * R, 8 * 16, 64, 4 * N, 4 * N, 64, 64, 256 };
         ws = 0;
         for (size_t b : per) ws += round256(b);
@@ -3575,9 +3578,9 @@ Status Engine::sort_keys(uint64_t*& kA, uint32_t*& vA, uint64_t*& kB, uint32_t*&
     }
     else
     {
-        QDT_HIP(hipMemcpyAsync(ghist_host_, b.ghist, 4 * 8 * 256, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipMemcpyAsync(b.host_hist, b.ghist, 4 * 8 * 256, hipMemcpyDeviceToHost, s));
         QDT_HIP(hipStreamSynchronize(s));
-        mask = pass_mask(ghist_host_, n);
+        mask = pass_mask(b.host_hist, n);
         if (mode_ == Mode::record) tr_->sort_mask.push_back(mask);
     }
     sort_passes(kA, vA, kB, vB, n, mask, b.ghist, b.hist, (uint32_t*)b.err, kSortErr, s);
@@ -3598,10 +3601,12 @@ Status Engine::ensure(size_t n)
     if (!b_->host_scalars)
     {
         void* h = nullptr;
-        QDT_HIP(hipHostMalloc(&h, 64, hipHostMallocDefault));
+        QDT_HIP(hipHostMalloc(&h, 4 * 64, hipHostMallocDefault));
         b_->host_scalars = (uint32_t*)h;
         QDT_HIP(hipHostMalloc(&h, 4 * kLvlHdr, hipHostMallocDefault));
         b_->host_table = (int*)h;
+        QDT_HIP(hipHostMalloc(&h, 4 * 2 * 8 * 256, hipHostMallocDefault));
+        b_->host_hist = (uint32_t*)h;
     }
     if (!ev0_)
     {
@@ -3635,7 +3640,7 @@ Status Engine::ensure_ws(size_t n)
         { (void**)&b.tris, 4 * 6 * N }, { (void**)&b.face, 4 * R },     { (void**)&b.queue, 4 * qcap },
         { (void**)&b.partials, 4 * (size_t)(nb_scan + 1) },
         { (void**)&b.hist, 4 * radix_state_words(N) },
-        { (void**)&b.ghist, 4 * 8 * 256 },
+        { (void**)&b.ghist, 4 * 2 * 8 * 256 },
         { (void**)&b.vrank, 4 * N },    { (void**)&b.rin, 4 * N },
         { (void**)&b.P0, 16 * N },      { (void**)&b.P1, 16 * N },      { (void**)&b.flagp, N },         { (void**)&b.NX, 4 * 4 * R },
         { (void**)&b.OR, 4 * 2 * R },   { (void**)&b.counters, 8 * 16 }, { (void**)&b.err, 64 },
@@ -4213,24 +4218,49 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     k_keys<<<cdiv(n, kBlock), kBlock, 0, s>>>(pts, n, b.kx, b.kA, b.vA, b.err);
     uint64_t *kA = b.kA, *kB = b.kB;
     uint32_t *vA = b.vA, *vB = b.vB;
-    Status st = sort_keys(kA, vA, kB, vB, n);
+    // One set run eagerly: the digit histograms of both sorts come first (the
+    // second sort's keys are the x keys in another order, so their histogram
+    // is that of the x keys), and one read-back gives both sorts' passes and
+    // the non-finite flag, reported before any further work.
+    const bool one_read = !multi && mode_ == Mode::eager && n > 1;
+    uint32_t mask_x = 0;
+    Status st = Status::ok;
+    if (one_read)
+    {
+        fill_words(b.ghist, 2 * 8 * 256, 0, s);
+        const uint32_t hb = std::min<uint32_t>(cdiv(n, kBlock), 1024);
+        k_radix_global_hist<<<hb, kBlock, 0, s>>>(kA, n, b.ghist);
+        k_radix_global_hist<<<hb, kBlock, 0, s>>>(b.kx, n, b.ghist + 8 * 256);
+        QDT_HIP(hipMemcpyAsync(b.host_hist, b.ghist, 4 * 2 * 8 * 256, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipMemcpyAsync(hs, b.err, 4, hipMemcpyDeviceToHost, s));
+        QDT_HIP(hipStreamSynchronize(s));
+        if (hs[0] & 1) return Status::not_finite;
+        mask_x = pass_mask(b.host_hist + 8 * 256, n);
+        sort_passes(kA, vA, kB, vB, n, pass_mask(b.host_hist, n), b.ghist, b.hist, (uint32_t*)b.err, kSortErr, s);
+    }
+    else
+        st = sort_keys(kA, vA, kB, vB, n);
     if (st != Status::ok) return st;
     // keep the y order for frame 1, runs of equal y put in x-descending
     // order (counters[14]: a run too long for that)
     k_ytie<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, n, b.Bt);
     k_yfix<<<cdiv(n, kBlock), kBlock, 0, s>>>(kA, b.kx, n, b.Bt, &b.counters[14]);
-    if (!multi)
+    if (!multi && !one_read)
     {
         // non-finite input is reported before any further work
         QDT_HIP(hipMemcpyAsync(hs, b.err, 4, hipMemcpyDeviceToHost, s));
         QDT_HIP(hipStreamSynchronize(s));
         if (hs[0] & 1) return Status::not_finite;
     }
-    fill_words(b.err, 16, 0, s);
+    // (the first sort's look-back reports into err: kept until the run's end)
+    if (!one_read) fill_words(b.err, 16, 0, s);
     mark(1);
     // 2. stable sort by x -> (x, y, index); several sets: then by rank
     k_gather_u64<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.kx, n, kA);
-    st = sort_keys(kA, vA, kB, vB, n);
+    if (one_read)
+        sort_passes(kA, vA, kB, vB, n, mask_x, b.ghist + 8 * 256, b.hist, (uint32_t*)b.err, kSortErr, s);
+    else
+        st = sort_keys(kA, vA, kB, vB, n);
     if (st != Status::ok) return st;
     if (multi)
     {
@@ -4510,8 +4540,11 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         std::vector<int> lk(kMaxLevels, 0);
         if (!replay)
         {
-            QDT_HIP(hipMemcpyAsync(lk.data(), b.lvlK, 4 * kMaxLevels, hipMemcpyDeviceToHost, s));
+            // through pinned memory (the level table's upload buffer, whose
+            // copy the stream has done by now)
+            QDT_HIP(hipMemcpyAsync(b.host_table, b.lvlK, 4 * kMaxLevels, hipMemcpyDeviceToHost, s));
             QDT_HIP(hipStreamSynchronize(s));
+            std::memcpy(lk.data(), b.host_table, 4 * kMaxLevels);
         }
         size_t seg_nodes = 1, anchor_cap = 1;
         for (int d = 0; d < D; ++d)
@@ -4665,13 +4698,13 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         return Status::ok;
     }
     if (ev1_) QDT_HIP(hipEventRecord(ev1_, s));
-    QDT_HIP(hipMemcpyAsync(hs, b.partials + nbx, 4, hipMemcpyDeviceToHost, s));
-    QDT_HIP(hipMemcpyAsync(hs + 1, b.err, 4, hipMemcpyDeviceToHost, s));
-    QDT_HIP(hipMemcpyAsync(hs + 2, b.spare, 4, hipMemcpyDeviceToHost, s));
-    if (use_cells) QDT_HIP(hipMemcpyAsync(hs + 3, b.redo, 4, hipMemcpyDeviceToHost, s));
-    unsigned long long cnts[16];
-    QDT_HIP(hipMemcpyAsync(cnts, b.counters, 8 * 16, hipMemcpyDeviceToHost, s));
+    // the run's scalars gathered on the device, read back in one copy
+    k_result_block<<<1, 64, 0, s>>>(b.partials + nbx, b.err, b.spare, use_cells ? b.redo : nullptr, b.counters,
+                                    b.rblock);
+    QDT_HIP(hipMemcpyAsync(hs, b.rblock, 4 * (4 + 2 * 16), hipMemcpyDeviceToHost, s));
     QDT_HIP(hipStreamSynchronize(s));
+    unsigned long long cnts[16];
+    std::memcpy(cnts, hs + 4, sizeof(cnts));
     st_.spare_records = hs[2];
     st_.cells_exact = use_cells ? hs[3] : 0;
     if (hs[1] & kSeamErr)

@@ -264,9 +264,9 @@ private:
     // the start of every run
     char* ext_ = nullptr;
     size_t ext_bytes_ = 0, ext_used_ = 0, ext_ws_end_ = 0;
-    uint32_t ext_scalars_[16] = {};
+    uint32_t ext_scalars_[64] = {};
     int ext_table_[4 * 64] = {};
-    uint32_t ghist_host_[8 * 256] = {}; // a sort's global digit histogram, read back
+    uint32_t ext_hist_[2 * 8 * 256] = {}; // the sorts' global digit histograms, read back (external workspace)
     template <size_t N>
     Status carve_region(const struct Item (&items)[N], char*& region, size_t& bytes);
     void free_region(char*& region, size_t& bytes);
