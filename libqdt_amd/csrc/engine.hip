@@ -2506,21 +2506,27 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_compact(int d, const int* s
 // finishes the stopped pieces bottom to top, with nothing else running on
 // the node (the bubble is a pure function of the store, so a resumed piece
 // does exactly what an uninterrupted one would).
+// Only every `spread`-th lane runs pieces: on levels whose pieces would
+// leave SIMDs idle they are spread over more wavefronts with fewer each
+// (measured on 10M uniform points: up to 1,024 wavefronts 22.47 -> 22.40 ms
+// per run, up to 2,048 or 4,096 slower).
 constexpr int kSegPieceBlock = 256; // threads per workgroup at most
+constexpr int kPieceWaves = 1024;   // wavefronts of a spread launch at most
 
 __global__ __launch_bounds__(kSegPieceBlock) void k_seg_pieces(int* NX, int* OR, const double2* PT, int* VE,
                                                               const int* off, int d, uint32_t m, int lim_e,
                                                               unsigned* spare, int spare_base, int spare_cap,
                                                               const int* segstart, const int* nseg, int tpn,
-                                                              int npb, int claim_entries, int2* segpool,
-                                                              int* segresume, int* err,
+                                                              int npb, int spread, int claim_entries,
+                                                              int2* segpool, int* segresume, int* err,
                                                               unsigned long long* counters)
 {
     __shared__ int nstop[kSegPieceBlock];
     extern __shared__ unsigned long long claim_table[];
-    const int ln = threadIdx.x / tpn, h = threadIdx.x % tpn;
+    const int slot = threadIdx.x / spread;
+    const int ln = slot / tpn, h = slot % tpn;
     const uint32_t j = blockIdx.x * (uint32_t)npb + (uint32_t)ln;
-    const bool node = ln < npb && j < lcount(off, d);
+    const bool node = threadIdx.x % spread == 0 && ln < npb && j < lcount(off, d);
     int ns = 0, s = 0;
     const int* st = nullptr;
     if (node)
@@ -3412,6 +3418,15 @@ size_t Engine::alloc_budget() const
 }
 
 constexpr uint32_t kSortErr = 1u << 23; // err bit: a radix pass's look-back gave up
+// The two sorts of a single set run eagerly, from kSplitMin keys on, take
+// radix passes over the top kSplitPasses bytes in which the keys differ and
+// finish with k_sort_fixup (10M uniform points: 5 passes and a fix-up
+// instead of 8 passes per sort; the runs of keys equal in those bytes hold
+// ~1.02 keys on average).  A long run out of order sets kSplitErr, and the
+// run is repeated with full sorts.
+constexpr uint32_t kSplitErr = 1u << 22;
+constexpr int kSplitPasses = 5;
+constexpr uint32_t kSplitMin = 1u << 20;
 
 // The radix passes of a sort (bit p of mask: pass p runs); the result ends
 // in kA/vA.  ghist: the keys' digit histograms (k_radix_global_hist); state:
@@ -3750,7 +3765,16 @@ Status Engine::run(const double* d_pts, size_t n)
         return s;
     }
     bool redo = false;
+    split_ = true;
     s = run_impl(d_pts, n, nullptr, 1, seams_, redo);
+    if (s == Status::ok && redo && !split_)
+    {
+        // a split sort met a long run out of order: repeat with full sorts
+        const uint32_t bits = st_.err_bits;
+        st_ = RunStats();
+        s = run_impl(d_pts, n, nullptr, 1, seams_, redo);
+        if (!redo) st_.err_bits = bits;
+    }
     if (s == Status::ok && redo)
     {
         // a seam piece failed (budget or record exhaustion): repeat unsplit
@@ -4223,6 +4247,19 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     // is that of the x keys), and one read-back gives both sorts' passes and
     // the non-finite flag, reported before any further work.
     const bool one_read = !multi && mode_ == Mode::eager && n > 1;
+    const bool split = one_read && split_ && n >= kSplitMin;
+    // the radix passes of mask (ghist: the keys' digit histograms), split
+    // into the top passes and a fix-up where that applies
+    auto split_passes = [&](uint64_t*& ka, uint32_t*& va, uint64_t*& kb, uint32_t*& vb, uint32_t mask,
+                            const uint32_t* gh) {
+        uint32_t top = mask;
+        if (split)
+            while (__builtin_popcount(top) > kSplitPasses) top &= top - 1; // the lowest passes go
+        sort_passes(ka, va, kb, vb, n, top, gh, b.hist, (uint32_t*)b.err, kSortErr, s);
+        if (top != mask)
+            k_sort_fixup<<<cdiv(n, kBlock), kBlock, 0, s>>>(ka, va, n, 8 * __builtin_ctz(top), (uint32_t*)b.err,
+                                                             kSplitErr);
+    };
     uint32_t mask_x = 0;
     Status st = Status::ok;
     if (one_read)
@@ -4236,7 +4273,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         QDT_HIP(hipStreamSynchronize(s));
         if (hs[0] & 1) return Status::not_finite;
         mask_x = pass_mask(b.host_hist + 8 * 256, n);
-        sort_passes(kA, vA, kB, vB, n, pass_mask(b.host_hist, n), b.ghist, b.hist, (uint32_t*)b.err, kSortErr, s);
+        split_passes(kA, vA, kB, vB, pass_mask(b.host_hist, n), b.ghist);
     }
     else
         st = sort_keys(kA, vA, kB, vB, n);
@@ -4258,7 +4295,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     // 2. stable sort by x -> (x, y, index); several sets: then by rank
     k_gather_u64<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.kx, n, kA);
     if (one_read)
-        sort_passes(kA, vA, kB, vB, n, mask_x, b.ghist + 8 * 256, b.hist, (uint32_t*)b.err, kSortErr, s);
+        split_passes(kA, vA, kB, vB, mask_x, b.ghist + 8 * 256);
     else
         st = sort_keys(kA, vA, kB, vB, n);
     if (st != Status::ok) return st;
@@ -4658,14 +4695,22 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
                 int tpn = 1;
                 while (tpn < hp) tpn <<= 1;
                 tpn = std::min(tpn, kSegPieceBlock);
-                const int npb = std::max(1, 64 / tpn);
-                const int threads = std::max(64, tpn);
+                // lanes per piece: doubled while the launch stays within
+                // kPieceWaves wavefronts and a node within one workgroup
+                int spread = 1;
+                auto waves = [&](int sp) {
+                    const int np = std::max(1, 64 / (tpn * sp));
+                    return (size_t)cdiv(cnt, (uint32_t)np) * (size_t)std::max(1, tpn * sp * np / 64);
+                };
+                while (tpn * spread * 2 <= kSegPieceBlock && waves(spread * 2) <= (size_t)kPieceWaves) spread *= 2;
+                const int npb = std::max(1, 64 / (tpn * spread));
+                const int threads = std::max(64, tpn * spread);
                 // claim table: ~48 vertices per piece of a phase, at most 64 KB
                 int entries = 256;
                 while (entries < 48 * npb * hp && entries < 8192) entries <<= 1;
                 k_seg_pieces<<<cdiv(cnt, (uint32_t)npb), threads, 8 * (size_t)entries, s>>>(
                     b.NX, b.OR, b.P0, b.VE, b.off, d, m, lim_e, b.spare, spare_base, spare_cap, b.segstart, b.nseg,
-                    tpn, npb, entries, b.segpool, b.segresume, b.err, b.counters);
+                    tpn, npb, spread, entries, b.segpool, b.segresume, b.err, b.counters);
             }
             k_seg_final<<<cnt, kSegThreads, 0, s>>>(b.NX, b.OR, b.P0, b.VE, b.off, d, K, m,
                                                                         lim_e, b.nseg, b.nstate, b.segpool,
@@ -4707,6 +4752,13 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     std::memcpy(cnts, hs + 4, sizeof(cnts));
     st_.spare_records = hs[2];
     st_.cells_exact = use_cells ? hs[3] : 0;
+    if (hs[1] & kSplitErr)
+    {
+        st_.err_bits = hs[1];
+        split_ = false;
+        redo = true;
+        return Status::ok;
+    }
     if (hs[1] & kSeamErr)
     {
         for (int i = 0; i < 8; ++i) st_.seam[i] = cnts[5 + i];

@@ -257,6 +257,7 @@ private:
     int block_points_ = 0;
     bool profiling_ = false;
     bool seams_ = true;
+    bool split_ = true; // eager sorts may take their top passes and a fix-up (cleared for a repeat)
     bool cells_ = true;
     size_t mem_cap_ = 0, spare_limit_ = 0;
     size_t alloc_budget() const;

@@ -415,4 +415,64 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const uint64_t* kin, cons
         if (opos[j] < n) vout[opos[j]] = sval[t + j * kBlock];
 }
 
+// Completes a sort whose radix passes covered only the key bits from `cut`
+// up (cut a multiple of 8 in [8, 56]): keys that agree in those bits stand
+// in their input order (the passes are stable).  The first thread of each
+// run of such keys puts a run of at most kFixRun keys in order by insertion
+// (stable: a key moves only past greater ones), the values along.  A thread
+// that finds its key below the one before it in a longer run sets flag_bit
+// in *flag: the caller then runs all the passes over the result, which is the full stable
+// sort (keys equal in every bit are still in input order).  A run is only
+// rewritten inside itself and all its keys share the bits from cut up, so
+// the run boundaries every thread reads stay as they are meanwhile.
+constexpr uint32_t kFixRun = 16;
+
+__global__ __launch_bounds__(kBlock) void k_sort_fixup(uint64_t* keys, uint32_t* vals, uint32_t n, int cut,
+                                                       uint32_t* flag, uint32_t flag_bit)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    const uint64_t hi = k >> cut;
+    const uint64_t p = i ? keys[i - 1] : 0;
+    if (i == 0 || (p >> cut) != hi)
+    {
+        // first key of a run: its length (up to kFixRun + 1) and order
+        uint32_t len = 1;
+        bool sorted = true;
+        uint64_t prev = k;
+        while (len <= kFixRun && i + len < n)
+        {
+            const uint64_t x = keys[i + len];
+            if ((x >> cut) != hi) break;
+            sorted = sorted && x >= prev;
+            prev = x;
+            ++len;
+        }
+        if (len <= kFixRun && !sorted)
+            for (uint32_t a = i + 1; a < i + len; ++a)
+            {
+                const uint64_t x = keys[a];
+                const uint32_t v = vals[a];
+                uint32_t b = a;
+                for (; b > i && keys[b - 1] > x; --b)
+                {
+                    keys[b] = keys[b - 1];
+                    vals[b] = vals[b - 1];
+                }
+                keys[b] = x;
+                vals[b] = v;
+            }
+    }
+    else if (p > k)
+    {
+        // out of order inside a run (a run being sorted by its first thread
+        // may show that for a moment; such a run is short): the run's length
+        uint32_t len = 2;
+        for (uint32_t a = i - 1; a > 0 && len <= kFixRun && (keys[a - 1] >> cut) == hi; --a) ++len;
+        for (uint32_t a = i + 1; a < n && len <= kFixRun && (keys[a] >> cut) == hi; ++a) ++len;
+        if (len > kFixRun) atomicOr(flag, flag_bit);
+    }
+}
+
 } // namespace qdt

@@ -1,7 +1,9 @@
 """The engine's alternative and fallback paths, each forced from the host and
 checked against the oracle: merges without seam pieces, kd levels without LDS
 cells, a spare-record region too small for the seam pieces (the run is
-repeated without them), and a device-memory limit that turns the workspace
+repeated without them), sorts split into their top radix passes and a fix-up
+whose fix-up meets a long run out of order (the run is repeated with full
+sorts), and a device-memory limit that turns the workspace
 allocation into QDT_ERR_OUT_OF_MEMORY with an emptied context
 (libqdt/qdt.cpp:63-77, 107-112: bad_alloc -> QDT_ERR_OUT_OF_MEMORY, context
 reset).  Every case prints its name before it runs."""
@@ -57,6 +59,33 @@ def test_seam_retry_when_spare_records_run_out(gpu):
         assert st["seam_retries"] == 0 and st["seam"]["pieces"] > 0, "seam pieces not used"
         st = same(name + " spare=64", p, spare=64)
         assert st["seam_retries"] == 1, st["seam_retries"]
+
+
+def near_ties(n, seed, axis):
+    """n points whose coordinate on `axis` takes 1,000 values, each moved by a
+    random multiple (below 2^16) of its unit in the last place: order keys
+    equal in all but their two lowest bytes, in runs of ~1,100 keys in input
+    order, far longer than the split sorts' fix-up takes."""
+    rng = np.random.default_rng(seed)
+    g = 0.5 + rng.integers(0, 1000, n) / 1000.0
+    v = g + rng.integers(0, 65536, n) * np.spacing(g)
+    p = np.empty((n, 2))
+    p[:, axis] = v
+    p[:, 1 - axis] = rng.random(n)
+    return p
+
+
+def test_split_sorts_and_the_full_sort_repeat(gpu):
+    # a single set of at least 2^20 points sorts on the top five bytes in
+    # which its keys differ, then fixes the runs equal in those bytes; a run
+    # too long for that, out of order, sets this error bit and the run is
+    # repeated with full sorts
+    split_err = 1 << 22
+    st = same("uniform 1.1M, split sorts", W.uniform(1100000, 9))
+    assert not st["err_bits"] & split_err, st["err_bits"]
+    for ax, name in ((1, "y"), (0, "x")):
+        st = same("near ties in %s, 1.1M" % name, near_ties(1100000, 11 + ax, ax))
+        assert st["err_bits"] & split_err, st["err_bits"]
 
 
 def test_out_of_memory_empties_the_context(gpu):
