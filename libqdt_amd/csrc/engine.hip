@@ -4344,7 +4344,17 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
         QDT_HIP(hipMemcpyAsync(hs, b.partials + nbs, 4, hipMemcpyDeviceToHost, s));
         if (multi) QDT_HIP(hipMemcpyAsync(b.host_sets, b.vstart, 4 * ((size_t)k + 1), hipMemcpyDeviceToHost, s));
         QDT_HIP(hipMemcpyAsync(hs + 4, &b.counters[14], 8, hipMemcpyDeviceToHost, s));
+        // split sorts: a fix-up that met a long run out of order is known
+        // here, before the tree is built on the wrong order
+        if (split) QDT_HIP(hipMemcpyAsync(hs + 2, b.err, 4, hipMemcpyDeviceToHost, s));
         QDT_HIP(hipStreamSynchronize(s));
+        if (split && (hs[2] & kSplitErr))
+        {
+            st_.err_bits = hs[2];
+            split_ = false;
+            redo = true;
+            return Status::ok;
+        }
         m = hs[0];
         memcpy(&ytie, hs + 4, 8);
         if (multi) vstart_h.assign(b.host_sets, b.host_sets + k + 1);
