@@ -2258,43 +2258,51 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mark(int* OR, int total, co
     wave_add(&counters[6], e >= 0 ? 1ull : 0ull);
 }
 
-// crossed edges: deletion marks and the rings of their endpoints
+// crossed edges: deletion marks and the rings of their endpoints.  Every
+// lane visits its anchor's crossing slots in the same order (slot, then
+// side) for as many slots as the longest list in the wavefront, so that the
+// vertex list grows by one atomic per wavefront and slot instead of one per
+// vertex (the list counter is a single word that every listing hits).
 __global__ __launch_bounds__(kMergeBlock) void k_seg_mark_cross(const int* OR, int total, const int* arec,
                                                                 const int* astate, const int* aedge, uint32_t m,
                                                                 int lim_e, int* owner, int* del, int* alist,
                                                                 int* alist_n, int acap, int* err)
 {
     const int t = blockIdx.x * kMergeBlock + threadIdx.x;
-    if (t >= total || astate[t] != 2 || aedge[t] < 0) return;
-    const int* r = arec + (size_t)t * kRec;
-    for (int side = 0; side < 2; ++side)
-    {
-        const int c = r[4 + side];
-        const int* ce = r + 6 + side * kCross;
-        for (int i = 0; i < c; ++i)
+    const bool act = t < total && astate[t] == 2 && aedge[t] >= 0;
+    const int* r = arec + (size_t)(act ? t : 0) * kRec;
+    const int c0 = act ? min(r[4], kCross) : 0, c1 = act ? min(r[5], kCross) : 0;
+    int cmax = max(c0, c1);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
+    for (int i = 0; i < cmax; ++i)
+        for (int side = 0; side < 2; ++side)
         {
-            const int e = ce[i];
-            if (e < 0 || e >= lim_e)
+            int w0 = -1, w1 = -1;
+            if (i < (side ? c1 : c0))
             {
-                atomicOr(err, kSeamErr);
-                continue;
-            }
-            atomicCAS(&del[e >> 2], -1, t);
-            const int ends[2] = { OR[e >> 1], OR[(e ^ 2) >> 1] };
-            for (int w : ends)
-            {
-                if ((unsigned)w >= m) continue;
-                if (atomicCAS(&owner[w], -1, -3) == -1)
+                const int e = r[6 + side * kCross + i];
+                if (e < 0 || e >= lim_e)
+                    atomicOr(err, kSeamErr);
+                else
                 {
-                    const int at = atomicAdd(alist_n, 1);
-                    if (at < acap)
-                        alist[at] = w;
-                    else
-                        atomicOr(err, kSeamErr);
+                    atomicCAS(&del[e >> 2], -1, t);
+                    const int a = OR[e >> 1], b = OR[(e ^ 2) >> 1];
+                    if ((unsigned)a < m && atomicCAS(&owner[a], -1, -3) == -1) w0 = a;
+                    if ((unsigned)b < m && atomicCAS(&owner[b], -1, -3) == -1) w1 = b;
                 }
             }
+            int at = wave_fetch_add(alist_n, (w0 >= 0 ? 1 : 0) + (w1 >= 0 ? 1 : 0));
+            for (int w : { w0, w1 })
+            {
+                if (w < 0) continue;
+                if (at < acap)
+                    alist[at] = w;
+                else
+                    atomicOr(err, kSeamErr);
+                ++at;
+            }
         }
-    }
 }
 
 // The same with one thread per (anchor, side, crossing slot), for levels of
