@@ -2258,11 +2258,15 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mark(int* OR, int total, co
     wave_add(&counters[6], e >= 0 ? 1ull : 0ull);
 }
 
-// crossed edges: deletion marks and the rings of their endpoints.  Every
-// lane visits its anchor's crossing slots in the same order (slot, then
-// side) for as many slots as the longest list in the wavefront, so that the
-// vertex list grows by one atomic per wavefront and slot instead of one per
-// vertex (the list counter is a single word that every listing hits).
+// crossed edges: deletion marks and the rings of their endpoints.  A
+// thread's crossing slots are independent, so they go in chunks of kMarkChunk
+// per side: the chunk's edge ids, then their end vertices, then the
+// deletion marks and the vertex claims are issued before any result is
+// waited on (a thread waits on a few round trips to L2 instead of four per
+// slot).  Every lane runs as many chunks as the longest list in its
+// wavefront, and the vertex list grows by one atomic per wavefront and chunk.
+constexpr int kMarkChunk = 4;
+
 __global__ __launch_bounds__(kMergeBlock) void k_seg_mark_cross(const int* OR, int total, const int* arec,
                                                                 const int* astate, const int* aedge, uint32_t m,
                                                                 int lim_e, int* owner, int* del, int* alist,
@@ -2275,34 +2279,54 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mark_cross(const int* OR, i
     int cmax = max(c0, c1);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
-    for (int i = 0; i < cmax; ++i)
-        for (int side = 0; side < 2; ++side)
+    for (int i0 = 0; i0 < cmax; i0 += kMarkChunk)
+    {
+        int e[2 * kMarkChunk], w[4 * kMarkChunk];
+#pragma unroll
+        for (int x = 0; x < 2 * kMarkChunk; ++x)
         {
-            int w0 = -1, w1 = -1;
-            if (i < (side ? c1 : c0))
-            {
-                const int e = r[6 + side * kCross + i];
-                if (e < 0 || e >= lim_e)
-                    atomicOr(err, kSeamErr);
-                else
-                {
-                    atomicCAS(&del[e >> 2], -1, t);
-                    const int a = OR[e >> 1], b = OR[(e ^ 2) >> 1];
-                    if ((unsigned)a < m && atomicCAS(&owner[a], -1, -3) == -1) w0 = a;
-                    if ((unsigned)b < m && atomicCAS(&owner[b], -1, -3) == -1) w1 = b;
-                }
-            }
-            int at = wave_fetch_add(alist_n, (w0 >= 0 ? 1 : 0) + (w1 >= 0 ? 1 : 0));
-            for (int w : { w0, w1 })
-            {
-                if (w < 0) continue;
-                if (at < acap)
-                    alist[at] = w;
-                else
-                    atomicOr(err, kSeamErr);
-                ++at;
-            }
+            const int side = x / kMarkChunk, i = i0 + x % kMarkChunk;
+            e[x] = i < (side ? c1 : c0) ? r[6 + side * kCross + i] : -1;
         }
+#pragma unroll
+        for (int x = 0; x < 2 * kMarkChunk; ++x)
+        {
+            const bool ok = e[x] >= 0 && e[x] < lim_e;
+            w[2 * x] = ok ? OR[e[x] >> 1] : -1;
+            w[2 * x + 1] = ok ? OR[(e[x] ^ 2) >> 1] : -1;
+        }
+        bool bad = false;
+#pragma unroll
+        for (int x = 0; x < 2 * kMarkChunk; ++x)
+        {
+            const int side = x / kMarkChunk, i = i0 + x % kMarkChunk;
+            if (i < (side ? c1 : c0) && !(e[x] >= 0 && e[x] < lim_e)) bad = true;
+            if (e[x] >= 0 && e[x] < lim_e) atomicCAS(&del[e[x] >> 2], -1, t);
+        }
+        if (bad) atomicOr(err, kSeamErr);
+        int got[4 * kMarkChunk];
+#pragma unroll
+        for (int y = 0; y < 4 * kMarkChunk; ++y)
+            got[y] = (unsigned)w[y] < m ? atomicCAS(&owner[w[y]], -1, -3) : 0;
+        int want = 0;
+#pragma unroll
+        for (int y = 0; y < 4 * kMarkChunk; ++y)
+        {
+            if ((unsigned)w[y] >= m || got[y] != -1) w[y] = -1;
+            want += w[y] >= 0 ? 1 : 0;
+        }
+        int at = wave_fetch_add(alist_n, want);
+#pragma unroll
+        for (int y = 0; y < 4 * kMarkChunk; ++y)
+        {
+            if (w[y] < 0) continue;
+            if (at < acap)
+                alist[at] = w[y];
+            else
+                atomicOr(err, kSeamErr);
+            ++at;
+        }
+    }
 }
 
 // The same with one thread per (anchor, side, crossing slot), for levels of
