@@ -8,6 +8,7 @@
 #include <mutex>
 #include <string>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <utility>
 
 namespace qdt
@@ -75,7 +76,9 @@ struct Cache
         // the directory chain (a failure shows up at fopen)
         for (size_t at = p.find('/', 1); at != std::string::npos; at = p.find('/', at + 1))
             (void)mkdir(p.substr(0, at).c_str(), 0755);
-        const std::string tmp = p + ".tmp";
+        // a name of this process's own: two processes saving at once each
+        // write a whole file, and the last rename wins
+        const std::string tmp = p + ".tmp." + std::to_string((long)getpid());
         FILE* f = std::fopen(tmp.c_str(), "w");
         if (!f) return;
         for (const auto& kv : m)
