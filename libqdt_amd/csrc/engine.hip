@@ -2541,9 +2541,10 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_compact(int d, const int* s
 // Only every `spread`-th lane runs pieces: on levels whose pieces would
 // leave SIMDs idle they are spread over more wavefronts with fewer each
 // (measured on 10M uniform points: up to 1,024 wavefronts 22.47 -> 22.40 ms
-// per run, up to 2,048 or 4,096 slower).
+// per run, up to 2,048 or 4,096 slower; later, up to 512 against 1,024:
+// 21.55 -> 21.45 ms, 256 the same as 512, Gaussian points unchanged).
 constexpr int kSegPieceBlock = 256; // threads per workgroup at most
-constexpr int kPieceWaves = 1024;   // wavefronts of a spread launch at most
+constexpr int kPieceWaves = 512;    // wavefronts of a spread launch at most
 
 __global__ __launch_bounds__(kSegPieceBlock) void k_seg_pieces(int* NX, int* OR, const double2* PT, int* VE,
                                                               const int* off, int d, uint32_t m, int lim_e,
