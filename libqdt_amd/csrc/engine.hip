@@ -470,11 +470,20 @@ __global__ __launch_bounds__(kBlock) void k_rep_local(uint32_t* rep, uint32_t n,
     if (i < n) rep[i] -= soff[set_of(soff, k, i)];
 }
 
+// (total: the distinct count; when every point is distinct each one is its
+// own representative, written in order instead of through the sort's
+// permutation)
 __global__ __launch_bounds__(kBlock) void k_rep(const uint32_t* idx, const uint8_t* F, const uint32_t* dist,
-                                                uint32_t n, const uint32_t* ID0, uint32_t* rep)
+                                                uint32_t n, const uint32_t* ID0, const uint32_t* total,
+                                                uint32_t* rep)
 {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
+    if (*total == n)
+    {
+        rep[p] = p;
+        return;
+    }
     const uint32_t d = F[p] ? dist[p] : dist[p] - 1;
     rep[idx[p]] = ID0[d];
 }
@@ -4348,7 +4357,7 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
     const uint32_t nbs = cdiv(n, kScanTile);
     k_dedupe_write<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.P1, vA, b.flagp, srank, b.dist, n, b.P0, b.ID0, b.vrank,
                                                       b.Z);
-    k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.flagp, b.dist, n, b.ID0, b.rep);
+    k_rep<<<cdiv(n, kBlock), kBlock, 0, s>>>(vA, b.flagp, b.dist, n, b.ID0, b.partials + nbs, b.rep);
     std::vector<uint32_t> vstart_h;
     if (multi)
         k_vstart<<<cdiv((size_t)k + 1, kBlock), kBlock, 0, s>>>(b.dist, b.rpos, k, n, b.partials + nbs, b.vstart);
