@@ -4501,9 +4501,18 @@ Status Engine::run_impl(const double* d_pts_raw, size_t n_sz, const size_t* set_
             g = reinterpret_cast<uint32_t*>(kB);
             out = vB;
         }
-        k_rep_id<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.rep, b.Z, (uint32_t)n, w);
-        k_rep_gather<<<cdiv(n, kBlock), kBlock, 0, s>>>(perm, w, n, g);
-        scan_u32(RepLoad{ g }, n, RepWrite{ g, out }, b.partials, s);
+        if (m == n)
+        {
+            // every point distinct: each is its own representative and the
+            // compaction keeps everything
+            k_rep_gather<<<cdiv(n, kBlock), kBlock, 0, s>>>(perm, b.Z, n, out);
+        }
+        else
+        {
+            k_rep_id<<<cdiv(n, kBlock), kBlock, 0, s>>>(b.rep, b.Z, (uint32_t)n, w);
+            k_rep_gather<<<cdiv(n, kBlock), kBlock, 0, s>>>(perm, w, n, g);
+            scan_u32(RepLoad{ g }, n, RepWrite{ g, out }, b.partials, s);
+        }
         vA = out;
     }
     else
